@@ -1,0 +1,217 @@
+/*
+ * tcstencil.h -- C ABI of the MI355X (gfx950) matrix-core 2D stencil library.
+ *
+ * Four stencils (star2d1r, star2d2r, box2d1r, box2d2r) run as small dense MFMA
+ * products over a 16x16 brick layout, next to a plain direct-FMA kernel.  All
+ * signatures use plain C types: pointers, sizes and ints.  `stream` is a
+ * hipStream_t passed as void* (NULL = the default stream).
+ *
+ * Two layers:
+ *   1. Drop-in drivers with the reference's host_code signature
+ *      (layout16/host_<stencil>.cu, baseline/host_<stencil>.cu).
+ *   2. A device-pointer API returning status codes: a problem descriptor, the
+ *      brick pack/unpack transform, sweeps, hipGraph capture, FP64 oracle sweep
+ *      and error norms, seeded generators.  It keeps no global mutable state;
+ *      every call takes the caller's stream.
+ */
+#ifndef TCSTENCIL_H
+#define TCSTENCIL_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---------------------------------------------------------------- constants */
+enum { TCS_STAR2D1R = 0, TCS_STAR2D2R = 1, TCS_BOX2D1R = 2, TCS_BOX2D2R = 3 };
+enum { TCS_METHOD_MFMA = 0, TCS_METHOD_DIRECT = 1 };
+/* FP16: FP16 storage, FP16 coefficients, MFMA f16 with FP32 accumulation.
+ * BF16: the same in BF16.
+ * SPLIT: storage is a hi+lo FP16 pair (two planes) with a per-grid power-of-two
+ *        scale; coefficients are split hi+lo too; four MFMA passes
+ *        (hi*hi, hi*lo, lo*hi, lo*lo) accumulate into one FP32 accumulator. */
+enum { TCS_PREC_FP16 = 0, TCS_PREC_BF16 = 1, TCS_PREC_SPLIT = 2 };
+
+enum {
+    TCS_OK = 0,
+    TCS_ERR_INVALID = 1,     /* bad argument (null pointer, size, stencil, n...) */
+    TCS_ERR_UNSUPPORTED = 2, /* valid combination that this build does not provide */
+    TCS_ERR_HIP = 3,         /* a HIP runtime call failed */
+    TCS_ERR_ALLOC = 4        /* device or host allocation failed */
+};
+
+/* Problem parameters of the reference (common/param.hpp:1-13), same names and
+ * defaults; tcs_params_from_env() overrides each from TCS_<NAME>.  TILE_SIZE
+ * (bricks per block along x in the reference) maps to tile_w; 0 means "use the
+ * tuned default". */
+typedef struct tcs_params {
+    double TOLERANCE;  /* 0.01 */
+    double MIN_NUMBER; /* 1e-6 */
+    int32_t RUN_TIMES; /* 500 */
+    int32_t STEP_TIMES;/* 1 */
+    int32_t SKIP_TIMES;/* 400 */
+    int32_t TILE_SIZE; /* 0 (tuned) -- the reference's compile-time default is 1 */
+} tcs_params;
+
+/* Brick layout of one rows x cols grid.  Bricks are 16x16, stored brick-row-major,
+ * each brick row-major (256 elements, 512 B in FP16): the reference's
+ * brick_layout_2D map (layout16/host_box2d2r.cu:75-86).  Around the nbr x nbc
+ * data bricks sits a one-brick ghost ring (all zero), like the reference's 16-wide
+ * padding ring (layout16/host_box2d2r.cu:61-73, :136-141).  nbr/nbc are rounded up
+ * to a multiple of `align` so that tiled kernels need no bounds checks. */
+typedef struct tcs_layout {
+    int32_t rows, cols;  /* logical grid, including the Dirichlet ring of width r */
+    int32_t nbr, nbc;    /* data bricks (padded to a multiple of align) */
+    int32_t sbr, sbc;    /* stored bricks = nbr + 2, nbc + 2 */
+    int32_t align;       /* 1, 2, 4 or 8 */
+    int32_t reserved;
+    int64_t plane_elems; /* sbr * sbc * 256: elements of one storage plane */
+} tcs_layout;
+
+/* Everything a sweep needs, by value: no device allocations behind it. */
+typedef struct tcs_problem {
+    int32_t stencil, method, precision;
+    int32_t radius;          /* 1 or 2 */
+    int32_t tile_h, tile_w;  /* bricks per workgroup tile (0 = tuned default) */
+    tcs_layout layout;
+    float coeff[25];         /* 5x5 embedding, coeff[(di+2)*5 + (dj+2)], FP32 */
+    uint16_t coeff_hi[25];   /* SPLIT: FP16 bits of hi(coeff * 2^coeff_exp) */
+    uint16_t coeff_lo[25];   /* SPLIT: FP16 bits of lo(coeff * 2^coeff_exp) */
+    int32_t coeff_exp;       /* SPLIT: coefficient scale exponent */
+    int32_t reserved;
+} tcs_problem;
+
+/* ---------------------------------------------------------- status, params */
+const char *tcs_status_string(int status);
+/* Message of the last failing call on this thread ("" if none). */
+const char *tcs_last_error(void);
+void tcs_params_default(tcs_params *p);
+void tcs_params_from_env(tcs_params *p);
+
+/* ------------------------------------------------------------- drop-in layer */
+/* Replace `extern "C" void host_code(float *h_A, float *h_B, float *h_C, int N, int n)`
+ * of layout16/host_box2d2r.cu:129 (and siblings layout16/host_box2d1r.cu,
+ * layout16/host_star2d1r.cu, layout16/host_star2d2r.cu), declared by the caller at
+ * common/main_2r.cu:10 / common/main_1r.cu:7.  One Jacobi sweep of the N x N FP32
+ * row-major grid h_A with the n x n coefficients h_B (n = 2r+1, row stride n),
+ * on the MFMA FP16 path; h_C receives the interior [r,N-r)^2 and, on the ring,
+ * h_A's values (gold's convention, common/main_2r.cu:29).  Any N >= 1 (the
+ * reference needs N % 16 == 0).  The kernel is launched RUN_TIMES times on the
+ * same input and "[Time] Time used: %f ms" is printed, as the reference does
+ * (layout16/host_box2d2r.cu:177-193).  On error it prints the message and
+ * exits with status -1, like the reference's check_error (common/common.hpp:10-16). */
+void host_star2d1r(float *h_A, float *h_B, float *h_C, int N, int n);
+void host_star2d2r(float *h_A, float *h_B, float *h_C, int N, int n);
+void host_box2d1r(float *h_A, float *h_B, float *h_C, int N, int n);
+void host_box2d2r(float *h_A, float *h_B, float *h_C, int N, int n);
+
+/* The same on the direct-FMA kernel: replaces host_code of baseline/host_box2d2r.cu:91
+ * (and siblings baseline/host_{box2d1r,star2d1r,star2d2r}.cu). */
+void host_direct_star2d1r(float *h_A, float *h_B, float *h_C, int N, int n);
+void host_direct_star2d2r(float *h_A, float *h_B, float *h_C, int N, int n);
+void host_direct_box2d1r(float *h_A, float *h_B, float *h_C, int N, int n);
+void host_direct_box2d2r(float *h_A, float *h_B, float *h_C, int N, int n);
+
+/* Status-returning generalisation of the drivers: rows x cols, any method and
+ * precision, `sweeps` ping-pong sweeps, no printing, no exit.  The per-sweep
+ * kernel time (mean over RUN_TIMES-SKIP_TIMES timed repetitions of the whole
+ * sweep loop, divided by sweeps) goes to *ms_per_sweep when non-NULL. */
+int tcs_host_sweep(int stencil, int method, int precision, const float *h_A, const float *h_B,
+                   int n, float *h_C, int rows, int cols, int sweeps, const tcs_params *params,
+                   float *ms_per_sweep);
+
+/* ------------------------------------------------------------- problem setup */
+int tcs_layout_init(tcs_layout *L, int rows, int cols, int align);
+/* Builds the descriptor from the n x n coefficient block B (row stride n,
+ * n = 2r+1 of the stencil).  Star stencils use only B's centre row and column,
+ * like gold. */
+int tcs_problem_init(tcs_problem *p, int stencil, int method, int precision, int rows, int cols,
+                     const float *B, int n);
+int tcs_radius(int stencil);
+/* Choose the workgroup tile (bricks, rows x cols; offered: 2x4, 4x4, 2x8, 4x8,
+ * 8x8, SPLIT without 8x8).  Re-derives the layout's alignment, so buffers
+ * packed for another tile must be packed again. */
+int tcs_problem_set_tile(tcs_problem *p, int tile_h, int tile_w);
+/* Bytes of one storage buffer for this problem (2 B/elem; SPLIT: two planes). */
+int64_t tcs_buffer_bytes(const tcs_problem *p);
+
+/* --------------------------------------------------------- device transforms */
+/* Row-major source (FP32 or FP64, leading dimension ld elements) -> brick storage
+ * in the problem's precision, ghost ring and padding zeroed.  scale_exp: the
+ * source is multiplied by 2^scale_exp (SPLIT's per-grid scaling; 0 otherwise). */
+int tcs_pack_f32(const tcs_problem *p, const float *d_src, int64_t ld, void *d_dst, int scale_exp,
+                 void *stream);
+int tcs_pack_f64(const tcs_problem *p, const double *d_src, int64_t ld, void *d_dst, int scale_exp,
+                 void *stream);
+/* Brick storage -> row-major FP32 (times 2^-scale_exp).  When d_ring is non-NULL
+ * (row-major FP32 with the same ld), points outside the interior are copied from
+ * it instead, so the ring keeps the caller's exact input values. */
+int tcs_unpack_f32(const tcs_problem *p, const void *d_src, float *d_dst, int64_t ld,
+                   int scale_exp, const float *d_ring, void *stream);
+/* Zero a whole storage buffer (a fill kernel, safe under graph capture). */
+int tcs_clear(const tcs_problem *p, void *d_buf, void *stream);
+/* max |x| over a row-major FP32/FP64 grid into *d_out (one double, device). */
+int tcs_maxabs_f32(const float *d_src, int rows, int cols, int64_t ld, double *d_out, void *stream);
+int tcs_maxabs_f64(const double *d_src, int rows, int cols, int64_t ld, double *d_out, void *stream);
+/* Per-grid scale exponent for SPLIT storage given max|x| (host helper). */
+int tcs_split_scale_exp(double maxabs);
+
+/* ----------------------------------------------------------------- sweeping */
+/* One sweep d_in -> d_out (distinct storage buffers of this problem; d_out's
+ * ghost ring must be zero, e.g. from tcs_clear or a previous pack). */
+int tcs_sweep(const tcs_problem *p, const void *d_in, void *d_out, void *stream);
+/* `sweeps` ping-pong sweeps between d_buf0 and d_buf1: sweep s reads
+ * buf[s % 2] and writes buf[(s+1) % 2], so the result is in buf[sweeps % 2]. */
+int tcs_run(const tcs_problem *p, void *d_buf0, void *d_buf1, int sweeps, void *stream);
+/* Capture tcs_run into a hipGraphExec (returned as void*) and replay it. */
+int tcs_graph_create(const tcs_problem *p, void *d_buf0, void *d_buf1, int sweeps, void *stream,
+                     void **graph_exec);
+int tcs_graph_launch(void *graph_exec, void *stream);
+int tcs_graph_destroy(void *graph_exec);
+
+/* ------------------------------------------------ FP64 oracle and error norms */
+/* One FP64 sweep on row-major grids (ld = cols), gold's loop and accumulation
+ * order with no fused multiply-add (common/gold_*.cpp); the ring of d_out is not
+ * written.  Used only as the accuracy oracle, never as a product sweep. */
+int tcs_oracle_sweep_f64(int stencil, const double *d_in, const double *B, int n, double *d_out,
+                         int rows, int cols, void *stream);
+/* Norms over the interior [r,rows-r) x [r,cols-r) of row-major FP32 output vs
+ * FP64 reference, reduced in FP64 (deterministic two-pass):
+ * out[0]=max|o-ref|, out[1]=||o-ref||_2/||ref||_2, out[2]=RMS abs error
+ * (checkError2D, common/common.hpp:38-64), out[3]=||ref||_2.  Host result. */
+int tcs_error_norms(const float *d_out, const double *d_ref, int rows, int cols, int r,
+                    double *h_out4, void *stream);
+
+/* ------------------------------------------------------------- generators */
+/* Counter-based (SplitMix64) fills, identical on every launch shape.
+ * uniform: element (i,j) = (T)(lo + (hi-lo) * u), u = top 24 bits of
+ *          mix64(key + (i*cols+j)*gamma) / 2^24, key = mix64(seed ^ stream*C).
+ * sinusoid: sum of 4 sinusoids (integer wavenumbers 1..64 per axis, random
+ *          phase and amplitude, drawn from seed) + uniform noise in
+ *          [-noise, noise). */
+int tcs_gen_uniform_f32(float *d_dst, int rows, int cols, uint64_t seed, uint64_t stream_id,
+                        float lo, float hi, void *stream);
+int tcs_gen_uniform_f64(double *d_dst, int rows, int cols, uint64_t seed, uint64_t stream_id,
+                        double lo, double hi, void *stream);
+int tcs_gen_sinusoid_f64(double *d_dst, int rows, int cols, uint64_t seed, double noise,
+                         void *stream);
+/* Round a row-major FP64/FP32 grid in place to FP16 (or BF16) values, kept in
+ * the wider type, so that the FP64 oracle sees exactly the FP16 input. */
+int tcs_round_f64(double *d_buf, int64_t count, int precision, void *stream);
+int tcs_round_f32(float *d_buf, int64_t count, int precision, void *stream);
+/* Host: coefficients U[0,1) normalised to sum 1 (in FP64, then rounded to FP32)
+ * into the n x n block B (star stencils: only centre row and column non-zero). */
+int tcs_gen_coeffs(int stencil, uint64_t seed, float *B, int n);
+
+/* ------------------------------------------------------------ introspection */
+/* Kernel name (as rocprofv3 reports it) used for this problem, or NULL. */
+const char *tcs_kernel_name(const tcs_problem *p);
+/* MFMA instructions issued per 16x16 brick (0 for the direct method). */
+int tcs_mfma_per_brick(const tcs_problem *p);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* TCSTENCIL_H */

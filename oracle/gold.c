@@ -1,0 +1,185 @@
+/*
+ * CPU oracle for the 2D stencil path -- TEST INFRASTRUCTURE ONLY (see gold.h).
+ *
+ * Restates the reference's gold loops (common/gold_star2d1r.cpp:4-14,
+ * common/gold_star2d2r.cpp:4-16, common/gold_box2d1r.cpp:3-15,
+ * common/gold_box2d2r.cpp:3-15) in plain C with the same loop nest, the same
+ * coefficient indexing (no kernel flip) and the same left-to-right accumulation
+ * order.  Compile with -ffp-contract=off.
+ */
+#include "gold.h"
+
+#include <math.h>
+#include <stddef.h>
+
+int oracle_radius(int stencil) {
+    switch (stencil) {
+    case ORACLE_STAR2D1R: case ORACLE_BOX2D1R: return 1;
+    case ORACLE_STAR2D2R: case ORACLE_BOX2D2R: return 2;
+    default: return -1;
+    }
+}
+
+/* The four loops are written once per element type with a macro so that the
+ * FP32 and FP64 twins cannot drift apart.  Each body follows its reference
+ * file line by line:
+ *   star r=1: common/gold_star2d1r.cpp:10-12  (N, W... order: up, centre, down, left, right)
+ *   star r=2: common/gold_star2d2r.cpp:10-14  (up2, up1, centre, down1, down2, left2, left1, right1, right2)
+ *   box:      common/gold_box2d1r.cpp:9-13 / gold_box2d2r.cpp:9-13
+ *             (C = 0; for k in 0..n*n-1: C += A[i-r+k/n][j-r+k%n] * B[k/n][k%n])
+ */
+#define DEFINE_SWEEP(NAME, T)                                                           \
+    int NAME(int stencil, const T *A, const T *B, T *C, int rows, int cols) {           \
+        const ptrdiff_t ld = cols;                                                      \
+        if (stencil == ORACLE_STAR2D1R) {                                               \
+            /* B is 3x3, row stride 3 */                                                \
+            for (int i = 1; i < rows - 1; i++)                                          \
+                for (int j = 1; j < cols - 1; j++)                                      \
+                    C[i * ld + j] = A[(i - 1) * ld + j] * B[0 * 3 + 1] +                \
+                                    A[i * ld + j] * B[1 * 3 + 1] +                      \
+                                    A[(i + 1) * ld + j] * B[2 * 3 + 1] +                \
+                                    A[i * ld + j - 1] * B[1 * 3 + 0] +                  \
+                                    A[i * ld + j + 1] * B[1 * 3 + 2];                   \
+            return 0;                                                                   \
+        }                                                                               \
+        if (stencil == ORACLE_STAR2D2R) {                                               \
+            /* B is 5x5, row stride 5 */                                                \
+            for (int i = 2; i < rows - 2; i++)                                          \
+                for (int j = 2; j < cols - 2; j++)                                      \
+                    C[i * ld + j] = A[(i - 2) * ld + j] * B[0 * 5 + 2] +                \
+                                    A[(i - 1) * ld + j] * B[1 * 5 + 2] +                \
+                                    A[i * ld + j] * B[2 * 5 + 2] +                      \
+                                    A[(i + 1) * ld + j] * B[3 * 5 + 2] +                \
+                                    A[(i + 2) * ld + j] * B[4 * 5 + 2] +                \
+                                    A[i * ld + j - 2] * B[2 * 5 + 0] +                  \
+                                    A[i * ld + j - 1] * B[2 * 5 + 1] +                  \
+                                    A[i * ld + j + 1] * B[2 * 5 + 3] +                  \
+                                    A[i * ld + j + 2] * B[2 * 5 + 4];                   \
+            return 0;                                                                   \
+        }                                                                               \
+        if (stencil == ORACLE_BOX2D1R || stencil == ORACLE_BOX2D2R) {                   \
+            const int r = (stencil == ORACLE_BOX2D1R) ? 1 : 2;                          \
+            const int n = 2 * r + 1;                                                    \
+            for (int i = r; i < rows - r; i++)                                          \
+                for (int j = r; j < cols - r; j++) {                                    \
+                    T acc = 0;                                                          \
+                    for (int k = 0; k < n * n; k++)                                     \
+                        acc += A[(i - r + k / n) * ld + (j - r + k % n)] *              \
+                               B[(k / n) * n + (k % n)];                                \
+                    C[i * ld + j] = acc;                                                \
+                }                                                                       \
+            return 0;                                                                   \
+        }                                                                               \
+        return -1;                                                                      \
+    }
+
+DEFINE_SWEEP(oracle_sweep_f32, float)
+DEFINE_SWEEP(oracle_sweep_f64, double)
+
+/* The box loop above accumulates into a local and stores once; the reference
+ * accumulates into C[i][j] itself (C[i][j] = 0; C[i][j] += ...).  With
+ * -ffp-contract=off both evaluate the identical sequence of rounded additions
+ * starting from +0, so the stored value is bitwise the same. */
+
+/* Reference-signature wrappers (square N x N, `n` ignored like the reference). */
+void gold_star2d1r(float *h_A, float *h_B, float *h_C, int N, int n) { (void)n; oracle_sweep_f32(ORACLE_STAR2D1R, h_A, h_B, h_C, N, N); }
+void gold_star2d2r(float *h_A, float *h_B, float *h_C, int N, int n) { (void)n; oracle_sweep_f32(ORACLE_STAR2D2R, h_A, h_B, h_C, N, N); }
+void gold_box2d1r(float *h_A, float *h_B, float *h_C, int N, int n) { (void)n; oracle_sweep_f32(ORACLE_BOX2D1R, h_A, h_B, h_C, N, N); }
+void gold_box2d2r(float *h_A, float *h_B, float *h_C, int N, int n) { (void)n; oracle_sweep_f32(ORACLE_BOX2D2R, h_A, h_B, h_C, N, N); }
+
+void gold_star2d1r_f64(double *h_A, double *h_B, double *h_C, int N, int n) { (void)n; oracle_sweep_f64(ORACLE_STAR2D1R, h_A, h_B, h_C, N, N); }
+void gold_star2d2r_f64(double *h_A, double *h_B, double *h_C, int N, int n) { (void)n; oracle_sweep_f64(ORACLE_STAR2D2R, h_A, h_B, h_C, N, N); }
+void gold_box2d1r_f64(double *h_A, double *h_B, double *h_C, int N, int n) { (void)n; oracle_sweep_f64(ORACLE_BOX2D1R, h_A, h_B, h_C, N, N); }
+void gold_box2d2r_f64(double *h_A, double *h_B, double *h_C, int N, int n) { (void)n; oracle_sweep_f64(ORACLE_BOX2D2R, h_A, h_B, h_C, N, N); }
+
+int oracle_sweeps_f64(int stencil, double *A, const double *B, double *tmp, int rows, int cols, int sweeps) {
+    const ptrdiff_t total = (ptrdiff_t)rows * cols;
+    /* The output buffer is seeded from the input so the ring keeps its values,
+     * as the reference's caller does (common/main_2r.cu:29). */
+    for (ptrdiff_t q = 0; q < total; q++) tmp[q] = A[q];
+    double *src = A, *dst = tmp;
+    for (int s = 0; s < sweeps; s++) {
+        if (oracle_sweep_f64(stencil, src, B, dst, rows, cols) != 0) return -1;
+        double *t = src; src = dst; dst = t;
+    }
+    if (src != A)
+        for (ptrdiff_t q = 0; q < total; q++) A[q] = src[q];
+    return 0;
+}
+
+void oracle_error_norms(const float *o, const double *ref, int rows, int cols, int r, double *out) {
+    double maxabs = 0.0, se = 0.0, sr = 0.0;
+    long long cnt = 0;
+    for (int i = r; i < rows - r; i++)
+        for (int j = r; j < cols - r; j++) {
+            const double rv = ref[(ptrdiff_t)i * cols + j];
+            const double d = (double)o[(ptrdiff_t)i * cols + j] - rv;
+            const double ad = fabs(d);
+            if (ad > maxabs || ad != ad) maxabs = ad;
+            se += d * d;
+            sr += rv * rv;
+            cnt++;
+        }
+    out[0] = maxabs;
+    out[1] = sr > 0.0 ? sqrt(se) / sqrt(sr) : sqrt(se);
+    out[2] = cnt > 0 ? sqrt(se / (double)cnt) : 0.0;
+    out[3] = sqrt(sr);
+}
+
+/* common/common.hpp:38-64: RMS of |output-reference| over [y_lb,y_ub)x[x_lb,x_ub). */
+double oracle_check_error_2d(int N, const float *output, const float *reference,
+                             int y_lb, int y_ub, int x_lb, int x_ub) {
+    double error = 0.0;
+    for (int j = y_lb; j < y_ub; j++)
+        for (int k = x_lb; k < x_ub; k++) {
+            double e = (double)output[(ptrdiff_t)j * N + k] - (double)reference[(ptrdiff_t)j * N + k];
+            e = e < 0.0 ? -e : e;
+            error += e * e;
+        }
+    return sqrt(error / ((double)(y_ub - y_lb) * (double)(x_ub - x_lb)));
+}
+
+/* common/common.hpp:66-92: RMS of the pointwise relative error, with reference
+ * magnitudes below MIN_NUMBER (1e-6) replaced by MIN_NUMBER.  The reference
+ * writes `abs(reference[j][k])` (:78); which overload that picks depends on the
+ * CUDA host headers in scope, so the evident intent -- the float magnitude --
+ * is restated here. */
+double oracle_check_relative_error_2d(int N, const float *output, const float *reference,
+                                      int y_lb, int y_ub, int x_lb, int x_ub) {
+    const double min_number = 1e-6;
+    double error = 0.0;
+    for (int j = y_lb; j < y_ub; j++)
+        for (int k = x_lb; k < x_ub; k++) {
+            const float rv = reference[(ptrdiff_t)j * N + k];
+            double e = (double)output[(ptrdiff_t)j * N + k] - (double)rv;
+            e = (fabs((double)rv) < min_number) ? e / min_number : e / (double)rv;
+            error += e * e;
+        }
+    return sqrt(error / ((double)(y_ub - y_lb) * (double)(x_ub - x_lb)));
+}
+
+/* SplitMix64 (Steele, Lea, Flood 2014; public-domain C by S. Vigna):
+ * output k of a stream seeded with x is mix64(x + (k-1)*gamma). */
+uint64_t oracle_mix64(uint64_t z) {
+    z += 0x9E3779B97F4A7C15ull;
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    return z ^ (z >> 31);
+}
+
+/* Counter-based uniform generator shared with the product (restated here so the
+ * GPU generator can be checked bitwise): element idx = i*cols+j of stream `stream`
+ * under `seed` draws u = top 24 bits of mix64(key + idx*gamma) scaled by 2^-24,
+ * and returns (float)(lo + (hi-lo)*u) with the arithmetic in double. */
+void oracle_gen_uniform_f32(float *dst, int rows, int cols, uint64_t seed, uint64_t stream,
+                            float lo, float hi) {
+    const uint64_t key = oracle_mix64(seed ^ (stream * 0xD1B54A32D192ED03ull));
+    const double span = (double)hi - (double)lo;
+    for (int i = 0; i < rows; i++)
+        for (int j = 0; j < cols; j++) {
+            const uint64_t idx = (uint64_t)i * (uint64_t)cols + (uint64_t)j;
+            const uint64_t bits = oracle_mix64(key + idx * 0x9E3779B97F4A7C15ull) >> 40;
+            const double u = (double)bits * (1.0 / 16777216.0);
+            dst[(ptrdiff_t)i * cols + j] = (float)((double)lo + span * u);
+        }
+}

@@ -1,0 +1,268 @@
+"""Python front end of the device-pointer API (include/tcstencil.h).
+
+PyTorch supplies device memory and streams only; every computation is a call
+into libtcstencil.so.  Device buffers are torch tensors on ``cuda``; streams
+default to torch's current stream, so ``torch.cuda.Event`` timing brackets
+exactly the kernels launched here.
+"""
+from __future__ import annotations
+
+import ctypes
+
+import numpy as np
+
+from . import _lib as L
+from ._lib import check, lib
+
+
+def _torch():
+    import torch  # imported lazily: the host drivers need no torch
+    return torch
+
+
+def _stream(stream=None):
+    torch = _torch()
+    s = stream if stream is not None else torch.cuda.current_stream()
+    return ctypes.c_void_p(s.cuda_stream)
+
+
+def _ptr(t) -> ctypes.c_void_p:
+    if not t.is_cuda:
+        raise ValueError("expected a device tensor")
+    if not t.is_contiguous():
+        raise ValueError("expected a contiguous tensor")
+    return ctypes.c_void_p(t.data_ptr())
+
+
+def _fp(a: np.ndarray):
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_float))
+
+
+def stencil_id(s) -> int:
+    return L.STENCIL_ID[s] if isinstance(s, str) else int(s)
+
+
+def radius(stencil) -> int:
+    return lib().tcs_radius(stencil_id(stencil))
+
+
+class Problem:
+    """A tcs_problem descriptor: stencil, method, precision, grid, coefficients."""
+
+    def __init__(self, stencil, rows: int, cols: int, B, method="mfma", precision="fp16",
+                 tile=None):
+        self.stencil = stencil_id(stencil)
+        self.method = L.METHOD_ID[method] if isinstance(method, str) else int(method)
+        self.precision = L.PREC_ID[precision] if isinstance(precision, str) else int(precision)
+        self.B = np.ascontiguousarray(B, dtype=np.float32)
+        n = self.B.shape[0]
+        self.s = L.ProblemStruct()
+        check(lib().tcs_problem_init(ctypes.byref(self.s), self.stencil, self.method,
+                                     self.precision, rows, cols, _fp(self.B), n))
+        if tile is not None:
+            self.set_tile(*tile)
+
+    # -- descriptor ---------------------------------------------------------
+    def set_tile(self, tile_h: int, tile_w: int) -> None:
+        check(lib().tcs_problem_set_tile(ctypes.byref(self.s), tile_h, tile_w))
+
+    @property
+    def ref(self):
+        return ctypes.byref(self.s)
+
+    @property
+    def rows(self) -> int:
+        return self.s.layout.rows
+
+    @property
+    def cols(self) -> int:
+        return self.s.layout.cols
+
+    @property
+    def radius(self) -> int:
+        return self.s.radius
+
+    @property
+    def tile(self):
+        return (self.s.tile_h, self.s.tile_w)
+
+    @property
+    def buffer_bytes(self) -> int:
+        return int(lib().tcs_buffer_bytes(self.ref))
+
+    @property
+    def kernel_name(self) -> str:
+        n = lib().tcs_kernel_name(self.ref)
+        return n.decode() if n else ""
+
+    @property
+    def mfma_per_brick(self) -> int:
+        return int(lib().tcs_mfma_per_brick(self.ref))
+
+    @property
+    def interior_points(self) -> int:
+        r = self.radius
+        return max(self.rows - 2 * r, 0) * max(self.cols - 2 * r, 0)
+
+    # -- device buffers -----------------------------------------------------
+    def alloc(self, device="cuda"):
+        torch = _torch()
+        return torch.empty(self.buffer_bytes, dtype=torch.uint8, device=device)
+
+    def clear(self, buf, stream=None) -> None:
+        check(lib().tcs_clear(self.ref, _ptr(buf), _stream(stream)))
+
+    def pack(self, src, buf, scale_exp: int = 0, stream=None) -> None:
+        """Row-major FP32/FP64 device grid -> brick storage."""
+        torch = _torch()
+        if src.dim() != 2 or src.shape[0] != self.rows or src.shape[1] != self.cols:
+            raise ValueError(f"source shape {tuple(src.shape)} != grid {self.rows}x{self.cols}")
+        if buf.numel() < self.buffer_bytes:
+            raise ValueError("storage buffer too small")
+        fn = {torch.float32: lib().tcs_pack_f32, torch.float64: lib().tcs_pack_f64}.get(src.dtype)
+        if fn is None:
+            raise TypeError("pack: source must be float32 or float64")
+        check(fn(self.ref, _ptr(src), src.stride(0), _ptr(buf), scale_exp, _stream(stream)))
+
+    def unpack(self, buf, out=None, scale_exp: int = 0, ring=None, stream=None):
+        """Brick storage -> row-major FP32 device grid (ring copied from `ring`)."""
+        torch = _torch()
+        if out is None:
+            out = torch.empty((self.rows, self.cols), dtype=torch.float32, device=buf.device)
+        ringp = _ptr(ring) if ring is not None else None
+        check(lib().tcs_unpack_f32(self.ref, _ptr(buf), _ptr(out), out.stride(0), scale_exp,
+                                   ringp, _stream(stream)))
+        return out
+
+    # -- sweeps -------------------------------------------------------------
+    def sweep(self, src, dst, stream=None) -> None:
+        check(lib().tcs_sweep(self.ref, _ptr(src), _ptr(dst), _stream(stream)))
+
+    def run(self, buf0, buf1, sweeps: int, stream=None):
+        """Ping-pong sweeps; returns the buffer holding the result."""
+        check(lib().tcs_run(self.ref, _ptr(buf0), _ptr(buf1), sweeps, _stream(stream)))
+        return buf1 if sweeps % 2 else buf0
+
+    def graph(self, buf0, buf1, sweeps: int, stream=None) -> "Graph":
+        h = ctypes.c_void_p()
+        check(lib().tcs_graph_create(self.ref, _ptr(buf0), _ptr(buf1), sweeps, _stream(stream),
+                                     ctypes.byref(h)))
+        return Graph(h)
+
+
+class Graph:
+    """A captured sweep loop (hipGraphExec)."""
+
+    def __init__(self, handle: ctypes.c_void_p):
+        self.h = handle
+
+    def launch(self, stream=None) -> None:
+        check(lib().tcs_graph_launch(self.h, _stream(stream)))
+
+    def destroy(self) -> None:
+        if self.h:
+            check(lib().tcs_graph_destroy(self.h))
+            self.h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.destroy()
+        except Exception:
+            pass
+
+
+# ---------------------------------------------------------------- utilities
+def oracle_sweep_f64(stencil, src, B, dst, stream=None) -> None:
+    """One FP64 sweep (gold's order, no FMA) on row-major device grids."""
+    B = np.ascontiguousarray(B, dtype=np.float64)
+    Bp = B.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+    check(lib().tcs_oracle_sweep_f64(stencil_id(stencil), _ptr(src), Bp, B.shape[0], _ptr(dst),
+                                     src.shape[0], src.shape[1], _stream(stream)))
+
+
+def oracle_sweeps_f64(stencil, x64, B, sweeps: int, stream=None):
+    """`sweeps` FP64 sweeps with ping-pong; returns the result tensor."""
+    a = x64.clone()
+    b = x64.clone()  # the ring keeps the input values in both buffers
+    for _ in range(sweeps):
+        oracle_sweep_f64(stencil, a, B, b, stream)
+        a, b = b, a
+    return a
+
+
+def error_norms(out32, ref64, r: int, stream=None) -> dict:
+    res = np.zeros(4, dtype=np.float64)
+    check(lib().tcs_error_norms(_ptr(out32), _ptr(ref64), out32.shape[0], out32.shape[1], r,
+                                res.ctypes.data_as(ctypes.POINTER(ctypes.c_double)), _stream(stream)))
+    return {"max_abs": float(res[0]), "rel_l2": float(res[1]), "rms_abs": float(res[2]),
+            "ref_l2": float(res[3])}
+
+
+def maxabs(x, stream=None) -> float:
+    torch = _torch()
+    d = torch.zeros(1, dtype=torch.float64, device=x.device)
+    fn = lib().tcs_maxabs_f64 if x.dtype == torch.float64 else lib().tcs_maxabs_f32
+    check(fn(_ptr(x), x.shape[0], x.shape[1], x.stride(0), _ptr(d), _stream(stream)))
+    return float(d.item())
+
+
+def split_scale_exp(mx: float) -> int:
+    return int(lib().tcs_split_scale_exp(mx))
+
+
+def gen_uniform(rows: int, cols: int, seed: int, stream_id: int = 0, lo=-1.0, hi=1.0,
+                dtype=None, device="cuda", stream=None):
+    torch = _torch()
+    dtype = dtype or torch.float32
+    out = torch.empty((rows, cols), dtype=dtype, device=device)
+    fn = lib().tcs_gen_uniform_f64 if dtype == torch.float64 else lib().tcs_gen_uniform_f32
+    check(fn(_ptr(out), rows, cols, seed, stream_id, lo, hi, _stream(stream)))
+    return out
+
+
+def gen_sinusoid(rows: int, cols: int, seed: int, noise: float = 1e-3, device="cuda", stream=None):
+    torch = _torch()
+    out = torch.empty((rows, cols), dtype=torch.float64, device=device)
+    check(lib().tcs_gen_sinusoid_f64(_ptr(out), rows, cols, seed, noise, _stream(stream)))
+    return out
+
+
+def round_to(x, precision="fp16", stream=None):
+    """Round a float32/float64 device grid in place to FP16/BF16 values."""
+    torch = _torch()
+    p = L.PREC_ID[precision] if isinstance(precision, str) else int(precision)
+    fn = lib().tcs_round_f64 if x.dtype == torch.float64 else lib().tcs_round_f32
+    check(fn(_ptr(x), x.numel(), p, _stream(stream)))
+    return x
+
+
+def gen_coeffs(stencil, seed: int) -> np.ndarray:
+    n = 2 * radius(stencil) + 1
+    B = np.zeros((n, n), dtype=np.float32)
+    check(lib().tcs_gen_coeffs(stencil_id(stencil), seed, _fp(B), n))
+    return B
+
+
+def params(**overrides) -> L.Params:
+    """The param.hpp parameters (defaults, then TCS_* environment, then overrides)."""
+    p = L.Params()
+    lib().tcs_params_from_env(ctypes.byref(p))
+    for k, v in overrides.items():
+        setattr(p, k, v)
+    return p
+
+
+def host_sweep(stencil, A: np.ndarray, B: np.ndarray, sweeps: int = 1, method="mfma",
+               precision="fp16", prm: L.Params | None = None):
+    """Status-returning host driver (tcs_host_sweep): host FP32 in, host FP32 out.
+    Returns (C, ms_per_sweep)."""
+    A = np.ascontiguousarray(A, dtype=np.float32)
+    B = np.ascontiguousarray(B, dtype=np.float32)
+    C = np.empty_like(A)
+    ms = ctypes.c_float(0.0)
+    m = L.METHOD_ID[method] if isinstance(method, str) else int(method)
+    p = L.PREC_ID[precision] if isinstance(precision, str) else int(precision)
+    prm = prm if prm is not None else params(RUN_TIMES=1, SKIP_TIMES=0)
+    check(lib().tcs_host_sweep(stencil_id(stencil), m, p, _fp(A), _fp(B), B.shape[0], _fp(C),
+                               A.shape[0], A.shape[1], sweeps, ctypes.byref(prm), ctypes.byref(ms)))
+    return C, ms.value

@@ -1,0 +1,414 @@
+// Host side of the C ABI: status reporting, parameters, problem set-up, sweep
+// loops, hipGraph capture and the drop-in host_<stencil> drivers that replace
+// host_code of layout16/host_<stencil>.cu and baseline/host_<stencil>.cu.
+#include <math.h>
+#include <stdarg.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+
+#include "tcs_internal.hpp"
+
+namespace tcs {
+
+static thread_local char g_err[512];
+
+int set_error(int status, const char *fmt, ...) {
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(g_err, sizeof g_err, fmt, ap);
+    va_end(ap);
+    return status;
+}
+
+int hip_error(hipError_t e, const char *what) {
+    return set_error(e == hipErrorOutOfMemory ? TCS_ERR_ALLOC : TCS_ERR_HIP, "%s: %s", what, hipGetErrorString(e));
+}
+
+// float -> IEEE binary16 bits, round to nearest even (overflow to inf, NaN kept quiet).
+uint16_t f32_to_f16_bits(float f) {
+    uint32_t x;
+    memcpy(&x, &f, 4);
+    const uint16_t sign = (uint16_t)((x >> 16) & 0x8000u);
+    const uint32_t ax = x & 0x7fffffffu;
+    if (ax > 0x7f800000u) return sign | 0x7e00u;
+    if (ax >= 0x477ff000u) return sign | 0x7c00u;  // |f| >= 65520 rounds to inf
+    if (ax >= 0x38800000u) {                       // normal binary16 range
+        const uint32_t m = ax - 0x38000000u;       // rebias the exponent by 127 - 15
+        return sign | (uint16_t)((m + 0x0fffu + ((m >> 13) & 1u)) >> 13);
+    }
+    if (ax < 0x33000000u) return sign;  // below 2^-25 (2^-25 itself ties to even = 0)
+    const uint32_t e = ax >> 23;
+    const uint32_t mant = (ax & 0x7fffffu) | 0x800000u;
+    const uint32_t shift = 126u - e;  // value / 2^-24 = mant >> shift
+    uint32_t q = mant >> shift;
+    const uint32_t rem = mant & ((1u << shift) - 1u), half = 1u << (shift - 1u);
+    if (rem > half || (rem == half && (q & 1u))) ++q;
+    return sign | (uint16_t)q;
+}
+
+float f16_bits_to_f32(uint16_t h) {
+    const uint32_t sign = (uint32_t)(h & 0x8000u) << 16;
+    const uint32_t e = (h >> 10) & 0x1fu, m = h & 0x3ffu;
+    uint32_t x;
+    if (e == 0) {
+        const float v = ldexpf((float)m, -24);
+        memcpy(&x, &v, 4);
+        x |= sign;
+    } else if (e == 31) {
+        x = sign | 0x7f800000u | (m << 13);
+    } else {
+        x = sign | ((e + 112u) << 23) | (m << 13);
+    }
+    float f;
+    memcpy(&f, &x, 4);
+    return f;
+}
+
+}  // namespace tcs
+
+using namespace tcs;
+
+// ------------------------------------------------------------------ status
+extern "C" const char *tcs_status_string(int status) {
+    switch (status) {
+    case TCS_OK: return "ok";
+    case TCS_ERR_INVALID: return "invalid argument";
+    case TCS_ERR_UNSUPPORTED: return "unsupported";
+    case TCS_ERR_HIP: return "HIP runtime error";
+    case TCS_ERR_ALLOC: return "allocation failed";
+    }
+    return "unknown status";
+}
+
+extern "C" const char *tcs_last_error(void) { return g_err; }
+
+// ------------------------------------------------------------------ params
+extern "C" void tcs_params_default(tcs_params *p) {
+    if (!p) return;
+    p->TOLERANCE = 0.01;
+    p->MIN_NUMBER = 1e-6;
+    p->RUN_TIMES = 500;
+    p->STEP_TIMES = 1;
+    p->SKIP_TIMES = 400;
+    p->TILE_SIZE = 0;
+}
+
+extern "C" void tcs_params_from_env(tcs_params *p) {
+    if (!p) return;
+    tcs_params_default(p);
+    const char *v;
+    if ((v = getenv("TCS_TOLERANCE"))) p->TOLERANCE = atof(v);
+    if ((v = getenv("TCS_MIN_NUMBER"))) p->MIN_NUMBER = atof(v);
+    if ((v = getenv("TCS_RUN_TIMES"))) p->RUN_TIMES = atoi(v);
+    if ((v = getenv("TCS_STEP_TIMES"))) p->STEP_TIMES = atoi(v);
+    if ((v = getenv("TCS_SKIP_TIMES"))) p->SKIP_TIMES = atoi(v);
+    if ((v = getenv("TCS_TILE_SIZE"))) p->TILE_SIZE = atoi(v);
+}
+
+// ------------------------------------------------------------ problem set-up
+extern "C" int tcs_radius(int stencil) {
+    switch (stencil) {
+    case TCS_STAR2D1R: case TCS_BOX2D1R: return 1;
+    case TCS_STAR2D2R: case TCS_BOX2D2R: return 2;
+    }
+    return -1;
+}
+
+extern "C" int tcs_layout_init(tcs_layout *L, int rows, int cols, int align) {
+    if (!L) return set_error(TCS_ERR_INVALID, "layout: null pointer");
+    if (rows < 1 || cols < 1) return set_error(TCS_ERR_INVALID, "layout: grid %dx%d is empty", rows, cols);
+    if (align != 1 && align != 2 && align != 4 && align != 8)
+        return set_error(TCS_ERR_INVALID, "layout: align %d not in {1,2,4,8}", align);
+    if (rows > (1 << 28) || cols > (1 << 28)) return set_error(TCS_ERR_INVALID, "layout: grid too large");
+    memset(L, 0, sizeof *L);
+    L->rows = rows;
+    L->cols = cols;
+    L->align = align;
+    L->nbr = ((rows + 15) / 16 + align - 1) / align * align;
+    L->nbc = ((cols + 15) / 16 + align - 1) / align * align;
+    L->sbr = L->nbr + 2;
+    L->sbc = L->nbc + 2;
+    L->plane_elems = (int64_t)L->sbr * L->sbc * 256;
+    return TCS_OK;
+}
+
+static bool tile_offered(int th, int tw) {
+    return (th == 2 && tw == 4) || (th == 4 && tw == 4) || (th == 2 && tw == 8) || (th == 4 && tw == 8) ||
+           (th == 8 && tw == 8);
+}
+
+// Tuned default tile per grid size (bricks per workgroup, rows x cols).
+static void default_tile(int rows, int cols, int *th, int *tw) {
+    const int br = (rows + 15) / 16, bc = (cols + 15) / 16;
+    if (br >= 4 && bc >= 8) {
+        *th = 4;
+        *tw = 8;
+    } else {
+        *th = 2;
+        *tw = 4;
+    }
+}
+
+extern "C" int tcs_problem_set_tile(tcs_problem *p, int tile_h, int tile_w) {
+    if (!p) return set_error(TCS_ERR_INVALID, "set_tile: null problem");
+    if (!tile_offered(tile_h, tile_w))
+        return set_error(TCS_ERR_UNSUPPORTED, "tile %dx%d is not offered", tile_h, tile_w);
+    if (p->precision == TCS_PREC_SPLIT && tile_h == 8 && tile_w == 8)
+        return set_error(TCS_ERR_UNSUPPORTED, "SPLIT has no 8x8 tile (LDS)");
+    const int align = tile_h > tile_w ? tile_h : tile_w;
+    int st = tcs_layout_init(&p->layout, p->layout.rows, p->layout.cols, align);
+    if (st) return st;
+    p->tile_h = tile_h;
+    p->tile_w = tile_w;
+    return TCS_OK;
+}
+
+extern "C" int tcs_problem_init(tcs_problem *p, int stencil, int method, int precision, int rows, int cols,
+                                const float *B, int n) {
+    if (!p) return set_error(TCS_ERR_INVALID, "problem: null pointer");
+    const int r = tcs_radius(stencil);
+    if (r < 0) return set_error(TCS_ERR_INVALID, "problem: unknown stencil %d", stencil);
+    if (method != TCS_METHOD_MFMA && method != TCS_METHOD_DIRECT)
+        return set_error(TCS_ERR_INVALID, "problem: unknown method %d", method);
+    if (precision != TCS_PREC_FP16 && precision != TCS_PREC_BF16 && precision != TCS_PREC_SPLIT)
+        return set_error(TCS_ERR_INVALID, "problem: unknown precision %d", precision);
+    if (method == TCS_METHOD_DIRECT && precision != TCS_PREC_FP16)
+        return set_error(TCS_ERR_UNSUPPORTED, "problem: the direct-FMA kernel runs FP16 storage only");
+    if (!B) return set_error(TCS_ERR_INVALID, "problem: null coefficients");
+    if (n != 2 * r + 1) return set_error(TCS_ERR_INVALID, "problem: n = %d, stencil needs n = %d", n, 2 * r + 1);
+    if (rows < 1 || cols < 1) return set_error(TCS_ERR_INVALID, "problem: grid %dx%d is empty", rows, cols);
+    memset(p, 0, sizeof *p);
+    p->stencil = stencil;
+    p->method = method;
+    p->precision = precision;
+    p->radius = r;
+    p->layout.rows = rows;
+    p->layout.cols = cols;
+    const bool star = stencil == TCS_STAR2D1R || stencil == TCS_STAR2D2R;
+    for (int i = 0; i < n; ++i)
+        for (int j = 0; j < n; ++j)
+            if (!star || i == r || j == r) p->coeff[(i + 2 - r) * 5 + (j + 2 - r)] = B[i * n + j];
+    float maxc = 0.0f;
+    for (int k = 0; k < 25; ++k) {
+        if (!isfinite(p->coeff[k])) return set_error(TCS_ERR_INVALID, "problem: non-finite coefficient");
+        maxc = fmaxf(maxc, fabsf(p->coeff[k]));
+    }
+    if (precision == TCS_PREC_SPLIT) {
+        // Scale so that the largest |coefficient| lies in [1, 2): the lo parts
+        // then stay in binary16's normal range for all but tiny taps.
+        p->coeff_exp = maxc > 0.0f ? -ilogbf(maxc) : 0;
+        for (int k = 0; k < 25; ++k) {
+            const float c = ldexpf(p->coeff[k], p->coeff_exp);
+            const uint16_t hb = f32_to_f16_bits(c);
+            p->coeff_hi[k] = hb;
+            p->coeff_lo[k] = f32_to_f16_bits(c - f16_bits_to_f32(hb));
+        }
+    }
+    int th, tw;
+    default_tile(rows, cols, &th, &tw);
+    return tcs_problem_set_tile(p, th, tw);
+}
+
+extern "C" int64_t tcs_buffer_bytes(const tcs_problem *p) {
+    if (!p) return 0;
+    return p->layout.plane_elems * 2 * (p->precision == TCS_PREC_SPLIT ? 2 : 1);
+}
+
+extern "C" int tcs_split_scale_exp(double maxabs) {
+    // Bring max|x| to [2^9, 2^10): the lo parts keep full precision and a
+    // stencil with |sum of coefficients| up to 32 cannot overflow binary16.
+    if (!(maxabs > 0.0) || !isfinite(maxabs)) return 0;
+    return 9 - ilogb(maxabs);
+}
+
+extern "C" const char *tcs_kernel_name(const tcs_problem *p) { return p ? sweep_kernel_name(p) : nullptr; }
+extern "C" int tcs_mfma_per_brick(const tcs_problem *p) { return p ? mfma_per_brick(p) : 0; }
+
+// ---------------------------------------------------------------- sweeping
+extern "C" int tcs_sweep(const tcs_problem *p, const void *d_in, void *d_out, void *stream) {
+    if (!p) return set_error(TCS_ERR_INVALID, "sweep: null problem");
+    if (!d_in || !d_out) return set_error(TCS_ERR_INVALID, "sweep: null buffer");
+    if (d_in == d_out) return set_error(TCS_ERR_INVALID, "sweep: in and out must differ (Jacobi)");
+    return launch_sweep(p, d_in, d_out, (hipStream_t)stream);
+}
+
+extern "C" int tcs_run(const tcs_problem *p, void *d_buf0, void *d_buf1, int sweeps, void *stream) {
+    if (sweeps < 0) return set_error(TCS_ERR_INVALID, "run: negative sweep count");
+    void *buf[2] = {d_buf0, d_buf1};
+    for (int s = 0; s < sweeps; ++s) {
+        int st = tcs_sweep(p, buf[s & 1], buf[(s + 1) & 1], stream);
+        if (st) return st;
+    }
+    return TCS_OK;
+}
+
+extern "C" int tcs_graph_create(const tcs_problem *p, void *d_buf0, void *d_buf1, int sweeps, void *stream,
+                                void **graph_exec) {
+    if (!graph_exec) return set_error(TCS_ERR_INVALID, "graph: null output");
+    *graph_exec = nullptr;
+    hipStream_t cs = (hipStream_t)stream;
+    bool own = false;
+    if (!cs) {
+        TCS_HIP_CHECK(hipStreamCreateWithFlags(&cs, hipStreamNonBlocking));
+        own = true;
+    }
+    hipError_t e = hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal);
+    if (e != hipSuccess) {
+        if (own) (void)hipStreamDestroy(cs);
+        return hip_error(e, "hipStreamBeginCapture");
+    }
+    const int st = tcs_run(p, d_buf0, d_buf1, sweeps, cs);
+    hipGraph_t g = nullptr;
+    e = hipStreamEndCapture(cs, &g);
+    if (own) (void)hipStreamDestroy(cs);
+    if (st) {
+        if (g) (void)hipGraphDestroy(g);
+        return st;
+    }
+    if (e != hipSuccess) return hip_error(e, "hipStreamEndCapture");
+    hipGraphExec_t ge = nullptr;
+    e = hipGraphInstantiate(&ge, g, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(g);
+    if (e != hipSuccess) return hip_error(e, "hipGraphInstantiate");
+    *graph_exec = ge;
+    return TCS_OK;
+}
+
+extern "C" int tcs_graph_launch(void *graph_exec, void *stream) {
+    if (!graph_exec) return set_error(TCS_ERR_INVALID, "graph: null exec");
+    TCS_HIP_CHECK(hipGraphLaunch((hipGraphExec_t)graph_exec, (hipStream_t)stream));
+    return TCS_OK;
+}
+
+extern "C" int tcs_graph_destroy(void *graph_exec) {
+    if (!graph_exec) return TCS_OK;
+    TCS_HIP_CHECK(hipGraphExecDestroy((hipGraphExec_t)graph_exec));
+    return TCS_OK;
+}
+
+// ----------------------------------------------------------- host drivers
+namespace {
+
+// Device allocations of one host call, released in every exit path.
+struct DeviceScope {
+    void *ptr[8] = {};
+    int n = 0;
+    hipStream_t stream = nullptr;
+    hipEvent_t ev[2] = {nullptr, nullptr};
+    ~DeviceScope() {
+        for (int i = 0; i < n; ++i) (void)hipFree(ptr[i]);
+        if (ev[0]) (void)hipEventDestroy(ev[0]);
+        if (ev[1]) (void)hipEventDestroy(ev[1]);
+        if (stream) (void)hipStreamDestroy(stream);
+    }
+    int alloc(void **out, size_t bytes) {
+        hipError_t e = hipMalloc(out, bytes);
+        if (e != hipSuccess) return set_error(TCS_ERR_ALLOC, "hipMalloc(%zu): %s", bytes, hipGetErrorString(e));
+        ptr[n++] = *out;
+        return TCS_OK;
+    }
+};
+
+}  // namespace
+
+#define TCS_TRY(expr)          \
+    do {                       \
+        int _st = (expr);      \
+        if (_st) return _st;   \
+    } while (0)
+
+static int host_sweep_impl(int stencil, int method, int precision, const float *h_A, const float *h_B, int n,
+                           float *h_C, int rows, int cols, int sweeps, const tcs_params *params, float *ms_rep,
+                           float *ms_per_sweep) {
+    if (!h_A || !h_B || !h_C) return set_error(TCS_ERR_INVALID, "host: null host array");
+    if (sweeps < 1) return set_error(TCS_ERR_INVALID, "host: sweeps must be >= 1");
+    tcs_params prm;
+    if (params)
+        prm = *params;
+    else
+        tcs_params_from_env(&prm);
+    tcs_problem p;
+    TCS_TRY(tcs_problem_init(&p, stencil, method, precision, rows, cols, h_B, n));
+    if (prm.TILE_SIZE > 0) TCS_TRY(tcs_problem_set_tile(&p, p.tile_h, prm.TILE_SIZE));
+
+    DeviceScope ds;
+    TCS_HIP_CHECK(hipStreamCreateWithFlags(&ds.stream, hipStreamNonBlocking));
+    TCS_HIP_CHECK(hipEventCreate(&ds.ev[0]));
+    TCS_HIP_CHECK(hipEventCreate(&ds.ev[1]));
+    const size_t grid_bytes = (size_t)rows * cols * sizeof(float);
+    const size_t buf_bytes = (size_t)tcs_buffer_bytes(&p);
+    void *d_src, *d_dst, *buf0, *buf1, *d_max;
+    TCS_TRY(ds.alloc(&d_src, grid_bytes));
+    TCS_TRY(ds.alloc(&d_dst, grid_bytes));
+    TCS_TRY(ds.alloc(&buf0, buf_bytes));
+    TCS_TRY(ds.alloc(&buf1, buf_bytes));
+    TCS_TRY(ds.alloc(&d_max, sizeof(double)));
+    hipStream_t s = ds.stream;
+    TCS_HIP_CHECK(hipMemcpyAsync(d_src, h_A, grid_bytes, hipMemcpyHostToDevice, s));
+
+    int scale_exp = 0;
+    if (precision == TCS_PREC_SPLIT) {
+        TCS_TRY(tcs_maxabs_f32((const float *)d_src, rows, cols, cols, (double *)d_max, s));
+        double mx = 0.0;
+        TCS_HIP_CHECK(hipMemcpyAsync(&mx, d_max, sizeof mx, hipMemcpyDeviceToHost, s));
+        TCS_HIP_CHECK(hipStreamSynchronize(s));
+        scale_exp = tcs_split_scale_exp(mx);
+    }
+    TCS_TRY(tcs_pack_f32(&p, (const float *)d_src, cols, buf0, scale_exp, s));
+    TCS_TRY(tcs_clear(&p, buf1, s));
+
+    // Timing as the reference does (layout16/host_box2d2r.cu:171-193): RUN_TIMES
+    // repetitions, the first SKIP_TIMES untimed; every repetition starts from
+    // the packed input.
+    const int reps = prm.RUN_TIMES > 0 ? prm.RUN_TIMES : 1;
+    const int skip = prm.SKIP_TIMES < reps ? (prm.SKIP_TIMES > 0 ? prm.SKIP_TIMES : 0) : 0;
+    double sum_ms = 0.0;
+    for (int t = 0; t < reps; ++t) {
+        if (t > 0 && sweeps > 1) TCS_TRY(tcs_pack_f32(&p, (const float *)d_src, cols, buf0, scale_exp, s));
+        TCS_HIP_CHECK(hipEventRecord(ds.ev[0], s));
+        TCS_TRY(tcs_run(&p, buf0, buf1, sweeps, s));
+        TCS_HIP_CHECK(hipEventRecord(ds.ev[1], s));
+        TCS_HIP_CHECK(hipEventSynchronize(ds.ev[1]));
+        float ms = 0.0f;
+        TCS_HIP_CHECK(hipEventElapsedTime(&ms, ds.ev[0], ds.ev[1]));
+        if (t >= skip) sum_ms += ms;
+    }
+    const double per_rep = sum_ms / (reps - skip);
+    if (ms_rep) *ms_rep = (float)(per_rep / (prm.STEP_TIMES > 0 ? prm.STEP_TIMES : 1));
+    if (ms_per_sweep) *ms_per_sweep = (float)(per_rep / sweeps);
+
+    void *result = (sweeps & 1) ? buf1 : buf0;
+    TCS_TRY(tcs_unpack_f32(&p, result, (float *)d_dst, cols, scale_exp, (const float *)d_src, s));
+    TCS_HIP_CHECK(hipMemcpyAsync(h_C, d_dst, grid_bytes, hipMemcpyDeviceToHost, s));
+    TCS_HIP_CHECK(hipStreamSynchronize(s));
+    return TCS_OK;
+}
+
+extern "C" int tcs_host_sweep(int stencil, int method, int precision, const float *h_A, const float *h_B, int n,
+                              float *h_C, int rows, int cols, int sweeps, const tcs_params *params,
+                              float *ms_per_sweep) {
+    return host_sweep_impl(stencil, method, precision, h_A, h_B, n, h_C, rows, cols, sweeps, params, nullptr,
+                           ms_per_sweep);
+}
+
+// The reference's driver contract: one sweep, timing printed, exit(-1) on error.
+static void dropin(int stencil, int method, float *h_A, float *h_B, float *h_C, int N, int n) {
+    float ms = 0.0f;
+    const int st =
+        host_sweep_impl(stencil, method, TCS_PREC_FP16, h_A, h_B, n, h_C, N, N, 1, nullptr, &ms, nullptr);
+    if (st != TCS_OK) {
+        printf("HIP error : %s, %s\n", tcs_status_string(st), tcs_last_error());
+        exit(-1);
+    }
+    printf("[Time] Time used: %f ms\n", ms);
+}
+
+extern "C" void host_star2d1r(float *h_A, float *h_B, float *h_C, int N, int n) { dropin(TCS_STAR2D1R, TCS_METHOD_MFMA, h_A, h_B, h_C, N, n); }
+extern "C" void host_star2d2r(float *h_A, float *h_B, float *h_C, int N, int n) { dropin(TCS_STAR2D2R, TCS_METHOD_MFMA, h_A, h_B, h_C, N, n); }
+extern "C" void host_box2d1r(float *h_A, float *h_B, float *h_C, int N, int n) { dropin(TCS_BOX2D1R, TCS_METHOD_MFMA, h_A, h_B, h_C, N, n); }
+extern "C" void host_box2d2r(float *h_A, float *h_B, float *h_C, int N, int n) { dropin(TCS_BOX2D2R, TCS_METHOD_MFMA, h_A, h_B, h_C, N, n); }
+extern "C" void host_direct_star2d1r(float *h_A, float *h_B, float *h_C, int N, int n) { dropin(TCS_STAR2D1R, TCS_METHOD_DIRECT, h_A, h_B, h_C, N, n); }
+extern "C" void host_direct_star2d2r(float *h_A, float *h_B, float *h_C, int N, int n) { dropin(TCS_STAR2D2R, TCS_METHOD_DIRECT, h_A, h_B, h_C, N, n); }
+extern "C" void host_direct_box2d1r(float *h_A, float *h_B, float *h_C, int N, int n) { dropin(TCS_BOX2D1R, TCS_METHOD_DIRECT, h_A, h_B, h_C, N, n); }
+extern "C" void host_direct_box2d2r(float *h_A, float *h_B, float *h_C, int N, int n) { dropin(TCS_BOX2D2R, TCS_METHOD_DIRECT, h_A, h_B, h_C, N, n); }

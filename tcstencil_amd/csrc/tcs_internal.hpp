@@ -1,0 +1,64 @@
+// Internal definitions shared by the HIP translation units of libtcstencil.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "../../include/tcstencil.h"
+
+namespace tcs {
+
+typedef _Float16 half8 __attribute__((ext_vector_type(8)));
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
+typedef uint32_t u32x2 __attribute__((ext_vector_type(2)));
+
+constexpr int kWG = 256;  // threads per workgroup of every tiled kernel (4 wave64)
+constexpr int kXCD = 8;   // MI355X: 8 XCDs, workgroups dispatched round-robin
+
+// Storage element of global grid point (gr, gc), gr, gc >= -16: bricks are
+// 16x16 row-major, brick-row-major, behind a one-brick ghost ring, so stored
+// brick (bi+1, bj+1) holds data brick (bi, bj).  (-2 & 15) == 14, so negative
+// coordinates inside the ghost ring map to the right brick row/column.
+__host__ __device__ inline int64_t stor_index(int gr, int gc, int sbc) {
+    const int bi = (gr + 16) >> 4, bj = (gc + 16) >> 4;
+    return ((int64_t)bi * sbc + bj) * 256 + ((gr & 15) << 4) + (gc & 15);
+}
+
+// Arguments of one sweep launch, passed by value.
+struct SweepArgs {
+    const uint16_t *in;
+    uint16_t *out;
+    int64_t plane;         // elements per storage plane (SPLIT: lo plane offset)
+    int32_t rows, cols;    // logical grid
+    int32_t sbc;           // stored bricks per brick row
+    int32_t tiles_x;       // tiles per tile row
+    int32_t ntiles;        // tiles in the grid
+    int32_t tiles_per_xcd; // ceil(ntiles / 8)
+    int32_t rs;            // stencil radius: points closer than rs to the edge are copied
+    float out_scale;       // SPLIT: 2^-coeff_exp
+    float cf[25];          // 5x5 FP32 coefficient embedding
+    uint16_t ch[25], cl[25];
+};
+
+// Kernel launchers (tcs_sweep.hip).
+int launch_sweep(const tcs_problem *p, const void *d_in, void *d_out, hipStream_t stream);
+const char *sweep_kernel_name(const tcs_problem *p);
+int mfma_per_brick(const tcs_problem *p);
+
+// Error reporting (tcs_host.cpp).
+int set_error(int status, const char *fmt, ...);
+int hip_error(hipError_t e, const char *what);
+
+// Host FP16/BF16 rounding helpers (round to nearest even).
+uint16_t f32_to_f16_bits(float f);
+float f16_bits_to_f32(uint16_t h);
+
+}  // namespace tcs
+
+#define TCS_HIP_CHECK(expr)                                          \
+    do {                                                             \
+        hipError_t _e = (expr);                                      \
+        if (_e != hipSuccess) return ::tcs::hip_error(_e, #expr);    \
+    } while (0)

@@ -1,0 +1,289 @@
+"""CPU emulator of the HIP kernels' index arithmetic -- test infrastructure.
+
+Replays, thread by thread (vectorised over threadIdx), the address and lane
+arithmetic of tcstencil_amd/csrc/tcs_transform.hip (pack/unpack) and
+tcstencil_amd/csrc/tcs_sweep.hip (stage_tile, tcs_mfma_sweep, tcs_direct_sweep):
+the storage index of every global load and store, every LDS read and write, the
+MFMA operand lane layouts and the accumulator layout.  Every access is checked
+against its buffer's bounds and alignment, and reads of LDS words that no
+thread wrote come back as NaN, so a staging hole poisons the output.
+
+MFMA semantics assumed (v_mfma_f32_16x16x32_{f16,bf16}, wave64):
+  A (16x32): lane l, element e holds A[l % 16][8*(l//16) + e]
+  B (32x16): lane l, element e holds B[8*(l//16) + e][l % 16]
+  D (16x16): lane l, element v holds D[4*(l//16) + v][l % 16]
+The kernels depend only on the row/column placement and on A and B sharing
+one K order; the GPU test tests/test_gpu_parity.py checks the real hardware.
+
+Arithmetic here is float64 (exact for the dyadic inputs the tests use), with
+FP16 rounding where the kernels round.
+"""
+from __future__ import annotations
+
+import numpy as np
+
+WG = 256
+XCD = 8
+
+
+def stor_index(gr, gc, sbc):
+    gr = np.asarray(gr, dtype=np.int64)
+    gc = np.asarray(gc, dtype=np.int64)
+    bi = (gr + 16) >> 4
+    bj = (gc + 16) >> 4
+    return (bi * sbc + bj) * 256 + ((gr & 15) << 4) + (gc & 15)
+
+
+class Layout:
+    def __init__(self, rows, cols, align):
+        self.rows, self.cols, self.align = rows, cols, align
+        self.nbr = -(-(-(-rows // 16)) // align) * align
+        self.nbc = -(-(-(-cols // 16)) // align) * align
+        self.sbr, self.sbc = self.nbr + 2, self.nbc + 2
+        self.plane = self.sbr * self.sbc * 256
+
+
+def _check(idx, size, align=1, what=""):
+    idx = np.asarray(idx)
+    assert idx.min() >= 0 and idx.max() < size, f"{what}: index out of bounds [{idx.min()}, {idx.max()}] size {size}"
+    assert np.all(idx % align == 0), f"{what}: misaligned access"
+
+
+def f16(x):
+    return np.asarray(x, dtype=np.float64).astype(np.float16).astype(np.float64)
+
+
+# ------------------------------------------------------------------ pack
+def pack(src: np.ndarray, L: Layout) -> np.ndarray:
+    """tcs_pack_kernel, FP16 precision: returns the storage plane (float64 of FP16 values)."""
+    out = np.full(L.plane, np.nan)
+    cpr = 2 * L.sbc
+    t = np.arange(16 * L.sbr * cpr, dtype=np.int64)
+    sr, cc = t // cpr, t % cpr
+    gr, gc0 = sr - 16, 8 * cc - 16
+    s = stor_index(gr, gc0, L.sbc)
+    _check(s, L.plane - 7, 8, "pack store")
+    for e in range(8):
+        gc = gc0 + e
+        inside = (gr >= 0) & (gr < L.rows) & (gc >= 0) & (gc < L.cols)
+        vals = np.zeros(t.shape)
+        vals[inside] = src[gr[inside], gc[inside]]
+        out[s + e] = f16(vals)
+    assert not np.isnan(out).any(), "pack leaves storage elements unwritten"
+    return out
+
+
+# ---------------------------------------------------------------- unpack
+def unpack(st: np.ndarray, L: Layout, ring=None, rs=0) -> np.ndarray:
+    q4 = (L.cols + 3) // 4
+    t = np.arange(L.rows * q4, dtype=np.int64)
+    gr, gc0 = t // q4, 4 * (t % q4)
+    s = stor_index(gr, gc0, L.sbc)
+    _check(s, L.plane - 3, 4, "unpack load")
+    out = np.full((L.rows, L.cols), np.nan)
+    for e in range(4):
+        gc = gc0 + e
+        ok = gc < L.cols
+        v = st[s[ok] + e]
+        if ring is not None:
+            g_r, g_c = gr[ok], gc[ok]
+            interior = (g_r >= rs) & (g_r < L.rows - rs) & (g_c >= rs) & (g_c < L.cols - rs)
+            v = np.where(interior, v, ring[g_r, g_c])
+        out[gr[ok], gc[ok]] = v
+    return out
+
+
+# ------------------------------------------------------------ tile staging
+class TileGeom:
+    def __init__(self, TH, TW):
+        self.TH, self.TW = TH, TW
+        self.LR = 16 * TH + 4
+        self.LP = 16 * TW + 8
+        self.NCH = TH * TW * 32
+        assert self.NCH % WG == 0
+        self.CPT = self.NCH // WG
+        self.HD = 8 * TW + 2
+        self.NH = 4 * self.HD + 32 * TH
+        self.HPT = -(-self.NH // WG)
+        self.PLANE = self.LR * self.LP
+
+
+def stage_tile(inp: np.ndarray, L: Layout, G: TileGeom, gr0: int, gc0: int) -> np.ndarray:
+    lds = np.full(G.PLANE, np.nan)
+    tid = np.arange(WG, dtype=np.int64)
+    TW, TH = G.TW, G.TH
+    for i in range(G.CPT):
+        c = tid + i * WG
+        b, w = c >> 5, c & 31
+        br, bc = b // TW, b % TW
+        s = stor_index(gr0 + 16 * br + (w >> 1), gc0 + 16 * bc + 8 * (w & 1), L.sbc)
+        _check(s, L.plane - 7, 8, "centre load (16 B)")
+        lr = 16 * br + (w >> 1) + 2
+        lc = 16 * bc + 8 * (w & 1) + 2
+        e0 = lr * G.LP + lc
+        assert np.all(e0 % 2 == 0), "centre LDS write not dword aligned"
+        _check(e0, G.PLANE - 7, 2, "centre LDS write")
+        for k in range(8):
+            lds[e0 + k] = inp[s + k]
+    for i in range(G.HPT):
+        h = tid + i * WG
+        lr = np.empty_like(h)
+        ld = np.empty_like(h)
+        m1 = h < 2 * G.HD
+        lr[m1], ld[m1] = h[m1] // G.HD, h[m1] % G.HD
+        m2 = (~m1) & (h < 4 * G.HD)
+        q = h[m2] - 2 * G.HD
+        lr[m2], ld[m2] = 16 * TH + 2 + q // G.HD, q % G.HD
+        m3 = (~m1) & (~m2) & (h < 4 * G.HD + 16 * TH)
+        lr[m3], ld[m3] = 2 + (h[m3] - 4 * G.HD), 0
+        m4 = ~(m1 | m2 | m3)
+        lr[m4], ld[m4] = 2 + (h[m4] - 4 * G.HD - 16 * TH), G.HD - 1
+        act = h < G.NH
+        s = stor_index(gr0 - 2 + lr[act], gc0 - 2 + 2 * ld[act], L.sbc)
+        _check(s, L.plane - 1, 2, "halo load (4 B)")
+        d = ((lr[act] * G.LP) >> 1) + ld[act]
+        _check(2 * d, G.PLANE - 1, 2, "halo LDS write")
+        lds[2 * d] = inp[s]
+        lds[2 * d + 1] = inp[s + 1]
+    for zc in range(2 * G.LR):
+        d = (((zc >> 1) * G.LP) >> 1) + 8 * TW + 2 + (zc & 1)
+        assert 0 <= 2 * d < G.PLANE
+        lds[2 * d] = 0.0
+        lds[2 * d + 1] = 0.0
+    return lds
+
+
+def _tiles(L: Layout, TH: int, TW: int):
+    tiles_x = L.nbc // TW
+    ntiles = (L.nbr // TH) * tiles_x
+    tpx = -(-ntiles // XCD)
+    seen = set()
+    for wg in range(tpx * XCD):
+        tile = (wg % XCD) * tpx + wg // XCD
+        if tile >= ntiles:
+            continue
+        assert tile not in seen
+        seen.add(tile)
+        ty, tx = tile // tiles_x, tile % tiles_x
+        yield 16 * TH * ty, 16 * TW * tx
+    assert len(seen) == ntiles
+
+
+def _interior(gr, gc, L: Layout, rs: int):
+    return (gr >= rs) & (gr < L.rows - rs) & (gc >= rs) & (gc < L.cols - rs)
+
+
+# ------------------------------------------------------------- MFMA sweep
+def mfma_sweep(inp, L: Layout, TH, TW, coeff5, rs, round_coeff=f16):
+    """tcs_mfma_sweep for FP16 (one plane).  coeff5: 5x5 FP32 embedding."""
+    assert L.nbr % TH == 0 and L.nbc % TW == 0
+    G = TileGeom(TH, TW)
+    ZLO, ZHI = (0, 4) if rs == 2 else (1, 3)
+    out = np.zeros(L.plane)
+    written = np.zeros(L.plane, dtype=bool)
+    lane = np.arange(64)
+    m, g = lane & 15, lane >> 4
+    gsel = np.minimum(g, 2)
+    # A fragments: afrag[zi][lane][e]
+    afrag = np.zeros((ZHI - ZLO + 1, 64, 8))
+    for zi in range(ZHI - ZLO + 1):
+        for e in range(8):
+            d = 8 * g + e - m
+            ok = (d >= 0) & (d <= 4)
+            afrag[zi, ok, e] = round_coeff(coeff5[ZLO + zi, d[ok]])
+    for gr0, gc0 in _tiles(L, TH, TW):
+        lds = stage_tile(inp, L, G, gr0, gc0)
+        edge = not (gr0 >= rs and gr0 + 16 * TH <= L.rows - rs and gc0 >= rs and gc0 + 16 * TW <= L.cols - rs)
+        for wave in range(4):
+            for q in range(TH * TW // 4):
+                b = wave + 4 * q
+                br, bc = b // TW, b % TW
+                D = np.zeros((16, 16))
+                for zi in range(ZHI - ZLO + 1):
+                    off = (16 * br + m + ZLO + zi) * G.LP + 16 * bc + 8 * gsel
+                    _check(off, G.PLANE - 7, 8, "B operand LDS read (16 B)")
+                    bfrag = lds[off[:, None] + np.arange(8)[None, :]]  # [lane][e]
+                    Am = np.zeros((16, 32))
+                    Bm = np.zeros((32, 16))
+                    for l in range(64):
+                        Am[l % 16, 8 * (l // 16):8 * (l // 16) + 8] = afrag[zi, l]
+                        Bm[8 * (l // 16):8 * (l // 16) + 8, l % 16] = bfrag[l]
+                    D += Am @ Bm
+                # lane l holds D[4g+v][n], n = l % 16: output row n, column 4g+v
+                gr = gr0 + 16 * br + m
+                gcb = gc0 + 16 * bc + 4 * g
+                o = np.stack([f16(D[4 * g + v, m]) for v in range(4)], axis=1)  # [lane][v]
+                if edge:
+                    cidx = (16 * br + m + 2) * G.LP + 16 * bc + 4 * g + 2
+                    for v in range(4):
+                        keep = ~_interior(gr, gcb + v, L, rs)
+                        o[keep, v] = lds[cidx[keep] + v]
+                s = stor_index(gr, gcb, L.sbc)
+                _check(s, L.plane - 3, 4, "output store (8 B)")
+                for v in range(4):
+                    assert not written[s + v].any(), "two lanes store the same element"
+                    out[s + v] = o[:, v]
+                    written[s + v] = True
+    return out, written
+
+
+# ----------------------------------------------------------- direct sweep
+def _taps(stencil):
+    t = np.zeros((5, 5), dtype=bool)
+    if stencil == "star2d1r":
+        t[2, 1:4] = True
+        t[1, 2] = t[3, 2] = True
+    elif stencil == "star2d2r":
+        t[2, :] = True
+        t[:, 2] = True
+    elif stencil == "box2d1r":
+        t[1:4, 1:4] = True
+    else:
+        t[:, :] = True
+    return t
+
+
+def direct_sweep(inp, L: Layout, TH, TW, coeff5, stencil, rs):
+    G = TileGeom(TH, TW)
+    CG = 4 * TW
+    RG = WG // CG
+    MR = 16 * TH // RG
+    assert RG * CG == WG and MR * RG == 16 * TH
+    taps = _taps(stencil)
+    out = np.zeros(L.plane)
+    written = np.zeros(L.plane, dtype=bool)
+    tid = np.arange(WG)
+    cg, rg = tid % CG, tid // CG
+    r0 = rg * MR
+    for gr0, gc0 in _tiles(L, TH, TW):
+        lds = stage_tile(inp, L, G, gr0, gc0)
+        acc = np.zeros((WG, MR, 4))
+        for ir in range(MR + 4):
+            base = (r0 + ir) * G.LP + 4 * cg
+            _check(base, G.PLANE - 7, 4, "direct LDS read (8 B)")
+            x = lds[base[:, None] + np.arange(8)[None, :]]
+            for o in range(MR):
+                dz = ir - o
+                if dz < 0 or dz > 4:
+                    continue
+                for dj in range(5):
+                    if taps[dz, dj]:
+                        acc[:, o, :] += x[:, dj:dj + 4] * np.float32(coeff5[dz, dj])
+        edge = not (gr0 >= rs and gr0 + 16 * TH <= L.rows - rs and gc0 >= rs and gc0 + 16 * TW <= L.cols - rs)
+        for o in range(MR):
+            tr = r0 + o
+            gr = gr0 + tr
+            gcb = gc0 + 4 * cg
+            h = f16(acc[:, o, :])
+            if edge:
+                cidx = (tr + 2) * G.LP + 4 * cg + 2
+                for v in range(4):
+                    keep = ~_interior(gr, gcb + v, L, rs)
+                    h[keep, v] = lds[cidx[keep] + v]
+            s = stor_index(gr, gcb, L.sbc)
+            _check(s, L.plane - 3, 4, "direct store (8 B)")
+            for v in range(4):
+                assert not written[s + v].any()
+                out[s + v] = h[:, v]
+                written[s + v] = True
+    return out, written

@@ -1,0 +1,139 @@
+"""The C ABI library loads here (no GPU) and exports every function that
+include/tcstencil.h declares; host-side validation and set-up need no GPU."""
+import ctypes
+import os
+import re
+
+import numpy as np
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+HEADER = os.path.join(ROOT, "include", "tcstencil.h")
+
+
+def declared_functions():
+    text = open(HEADER).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    names = re.findall(r"^[A-Za-z_][\w \*]*?\b(\w+)\s*\(", text, flags=re.M)
+    return sorted(set(n for n in names if n not in ("if", "while")))
+
+
+def test_header_declares_the_dropins():
+    names = declared_functions()
+    for s in ("star2d1r", "star2d2r", "box2d1r", "box2d2r"):
+        assert f"host_{s}" in names and f"host_direct_{s}" in names
+    assert len(names) >= 40
+
+
+def test_library_exports_every_declared_symbol():
+    import tcstencil_amd as T
+    L = T.lib()
+    missing = [n for n in declared_functions() if not hasattr(L, n)]
+    assert not missing, missing
+
+
+def test_python_signatures_cover_the_header():
+    from tcstencil_amd import _lib
+    assert sorted(_lib.SIGNATURES) == declared_functions()
+
+
+@pytest.mark.parametrize("stencil", ["star2d1r", "star2d2r", "box2d1r", "box2d2r"])
+def test_host_code_shims_export_host_code(stencil):
+    for kind in ("", "direct_"):
+        path = os.path.join(ROOT, "tcstencil_amd", f"libtcs_host_code_{kind}{stencil}.so")
+        lib = ctypes.CDLL(path)
+        assert hasattr(lib, "host_code")
+
+
+def test_struct_sizes_match_c():
+    from tcstencil_amd import _lib
+    assert ctypes.sizeof(_lib.Layout) == 40
+    assert ctypes.sizeof(_lib.ProblemStruct) == 272
+    assert ctypes.sizeof(_lib.Params) == 32
+
+
+def test_params_mirror_param_hpp(monkeypatch):
+    import tcstencil_amd as T
+    for k in ("TCS_RUN_TIMES", "TCS_SKIP_TIMES", "TCS_TILE_SIZE"):
+        monkeypatch.delenv(k, raising=False)
+    p = T.params()
+    assert (p.TOLERANCE, p.MIN_NUMBER, p.RUN_TIMES, p.STEP_TIMES, p.SKIP_TIMES) == (0.01, 1e-6, 500, 1, 400)
+    monkeypatch.setenv("TCS_RUN_TIMES", "7")
+    assert T.params().RUN_TIMES == 7
+
+
+def test_layout_and_problem_setup():
+    import tcstencil_amd as T
+    B = np.arange(25, dtype=np.float32).reshape(5, 5)
+    p = T.Problem("box2d2r", 16384, 16384, B)
+    L = p.s.layout
+    assert (L.nbr, L.nbc, L.sbr, L.sbc, L.align) == (1024, 1024, 1026, 1026, 8)
+    assert p.buffer_bytes == 1026 * 1026 * 256 * 2
+    assert p.tile == (4, 8) and p.mfma_per_brick == 5
+    assert list(p.s.coeff) == list(range(25))
+    p = T.Problem("star2d1r", 100, 37, np.arange(9, dtype=np.float32).reshape(3, 3))
+    c = np.array(p.s.coeff).reshape(5, 5)
+    want = np.zeros((5, 5))
+    want[1, 2], want[2, 1], want[2, 2], want[2, 3], want[3, 2] = 1, 3, 4, 5, 7
+    assert np.array_equal(c, want)
+    p.set_tile(2, 8)
+    assert p.tile == (2, 8) and p.s.layout.align == 8
+
+
+def test_split_coefficients_are_exact_hi_lo():
+    import tcstencil_amd as T
+    rng = np.random.default_rng(2)
+    B = rng.uniform(0, 1, (5, 5)).astype(np.float32)
+    p = T.Problem("box2d2r", 64, 64, B, precision="split")
+    e = p.s.coeff_exp
+    c = np.array(p.s.coeff, dtype=np.float32) * np.float32(2.0 ** e)
+    hi = np.array(p.s.coeff_hi, dtype=np.uint16).view(np.float16)
+    lo = np.array(p.s.coeff_lo, dtype=np.uint16).view(np.float16)
+    assert np.array_equal(hi, c.astype(np.float16))  # RNE like numpy
+    assert np.array_equal(lo, (c - hi.astype(np.float32)).astype(np.float16))
+    assert 1.0 <= np.abs(c).max() < 2.0
+    err = np.abs(hi.astype(np.float64) + lo.astype(np.float64) - c.astype(np.float64))
+    assert np.all(err <= np.abs(c) * 2.0 ** -21)
+
+
+def test_invalid_arguments_return_status():
+    import tcstencil_amd as T
+    with pytest.raises(T.TcsError, match="n = 3"):
+        T.Problem("box2d2r", 10, 10, np.ones((3, 3), np.float32))
+    with pytest.raises(T.TcsError, match="empty"):
+        T.Problem("box2d1r", 0, 10, np.ones((3, 3), np.float32))
+    with pytest.raises(T.TcsError, match="direct-FMA"):
+        T.Problem("box2d1r", 10, 10, np.ones((3, 3), np.float32), method="direct", precision="bf16")
+    p = T.Problem("box2d1r", 64, 64, np.ones((3, 3), np.float32), precision="split")
+    with pytest.raises(T.TcsError, match="no 8x8"):
+        p.set_tile(8, 8)
+    with pytest.raises(T.TcsError, match="not offered"):
+        p.set_tile(3, 8)
+    with pytest.raises(T.TcsError, match="non-finite"):
+        T.Problem("box2d1r", 64, 64, np.full((3, 3), np.nan, np.float32))
+    L = T.lib()
+    assert L.tcs_sweep(None, None, None, None) == T._lib.ERR_INVALID
+    assert L.tcs_status_string(4) == b"allocation failed"
+
+
+def test_split_scale_exponent():
+    import tcstencil_amd as T
+    for mx in (1.0, 0.75, 3e-3, 1e3, 65000.0):
+        e = T.split_scale_exp(mx)
+        assert 2 ** 9 <= mx * 2.0 ** e < 2 ** 10
+    assert T.split_scale_exp(0.0) == 0
+
+
+def test_gen_coeffs_normalised():
+    import tcstencil_amd as T
+    for s in T.STENCIL_NAMES:
+        B = T.gen_coeffs(s, seed=1)
+        assert abs(float(B.astype(np.float64).sum()) - 1.0) < 1e-6
+        assert B.min() >= 0.0
+        if s.startswith("star"):
+            r = T.radius(s)
+            off = B.copy()
+            off[r, :] = 0
+            off[:, r] = 0
+            assert not off.any()
+    assert not np.array_equal(T.gen_coeffs("box2d2r", 1), T.gen_coeffs("box2d2r", 2))

@@ -1,0 +1,83 @@
+"""CPU pin of the kernels' index arithmetic (tests/kernel_emulator.py replays
+the HIP kernels thread by thread with bounds and alignment checks).
+
+Dyadic inputs (multiples of 2^-6 in [-1, 1]) and coefficients (multiples of
+2^-5) make every product and partial sum exact in FP32 and FP64, so the
+emulated kernels must reproduce fp16(FP64 oracle) bit for bit; any wrong
+index, transposed operand or dropped halo shows up as a mismatch."""
+import numpy as np
+import pytest
+
+import kernel_emulator as E
+
+STENCILS = ("star2d1r", "star2d2r", "box2d1r", "box2d2r")
+RADIUS = {"star2d1r": 1, "star2d2r": 2, "box2d1r": 1, "box2d2r": 2}
+CASES = [(70, 150, 2, 4), (70, 150, 4, 8), (33, 17, 2, 4), (130, 140, 8, 8), (5, 3, 4, 4), (64, 128, 2, 8)]
+
+
+def dyadic_problem(stencil, rows, cols, seed):
+    rng = np.random.default_rng(seed)
+    r = RADIUS[stencil]
+    n = 2 * r + 1
+    A = rng.integers(-64, 65, (rows, cols)) / 64.0
+    B = rng.integers(0, 33, (n, n)) / 32.0
+    if stencil.startswith("star"):
+        mask = np.zeros((n, n), bool)
+        mask[r, :] = mask[:, r] = True
+        B = B * mask
+    c5 = np.zeros((5, 5))
+    c5[2 - r:3 + r, 2 - r:3 + r] = B
+    return A, B, c5
+
+
+@pytest.mark.parametrize("stencil", STENCILS)
+@pytest.mark.parametrize("case", CASES, ids=lambda c: "%dx%d_t%dx%d" % c)
+@pytest.mark.parametrize("method", ["mfma", "direct"])
+def test_emulated_kernel_exact(oracle, stencil, case, method):
+    rows, cols, TH, TW = case
+    r = RADIUS[stencil]
+    A, B, c5 = dyadic_problem(stencil, rows, cols, seed=rows * 1000 + cols)
+    L = E.Layout(rows, cols, max(TH, TW))
+    st = E.pack(A, L)
+    if method == "mfma":
+        out, written = E.mfma_sweep(st, L, TH, TW, c5, r)
+    else:
+        out, written = E.direct_sweep(st, L, TH, TW, c5, stencil, r)
+    # every data brick is written, the ghost ring never
+    data = np.zeros(L.plane, dtype=bool)
+    for bi in range(L.nbr):
+        base = ((bi + 1) * L.sbc + 1) * 256
+        data[base:base + L.nbc * 256] = True
+    assert np.array_equal(written, data)
+    got = E.unpack(out, L)
+    want = E.f16(oracle.sweep_f64(stencil, A, B))
+    assert np.array_equal(got, want)
+
+
+def test_emulated_unpack_ring_and_roundtrip():
+    rng = np.random.default_rng(1)
+    A = rng.integers(-64, 65, (37, 45)) / 64.0
+    L = E.Layout(37, 45, 4)
+    st = E.pack(A, L)
+    assert np.array_equal(E.unpack(st, L), A)
+    ring = rng.standard_normal((37, 45))
+    got = E.unpack(st, L, ring=ring, rs=2)
+    assert np.array_equal(got[2:-2, 2:-2], A[2:-2, 2:-2])
+    assert np.array_equal(got[:2], ring[:2]) and np.array_equal(got[:, -2:], ring[:, -2:])
+
+
+def test_storage_index_is_the_reference_brick_map():
+    """With rows = cols = N (N % 16 == 0) and align 1 the storage is the
+    reference's padded brick layout: pad by one 16-wide ring, then
+    out[(i*nb + j)*256 + x*16 + y] = in[(16i + x)*R + 16j + y]
+    (layout16/host_box2d2r.cu:61-86)."""
+    N = 48
+    L = E.Layout(N, N, 1)
+    R = N + 32
+    nb = R // 16
+    i, j, x, y = np.meshgrid(np.arange(nb), np.arange(nb), np.arange(16), np.arange(16), indexing="ij")
+    ref_out = (i * nb + j) * 256 + x * 16 + y
+    padded_r, padded_c = 16 * i + x, 16 * j + y  # coordinates in the padded grid
+    mine = E.stor_index(padded_r - 16, padded_c - 16, L.sbc)
+    assert L.sbc == nb and L.plane == R * R
+    assert np.array_equal(mine, ref_out)

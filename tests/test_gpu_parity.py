@@ -1,0 +1,377 @@
+"""GPU parity of the HIP kernels against the oracle (run on an MI355X).
+
+Bars (the tolerance is written at each test):
+  * dyadic inputs: FP16/BF16/SPLIT MFMA and direct-FMA outputs are bitwise
+    equal to the FP64 oracle rounded to the storage format (every product and
+    partial sum is exact, so any index, lane-layout or halo error shows);
+  * random inputs: max-abs within the derived rounding bound and rel-L2
+    within the stated targets (FP16 <= 2e-3 per sweep, SPLIT <= 1e-6);
+  * the GPU FP64 oracle is bitwise equal to the CPU FP64 oracle;
+  * full size (16384^2): rel-L2 vs the GPU FP64 oracle.
+"""
+import subprocess
+import sys
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+STENCILS = ("star2d1r", "star2d2r", "box2d1r", "box2d2r")
+RADIUS = {"star2d1r": 1, "star2d2r": 2, "box2d1r": 1, "box2d2r": 2}
+MFMA_TILES = [(2, 4), (4, 4), (2, 8), (4, 8), (8, 8)]
+
+
+@pytest.fixture(scope="module")
+def torch():
+    import torch as _t
+    assert _t.cuda.is_available(), "GPU tests need a GPU"
+    return _t
+
+
+@pytest.fixture(scope="module")
+def T():
+    import tcstencil_amd
+    return tcstencil_amd
+
+
+def say(*a):
+    print(*a, flush=True)
+
+
+def dyadic(stencil, rows, cols, seed):
+    rng = np.random.default_rng(seed)
+    r = RADIUS[stencil]
+    n = 2 * r + 1
+    A = rng.integers(-64, 65, (rows, cols)) / 64.0
+    B = rng.integers(0, 33, (n, n)) / 32.0
+    if stencil.startswith("star"):
+        m = np.zeros((n, n), bool)
+        m[r, :] = m[:, r] = True
+        B = B * m
+    return A, B.astype(np.float32)
+
+
+def gpu_sweeps(T, torch, stencil, A, B, method="mfma", precision="fp16", tile=None, sweeps=1,
+               scale_exp=0, graph=False):
+    rows, cols = A.shape
+    p = T.Problem(stencil, rows, cols, B, method=method, precision=precision, tile=tile)
+    src = torch.from_numpy(np.ascontiguousarray(A)).cuda()
+    b0, b1 = p.alloc(), p.alloc()
+    p.clear(b1)
+    p.pack(src, b0, scale_exp)
+    if graph:
+        g = p.graph(b0, b1, sweeps)
+        g.launch()
+        res = b1 if sweeps % 2 else b0
+    else:
+        res = p.run(b0, b1, sweeps)
+    out = p.unpack(res, scale_exp=scale_exp)
+    torch.cuda.synchronize()
+    return out.cpu().numpy().astype(np.float64)
+
+
+def round_bf16(torch, x):
+    return torch.from_numpy(np.asarray(x, np.float64)).to(torch.bfloat16).to(torch.float64).numpy()
+
+
+# ------------------------------------------------------------ bitwise pins
+@pytest.mark.parametrize("stencil", STENCILS)
+@pytest.mark.parametrize("tile", MFMA_TILES, ids=lambda t: "t%dx%d" % t)
+@pytest.mark.parametrize("method", ["mfma", "direct"])
+def test_dyadic_exact_fp16(T, torch, oracle, stencil, tile, method):
+    for rows, cols in ((70, 150), (33, 17), (130, 260)):
+        say("case", stencil, method, tile, rows, cols)
+        A, B = dyadic(stencil, rows, cols, seed=rows + cols)
+        got = gpu_sweeps(T, torch, stencil, A, B, method=method, tile=tile)
+        want = oracle.sweep_f64(stencil, A, B).astype(np.float16).astype(np.float64)
+        assert np.array_equal(got, want), f"max diff {np.abs(got - want).max()}"
+
+
+@pytest.mark.parametrize("stencil", STENCILS)
+def test_dyadic_exact_bf16(T, torch, oracle, stencil):
+    for tile in MFMA_TILES:
+        say("case bf16", stencil, tile)
+        A, B = dyadic(stencil, 70, 150, seed=3)
+        got = gpu_sweeps(T, torch, stencil, A, B, precision="bf16", tile=tile)
+        want = round_bf16(torch, oracle.sweep_f64(stencil, A, B))
+        assert np.array_equal(got, want)
+
+
+@pytest.mark.parametrize("stencil", STENCILS)
+def test_dyadic_exact_split(T, torch, oracle, stencil):
+    for tile in MFMA_TILES[:4]:
+        say("case split", stencil, tile)
+        A, B = dyadic(stencil, 70, 150, seed=4)
+        got = gpu_sweeps(T, torch, stencil, A, B, precision="split", tile=tile, scale_exp=3)
+        assert np.array_equal(got, oracle.sweep_f64(stencil, A, B))
+
+
+def test_dyadic_exact_multisweep(T, torch, oracle):
+    # Exactness survives 3 sweeps while values stay on a coarse dyadic grid:
+    # coefficients in {0, 1/2} with one tap keep every value a multiple of 2^-9.
+    for stencil in STENCILS:
+        say("case multisweep", stencil)
+        r = RADIUS[stencil]
+        n = 2 * r + 1
+        A, _ = dyadic(stencil, 96, 160, seed=8)
+        B = np.zeros((n, n), np.float32)
+        B[r, r] = 0.5
+        B[r, 0] = 0.5
+        got = gpu_sweeps(T, torch, stencil, A, B, sweeps=3)
+        want = oracle.sweeps_f64(stencil, A, B, 3)
+        assert np.array_equal(got, want)
+
+
+# ----------------------------------------------------------- random inputs
+def random_problem(T, torch, stencil, rows, cols, seed):
+    x = T.gen_uniform(rows, cols, seed=seed, dtype=torch.float64)
+    T.round_to(x, "fp16")
+    return x.cpu().numpy(), T.gen_coeffs(stencil, seed).astype(np.float64)
+
+
+@pytest.mark.parametrize("stencil", STENCILS)
+@pytest.mark.parametrize("method", ["mfma", "direct"])
+def test_random_fp16_within_bound(T, torch, oracle, stencil, method):
+    say("case random fp16", stencil, method)
+    A, B = random_problem(T, torch, stencil, 300, 517, seed=1)
+    got = gpu_sweeps(T, torch, stencil, A, B.astype(np.float32), method=method)
+    ref = oracle.sweep_f64(stencil, A, B)
+    res = oracle.error_norms(got.astype(np.float32), ref, RADIUS[stencil])
+    say("  ", res)
+    # |err| <= u16*(sum|c|*max|x|) [coefficient rounding, MFMA only]
+    #        + u16*|out| [output rounding] + FP32 accumulation (< 1e-6); u16 = 2^-11
+    bound = (2.0 ** -11) * (2.0 if method == "mfma" else 1.0) * np.abs(A).max() + 1e-6
+    assert res["max_abs"] <= bound
+    assert res["rel_l2"] <= 2e-3
+    ring = np.ones_like(A, bool)
+    r = RADIUS[stencil]
+    ring[r:-r, r:-r] = False
+    assert np.array_equal(got[ring], A[ring])
+
+
+@pytest.mark.parametrize("stencil", STENCILS)
+def test_random_bf16_within_bound(T, torch, oracle, stencil):
+    say("case random bf16", stencil)
+    x = T.gen_uniform(300, 517, seed=2, dtype=torch.float64)
+    T.round_to(x, "bf16")
+    A = x.cpu().numpy()
+    B = T.gen_coeffs(stencil, 2)
+    got = gpu_sweeps(T, torch, stencil, A, B, precision="bf16")
+    res = oracle.error_norms(got.astype(np.float32), oracle.sweep_f64(stencil, A, B.astype(np.float64)), RADIUS[stencil])
+    say("  ", res)
+    assert res["max_abs"] <= 2 * (2.0 ** -8) * np.abs(A).max() + 1e-6  # u_bf16 = 2^-8
+    assert res["rel_l2"] <= 8e-3
+
+
+@pytest.mark.parametrize("stencil", STENCILS)
+def test_random_split_rel_l2(T, torch, oracle, stencil):
+    say("case random split", stencil)
+    x = T.gen_uniform(300, 517, seed=3, dtype=torch.float32)
+    A = x.cpu().numpy().astype(np.float64)
+    B = T.gen_coeffs(stencil, 3)
+    e = T.split_scale_exp(float(np.abs(A).max()))
+    got = gpu_sweeps(T, torch, stencil, A.astype(np.float32), B, precision="split", scale_exp=e)
+    res = oracle.error_norms(got.astype(np.float32), oracle.sweep_f64(stencil, A, B.astype(np.float64)), RADIUS[stencil])
+    say("  ", res)
+    assert res["rel_l2"] <= 1e-6
+    assert res["max_abs"] <= 1e-6
+
+
+@pytest.mark.parametrize("stencil", STENCILS)
+def test_ten_sweeps(T, torch, oracle, stencil):
+    say("case ten sweeps", stencil)
+    A, B = random_problem(T, torch, stencil, 200, 333, seed=4)
+    r = RADIUS[stencil]
+    ref = oracle.sweeps_f64(stencil, A, B, 10)
+    for precision, tol in (("fp16", 10 * 2e-3), ("split", 10 * 1e-6)):
+        e = T.split_scale_exp(1.0) if precision == "split" else 0
+        got = gpu_sweeps(T, torch, stencil, A, B.astype(np.float32), precision=precision, sweeps=10, scale_exp=e)
+        res = oracle.error_norms(got.astype(np.float32), ref, r)
+        say("  ", precision, res)
+        assert res["rel_l2"] <= tol
+    got = gpu_sweeps(T, torch, stencil, A, B.astype(np.float32), method="direct", sweeps=10)
+    assert oracle.error_norms(got.astype(np.float32), ref, r)["rel_l2"] <= 10 * 2e-3
+
+
+# ------------------------------------------------------------------ edges
+@pytest.mark.parametrize("stencil", STENCILS)
+def test_tiny_and_degenerate_grids(T, torch, oracle, stencil):
+    for rows, cols in ((1, 1), (2, 3), (3, 3), (4, 5), (5, 5), (6, 40), (17, 1), (16, 16)):
+        say("case tiny", stencil, rows, cols)
+        A, B = dyadic(stencil, rows, cols, seed=rows * 31 + cols)
+        for method in ("mfma", "direct"):
+            got = gpu_sweeps(T, torch, stencil, A, B, method=method)
+            want = oracle.sweep_f64(stencil, A, B).astype(np.float16).astype(np.float64)
+            assert np.array_equal(got, want)
+
+
+def test_reference_driver_inputs(T, torch, oracle):
+    """The reference driver's own case: all-ones input, init_B coefficients
+    (common/main_2r.cu:13-28); interior = sum(B) exactly."""
+    for stencil in STENCILS:
+        say("case init_B", stencil)
+        B = oracle.init_B(stencil)
+        A = np.ones((96, 96))
+        for method in ("mfma", "direct"):
+            got = gpu_sweeps(T, torch, stencil, A, B, method=method)
+            assert np.array_equal(got, oracle.gold(stencil, A.astype(np.float32), B).astype(np.float64))
+
+
+def test_nan_propagation(T, torch, oracle):
+    """A NaN input poisons at least every output gold poisons (the direct kernel
+    exactly those; the MFMA form also the brick rows whose window holds it)."""
+    A, B = dyadic("box2d2r", 64, 64, seed=5)
+    A[30, 33] = np.nan
+    ref = oracle.sweep_f64("box2d2r", A, B)
+    for method in ("direct", "mfma"):
+        got = gpu_sweeps(T, torch, "box2d2r", A, B, method=method)
+        assert np.all(np.isnan(got[np.isnan(ref)]))
+        if method == "direct":
+            assert np.array_equal(np.isnan(got), np.isnan(ref))
+
+
+# ------------------------------------------------------- oracle and utils
+def test_gpu_fp64_oracle_bitwise(T, torch, oracle):
+    rng = np.random.default_rng(6)
+    for stencil in STENCILS:
+        n = 2 * RADIUS[stencil] + 1
+        A = rng.standard_normal((101, 77))
+        B = rng.standard_normal((n, n))
+        x = torch.from_numpy(A).cuda()
+        y = x.clone()
+        T.oracle_sweep_f64(stencil, x, B, y)
+        assert np.array_equal(y.cpu().numpy(), oracle.sweep_f64(stencil, A, B))
+
+
+def test_gpu_error_norms(T, torch, oracle):
+    rng = np.random.default_rng(7)
+    ref = rng.standard_normal((123, 211))
+    out = (ref + rng.standard_normal(ref.shape) * 1e-3).astype(np.float32)
+    g = T.error_norms(torch.from_numpy(out).cuda(), torch.from_numpy(ref).cuda(), 2)
+    c = oracle.error_norms(out, ref, 2)
+    assert g["max_abs"] == c["max_abs"]
+    for k in ("rel_l2", "rms_abs", "ref_l2"):
+        np.testing.assert_allclose(g[k], c[k], rtol=1e-12)
+
+
+def test_generators_match_oracle(T, torch, oracle):
+    g = T.gen_uniform(77, 131, seed=5, stream_id=0, lo=-1.0, hi=1.0).cpu().numpy()
+    assert np.array_equal(g, oracle.gen_uniform_f32(77, 131, 5, 0, -1.0, 1.0))
+    x = T.gen_sinusoid(64, 96, seed=1)
+    assert float(x.abs().max()) < 1.0 + 1e-3
+    assert T.maxabs(x) == float(x.abs().max())
+
+
+def test_pack_unpack_roundtrip_and_ring(T, torch):
+    x = T.gen_uniform(45, 77, seed=9, dtype=torch.float32)
+    T.round_to(x, "fp16")
+    for precision in ("fp16", "split"):
+        p = T.Problem("box2d2r", 45, 77, np.ones((5, 5), np.float32), precision=precision)
+        b = p.alloc()
+        p.pack(x, b)
+        assert torch.equal(p.unpack(b), x)
+    ring = torch.full_like(x, 7.0)
+    y = p.unpack(b, ring=ring)
+    assert torch.equal(y[2:-2, 2:-2], x[2:-2, 2:-2]) and bool((y[:2] == 7.0).all())
+
+
+# ------------------------------------------------ graphs and determinism
+def test_graph_replays_match_eager(T, torch):
+    A, B = dyadic("box2d2r", 256, 384, seed=10)
+    p = T.Problem("box2d2r", 256, 384, B)
+    src = torch.from_numpy(A).cuda()
+    b0, b1 = p.alloc(), p.alloc()
+    p.clear(b1)
+    p.pack(src, b0)
+    eager = p.unpack(p.run(b0, b1, 5)).clone()
+    g = p.graph(b0, b1, 5)
+    for rep in range(3):
+        p.pack(src, b0)           # eager work between replays
+        g.launch()
+        torch.cuda.synchronize()
+        assert torch.equal(p.unpack(b1), eager), f"replay {rep}"
+    g.destroy()
+
+
+def test_runs_are_deterministic(T, torch):
+    x = T.gen_uniform(500, 700, seed=11, dtype=torch.float32)
+    B = T.gen_coeffs("box2d2r", 11)
+    p = T.Problem("box2d2r", 500, 700, B)
+    outs = []
+    for _ in range(2):
+        b0, b1 = p.alloc(), p.alloc()
+        p.clear(b1)
+        p.pack(x, b0)
+        outs.append(p.unpack(p.run(b0, b1, 4)))
+    assert torch.equal(outs[0], outs[1])
+
+
+# ------------------------------------------------------------ host drivers
+@pytest.mark.parametrize("stencil", STENCILS)
+def test_host_sweep_vs_gold(T, torch, oracle, stencil):
+    say("case host_sweep", stencil)
+    rng = np.random.default_rng(12)
+    A = rng.uniform(-1, 1, (250, 190)).astype(np.float32)
+    B = T.gen_coeffs(stencil, 12)
+    gold = oracle.sweep_f32(stencil, A, B)
+    r = RADIUS[stencil]
+    for method in ("mfma", "direct"):
+        C, ms = T.host_sweep(stencil, A, B, method=method)
+        # input rounding to FP16 + coefficient rounding (MFMA) + output rounding
+        assert np.abs(C - gold)[r:-r, r:-r].max() <= 3 * 2.0 ** -11 + 1e-6
+        ring = np.ones_like(A, bool)
+        ring[r:-r, r:-r] = False
+        assert np.array_equal(C[ring], A[ring]) and ms > 0
+    C, _ = T.host_sweep(stencil, A, B, precision="split", sweeps=2)
+    want = oracle.sweep_f32(stencil, gold, B)
+    assert np.abs(C - want)[r:-r, r:-r].max() <= 1e-5
+
+
+@pytest.mark.parametrize("method", ["mfma", "direct"])
+@pytest.mark.parametrize("stencil", STENCILS)
+def test_dropin_driver_binary(stencil, method):
+    """The main_Xr-style C++ driver linked against the host_code and gold shims."""
+    import os
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    exe = os.path.join(root, "tests", "dropin", "_build", f"main_{method}_{stencil}")
+    n = 2 * RADIUS[stencil] + 1
+    env = dict(os.environ, TCS_RUN_TIMES="20", TCS_SKIP_TIMES="10")
+    out = subprocess.run([exe, "200", str(n)], capture_output=True, text=True, timeout=120, env=env)
+    say(out.stdout)
+    assert out.returncode == 0, out.stderr
+    lines = dict(l.split(":", 1) for l in out.stdout.splitlines() if l.startswith("[Test]"))
+    assert "[Time] Time used" in out.stdout
+    assert int(lines["[Test] Ring mismatches"]) == 0
+    max_err = float(lines["[Test] Max Error"].split()[0])
+    max_ref = float(lines["[Test] Max Error"].split()[2].rstrip(")"))
+    # |A| <= 0.5; init_B is integral (exact in FP16); FP16 rounding of the
+    # input (<= u*sum|B|*0.5) and of the output (<= u*|out|), u = 2^-11.
+    sum_abs_b = 72.0 if n == 3 else 578.0
+    assert max_err <= 2.0 ** -11 * (0.5 * sum_abs_b + max_ref) * 1.01 + 1e-6
+
+
+# ---------------------------------------------------------------- full size
+@pytest.mark.parametrize("method", ["mfma", "direct"])
+def test_full_size_box2d2r_16384(T, torch, method):
+    """North-star size: box2d2r 16384^2 FP16, one sweep, vs the GPU FP64 oracle
+    (itself bitwise-pinned to the CPU oracle above)."""
+    say("case full size", method)
+    N = 16384
+    B = T.gen_coeffs("box2d2r", 1)
+    x64 = T.gen_sinusoid(N, N, seed=1, noise=1e-3)
+    T.round_to(x64, "fp16")
+    ref = torch.empty_like(x64).copy_(x64)
+    T.oracle_sweep_f64("box2d2r", x64, B.astype(np.float64), ref)
+    p = T.Problem("box2d2r", N, N, B, method=method)
+    b0, b1 = p.alloc(), p.alloc()
+    p.clear(b1)
+    p.pack(x64, b0)
+    del x64
+    p.sweep(b0, b1)
+    out = p.unpack(b1)
+    res = T.error_norms(out, ref, 2)
+    say("  ", res)
+    assert res["rel_l2"] <= 2e-3
+    # coefficient + output rounding, max|x| <= 1 + 1e-3 (sum of amplitudes + noise)
+    assert res["max_abs"] <= (2.0 ** -11) * 2.0 * 1.001 + 1e-6
