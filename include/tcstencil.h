@@ -152,6 +152,9 @@ int tcs_unpack_f32(const tcs_problem *p, const void *d_src, float *d_dst, int64_
                    int scale_exp, const float *d_ring, void *stream);
 /* Zero a whole storage buffer (a fill kernel, safe under graph capture). */
 int tcs_clear(const tcs_problem *p, void *d_buf, void *stream);
+/* Device-to-device streaming copy (bytes % 16 == 0): the measured bandwidth
+ * ceiling reported next to the HBM spec figure. */
+int tcs_copy(void *d_dst, const void *d_src, int64_t bytes, void *stream);
 /* max |x| over a row-major FP32/FP64 grid into *d_out (one double, device). */
 int tcs_maxabs_f32(const float *d_src, int rows, int cols, int64_t ld, double *d_out, void *stream);
 int tcs_maxabs_f64(const double *d_src, int rows, int cols, int64_t ld, double *d_out, void *stream);

@@ -83,6 +83,7 @@ SIGNATURES = {
     "tcs_pack_f64": (_i, [_prob, _vp, _i64, _vp, _i, _vp]),
     "tcs_unpack_f32": (_i, [_prob, _vp, _vp, _i64, _i, _vp, _vp]),
     "tcs_clear": (_i, [_prob, _vp, _vp]),
+    "tcs_copy": (_i, [_vp, _vp, _i64, _vp]),
     "tcs_maxabs_f32": (_i, [_vp, _i, _i, _i64, _vp, _vp]),
     "tcs_maxabs_f64": (_i, [_vp, _i, _i, _i64, _vp, _vp]),
     "tcs_split_scale_exp": (_i, [ctypes.c_double]),

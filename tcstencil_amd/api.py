@@ -198,6 +198,14 @@ def error_norms(out32, ref64, r: int, stream=None) -> dict:
             "ref_l2": float(res[3])}
 
 
+def copy(dst, src, stream=None) -> None:
+    """Streaming device copy of src's bytes into dst (the bandwidth ceiling kernel)."""
+    nbytes = src.numel() * src.element_size()
+    if dst.numel() * dst.element_size() < nbytes:
+        raise ValueError("copy: destination too small")
+    check(lib().tcs_copy(_ptr(dst), _ptr(src), nbytes, _stream(stream)))
+
+
 def maxabs(x, stream=None) -> float:
     torch = _torch()
     d = torch.zeros(1, dtype=torch.float64, device=x.device)

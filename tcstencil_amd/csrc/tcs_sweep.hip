@@ -12,7 +12,7 @@
 //                                           S_z[k][i] = X[i+z][k]
 //           is one v_mfma_f32_16x16x32_{f16,bf16} per row shift z: M = output
 //           column j, N = output row i, K = haloed column k (20 of 32 slots
-//           used, the rest multiply zero coefficients).  The banded Toeplitz
+//           used, the rest multiply zero coefficients; slots 24..31 read no LDS).  The banded Toeplitz
 //           C_z is the reference's init_coe matrix (layout16/host_box2d2r.cu:29-50)
 //           transposed and widened to cover the halo columns, so no scalar
 //           halo fix-ups are needed.  C_z lives in registers for the whole
@@ -218,9 +218,9 @@ __global__ __launch_bounds__(kWG) void tcs_mfma_sweep(SweepArgs a) {
     __syncthreads();
 
     const bool edge = !tile_is_interior(a, gr0, gc0, 16 * TH, 16 * TW);
-    // Slot group 3 (k = 24..31) only meets zero coefficients; it re-reads
-    // group 2's columns so that every operand is finite data from the tile.
-    const int gsel = g < 2 ? g : 2;
+    // Slot group 3 (k = 24..31) only meets zero coefficients: those lanes skip
+    // the LDS read and feed zeros.
+    const bool reads = g < 3;
     constexpr int NB = TH * TW / 4;
 #pragma unroll
     for (int q = 0; q < NB; ++q) {
@@ -229,12 +229,15 @@ __global__ __launch_bounds__(kWG) void tcs_mfma_sweep(SweepArgs a) {
         f32x4 acc = {0.0f, 0.0f, 0.0f, 0.0f};
 #pragma unroll
         for (int zi = 0; zi < NZ; ++zi) {
-            const int off = (16 * br + m + ZLO + zi) * T::LP + 16 * bc + 8 * gsel;
-            const frag_t bh = __builtin_bit_cast(frag_t, *reinterpret_cast<const u32x4 *>(lds + off));
+            const int off = (16 * br + m + ZLO + zi) * T::LP + 16 * bc + 8 * g;
+            u32x4 rh = {0u, 0u, 0u, 0u};
+            if (reads) rh = *reinterpret_cast<const u32x4 *>(lds + off);
+            const frag_t bh = __builtin_bit_cast(frag_t, rh);
             acc = F::mma(ahi[zi], bh, acc);
             if constexpr (PREC == TCS_PREC_SPLIT) {
-                const frag_t bl = __builtin_bit_cast(
-                    frag_t, *reinterpret_cast<const u32x4 *>(lds + T::PLANE + off));
+                u32x4 rl = {0u, 0u, 0u, 0u};
+                if (reads) rl = *reinterpret_cast<const u32x4 *>(lds + T::PLANE + off);
+                const frag_t bl = __builtin_bit_cast(frag_t, rl);
                 acc = F::mma(ahi[zi], bl, acc);
                 acc = F::mma(alo[zi], bh, acc);
                 acc = F::mma(alo[zi], bl, acc);

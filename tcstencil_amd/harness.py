@@ -1,0 +1,168 @@
+"""Profiling harness: one workload per process, optionally under rocprofv3.
+
+Replaces the reference's sweep pipeline -- compile one binary per
+(stencil, method, mesh size, tile size) (script/nvcc_compile.py:12-27,
+FLAG_DICT script/param.py:1-6), run each under `ncu` and scrape "Duration"
+(script/run_file_with_ncu.sh:5), then turn it into CSV
+(script/process_data.py:46-57) -- with a single library and runtime parameters:
+
+  python -m tcstencil_amd.harness run --stencil box2d2r --size 16384 --method mfma --sweeps 20
+      one workload: pack, warm up, time `sweeps` back-to-back sweeps with HIP
+      events; --copy-mb adds a streaming copy of known size (the bandwidth
+      ceiling, and the calibration kernel for FETCH_SIZE / WRITE_SIZE).
+      Prints one JSON line.
+
+  python -m tcstencil_amd.harness sweep --sizes 1024 4096 16384 --out results.csv [--rocprof]
+      every (stencil, method, size, tile) in the cross product; with --rocprof
+      each workload runs in its own `rocprofv3 --kernel-trace --stats` process
+      and the kernel's average duration comes from the profiler's stats file,
+      otherwise from HIP events.  The CSV keeps the reference's columns
+      `type,stencil,mesh size,tile size,time,unit` (process_data.py:14) and adds
+      GStencil/s and HBM GB/s.
+"""
+from __future__ import annotations
+
+import argparse
+import csv
+import glob
+import itertools
+import json
+import os
+import subprocess
+import sys
+import tempfile
+
+HBM_PEAK_GBS = 8000.0
+BYTES_PER_UPDATE = {"fp16": 4.0, "bf16": 4.0, "split": 8.0}
+TILES = [(2, 4), (4, 4), (2, 8), (4, 8), (8, 8)]
+
+
+def run_workload(stencil, rows, cols, method="mfma", precision="fp16", tile=None, sweeps=20,
+                 warmup=5, copy_mb=0, seed=1):
+    import numpy as np
+    import torch
+
+    from . import api as T
+
+    B = T.gen_coeffs(stencil, seed)
+    p = T.Problem(stencil, rows, cols, B, method=method, precision=precision, tile=tile)
+    x = T.gen_uniform(rows, cols, seed=seed, dtype=torch.float32)
+    e = T.split_scale_exp(1.0) if precision == "split" else 0
+    b0, b1 = p.alloc(), p.alloc()
+    p.clear(b1)
+    p.pack(x, b0, e)
+    del x
+    p.run(b0, b1, warmup + warmup % 2)
+    torch.cuda.synchronize()
+    ev = [torch.cuda.Event(enable_timing=True) for _ in range(2)]
+    ev[0].record()
+    p.run(b0, b1, sweeps)
+    ev[1].record()
+    torch.cuda.synchronize()
+    ms = ev[0].elapsed_time(ev[1]) / sweeps
+    pts = p.interior_points
+    out = {"stencil": stencil, "rows": rows, "cols": cols, "method": method, "precision": precision,
+           "tile": list(p.tile), "kernel": p.kernel_name, "sweeps": sweeps, "ms_per_sweep": ms,
+           "gstencil_s": pts / (ms * 1e-3) / 1e9,
+           "hbm_gbs": BYTES_PER_UPDATE[precision] * pts / (ms * 1e-3) / 1e9}
+    out["hbm_frac"] = out["hbm_gbs"] / HBM_PEAK_GBS
+    if copy_mb:
+        n = int(copy_mb * 2 ** 20) // 16 * 16
+        src = torch.empty(n, dtype=torch.uint8, device="cuda")
+        dst = torch.empty_like(src)
+        src.fill_(1)
+        for _ in range(3):
+            T.copy(dst, src)
+        torch.cuda.synchronize()
+        reps = 10
+        ev[0].record()
+        for _ in range(reps):
+            T.copy(dst, src)
+        ev[1].record()
+        torch.cuda.synchronize()
+        cms = ev[0].elapsed_time(ev[1]) / reps
+        out["copy"] = {"bytes_per_launch": 2 * n, "ms": cms, "gbs": 2 * n / (cms * 1e-3) / 1e9}
+    del b0, b1
+    return out
+
+
+def rocprof_kernel_avg_ns(cmd, kernel_prefix, workdir):
+    """Run `cmd` under rocprofv3 --kernel-trace --stats; return (average ns, calls)
+    of the kernels whose name contains kernel_prefix."""
+    full = ["rocprofv3", "--kernel-trace", "--stats", "-d", workdir, "-o", "run", "--"] + cmd
+    subprocess.run(full, check=True, stdout=subprocess.DEVNULL, stderr=subprocess.DEVNULL)
+    files = glob.glob(os.path.join(workdir, "**", "*kernel_stats.csv"), recursive=True)
+    if not files:
+        raise RuntimeError("rocprofv3 wrote no kernel_stats.csv")
+    total, calls = 0.0, 0
+    with open(files[0]) as f:
+        for row in csv.DictReader(f):
+            if kernel_prefix in row["Name"]:
+                total += float(row["TotalDurationNs"])
+                calls += int(row["Calls"])
+    if calls == 0:
+        raise RuntimeError(f"no kernel matching {kernel_prefix}")
+    return total / calls, calls
+
+
+def main(argv=None):
+    ap = argparse.ArgumentParser(prog="python -m tcstencil_amd.harness")
+    sub = ap.add_subparsers(dest="cmd", required=True)
+    r = sub.add_parser("run")
+    r.add_argument("--stencil", default="box2d2r")
+    r.add_argument("--size", type=int, default=16384)
+    r.add_argument("--cols", type=int, default=0)
+    r.add_argument("--method", default="mfma")
+    r.add_argument("--precision", default="fp16")
+    r.add_argument("--tile", type=int, nargs=2, default=None)
+    r.add_argument("--sweeps", type=int, default=20)
+    r.add_argument("--warmup", type=int, default=5)
+    r.add_argument("--copy-mb", type=float, default=0)
+    s = sub.add_parser("sweep")
+    s.add_argument("--stencils", nargs="+", default=["star2d1r", "star2d2r", "box2d1r", "box2d2r"])
+    s.add_argument("--methods", nargs="+", default=["mfma", "direct"])
+    s.add_argument("--precision", default="fp16")
+    s.add_argument("--sizes", type=int, nargs="+", default=[1024, 4096, 16384])
+    s.add_argument("--tiles", nargs="+", default=["%dx%d" % t for t in TILES])
+    s.add_argument("--sweeps", type=int, default=20)
+    s.add_argument("--rocprof", action="store_true")
+    s.add_argument("--out", default="")
+    a = ap.parse_args(argv)
+
+    if a.cmd == "run":
+        res = run_workload(a.stencil, a.size, a.cols or a.size, a.method, a.precision,
+                           tuple(a.tile) if a.tile else None, a.sweeps, a.warmup, a.copy_mb)
+        print(json.dumps(res), flush=True)
+        return 0
+
+    rows = []
+    for stencil, method, size, tile in itertools.product(a.stencils, a.methods, a.sizes, a.tiles):
+        th, tw = (int(v) for v in tile.split("x"))
+        if a.rocprof:
+            cmd = [sys.executable, "-m", "tcstencil_amd.harness", "run", "--stencil", stencil,
+                   "--size", str(size), "--method", method, "--precision", a.precision,
+                   "--tile", str(th), str(tw), "--sweeps", str(a.sweeps)]
+            prefix = "tcs_mfma_sweep" if method == "mfma" else "tcs_direct_sweep"
+            with tempfile.TemporaryDirectory() as d:
+                ns, _ = rocprof_kernel_avg_ns(cmd, prefix, d)
+            ms = ns * 1e-6
+        else:
+            ms = run_workload(stencil, size, size, method, a.precision, (th, tw), a.sweeps)["ms_per_sweep"]
+        r = 1 if stencil in ("star2d1r", "box2d1r") else 2
+        pts = (size - 2 * r) ** 2
+        row = {"type": "tensor" if method == "mfma" else "baseline", "stencil": stencil,
+               "mesh size": size, "tile size": f"{th}x{tw}", "time": round(ms * 1e3, 3), "unit": "us",
+               "gstencil_s": round(pts / (ms * 1e-3) / 1e9, 1),
+               "hbm_gbs": round(BYTES_PER_UPDATE[a.precision] * pts / (ms * 1e-3) / 1e9, 1)}
+        rows.append(row)
+        print(json.dumps(row), flush=True)
+    if a.out:
+        with open(a.out, "w", newline="") as f:
+            w = csv.DictWriter(f, fieldnames=list(rows[0]))
+            w.writeheader()
+            w.writerows(rows)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

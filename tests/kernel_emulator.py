@@ -183,7 +183,7 @@ def mfma_sweep(inp, L: Layout, TH, TW, coeff5, rs, round_coeff=f16):
     written = np.zeros(L.plane, dtype=bool)
     lane = np.arange(64)
     m, g = lane & 15, lane >> 4
-    gsel = np.minimum(g, 2)
+    reads = g < 3  # slot group 3 reads no LDS and feeds zeros
     # A fragments: afrag[zi][lane][e]
     afrag = np.zeros((ZHI - ZLO + 1, 64, 8))
     for zi in range(ZHI - ZLO + 1):
@@ -200,9 +200,10 @@ def mfma_sweep(inp, L: Layout, TH, TW, coeff5, rs, round_coeff=f16):
                 br, bc = b // TW, b % TW
                 D = np.zeros((16, 16))
                 for zi in range(ZHI - ZLO + 1):
-                    off = (16 * br + m + ZLO + zi) * G.LP + 16 * bc + 8 * gsel
-                    _check(off, G.PLANE - 7, 8, "B operand LDS read (16 B)")
-                    bfrag = lds[off[:, None] + np.arange(8)[None, :]]  # [lane][e]
+                    off = (16 * br + m + ZLO + zi) * G.LP + 16 * bc + 8 * g
+                    _check(off[reads], G.PLANE - 7, 8, "B operand LDS read (16 B)")
+                    bfrag = np.zeros((64, 8))
+                    bfrag[reads] = lds[off[reads][:, None] + np.arange(8)[None, :]]  # [lane][e]
                     Am = np.zeros((16, 32))
                     Bm = np.zeros((32, 16))
                     for l in range(64):

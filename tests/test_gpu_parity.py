@@ -344,7 +344,7 @@ def test_dropin_driver_binary(stencil, method):
     assert "[Time] Time used" in out.stdout
     assert int(lines["[Test] Ring mismatches"]) == 0
     max_err = float(lines["[Test] Max Error"].split()[0])
-    max_ref = float(lines["[Test] Max Error"].split()[2].rstrip(")"))
+    max_ref = float(lines["[Test] Max Error"].split()[3].rstrip(")"))
     # |A| <= 0.5; init_B is integral (exact in FP16); FP16 rounding of the
     # input (<= u*sum|B|*0.5) and of the output (<= u*|out|), u = 2^-11.
     sum_abs_b = 72.0 if n == 3 else 578.0
