@@ -80,7 +80,8 @@ typedef struct tcs_problem {
     uint16_t coeff_hi[25];   /* SPLIT: FP16 bits of hi(coeff * 2^coeff_exp) */
     uint16_t coeff_lo[25];   /* SPLIT: FP16 bits of lo(coeff * 2^coeff_exp) */
     int32_t coeff_exp;       /* SPLIT: coefficient scale exponent */
-    int32_t reserved;
+    int32_t wg_per_cu;       /* 0: one workgroup per tile; n > 0: a persistent grid
+                                of n workgroups per CU that prefetch their next tile */
 } tcs_problem;
 
 /* ---------------------------------------------------------- status, params */

@@ -40,7 +40,7 @@ class ProblemStruct(ctypes.Structure):
                 ("coeff", ctypes.c_float * 25),
                 ("coeff_hi", ctypes.c_uint16 * 25),
                 ("coeff_lo", ctypes.c_uint16 * 25),
-                ("coeff_exp", ctypes.c_int32), ("reserved", ctypes.c_int32)]
+                ("coeff_exp", ctypes.c_int32), ("wg_per_cu", ctypes.c_int32)]
 
 
 class Params(ctypes.Structure):

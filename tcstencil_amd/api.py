@@ -50,7 +50,7 @@ class Problem:
     """A tcs_problem descriptor: stencil, method, precision, grid, coefficients."""
 
     def __init__(self, stencil, rows: int, cols: int, B, method="mfma", precision="fp16",
-                 tile=None):
+                 tile=None, wg_per_cu: int = 0):
         self.stencil = stencil_id(stencil)
         self.method = L.METHOD_ID[method] if isinstance(method, str) else int(method)
         self.precision = L.PREC_ID[precision] if isinstance(precision, str) else int(precision)
@@ -61,6 +61,7 @@ class Problem:
                                      self.precision, rows, cols, _fp(self.B), n))
         if tile is not None:
             self.set_tile(*tile)
+        self.s.wg_per_cu = wg_per_cu
 
     # -- descriptor ---------------------------------------------------------
     def set_tile(self, tile_h: int, tile_w: int) -> None:
@@ -81,6 +82,14 @@ class Problem:
     @property
     def radius(self) -> int:
         return self.s.radius
+
+    @property
+    def wg_per_cu(self) -> int:
+        return self.s.wg_per_cu
+
+    @wg_per_cu.setter
+    def wg_per_cu(self, n: int) -> None:
+        self.s.wg_per_cu = int(n)
 
     @property
     def tile(self):

@@ -7,7 +7,7 @@ candidate in-process with HIP events on the same data and returns the ranking:
 
   python -m tcstencil_amd.autotune --stencils box2d2r --sizes 16384 --methods mfma direct
 prints one JSON line per (stencil, method, precision, size) with the per-tile
-times and the winner.  `best_tile()` is the library call.
+times and the winner.  `best_config()` is the library call.
 """
 from __future__ import annotations
 
@@ -20,8 +20,10 @@ TILES = [(2, 4), (4, 4), (2, 8), (4, 8), (8, 8)]
 
 
 def tune(stencil, rows, cols, method="mfma", precision="fp16", sweeps=20, warmup=4, seed=1,
-         tiles=None):
-    """[(tile, ms_per_sweep)] sorted fastest first."""
+         tiles=None, wgs=(0,)):
+    """[((tile, wg_per_cu), ms_per_sweep)] sorted fastest first.  wg_per_cu is the
+    number of resident workgroups per CU of the persistent grid (0 = as many as
+    fit)."""
     import torch
 
     from . import api as T
@@ -30,10 +32,10 @@ def tune(stencil, rows, cols, method="mfma", precision="fp16", sweeps=20, warmup
     x = T.gen_uniform(rows, cols, seed=seed, dtype=torch.float32)
     e = T.split_scale_exp(1.0) if precision == "split" else 0
     res = []
-    for tile in tiles or TILES:
+    for tile, wg in itertools.product(tiles or TILES, wgs):
         if precision == "split" and tile == (8, 8):
             continue
-        p = T.Problem(stencil, rows, cols, B, method=method, precision=precision, tile=tile)
+        p = T.Problem(stencil, rows, cols, B, method=method, precision=precision, tile=tile, wg_per_cu=wg)
         b0, b1 = p.alloc(), p.alloc()
         p.clear(b1)
         p.pack(x, b0, e)
@@ -44,12 +46,13 @@ def tune(stencil, rows, cols, method="mfma", precision="fp16", sweeps=20, warmup
         p.run(b0, b1, sweeps)
         ev[1].record()
         torch.cuda.synchronize()
-        res.append((tile, ev[0].elapsed_time(ev[1]) / sweeps))
+        res.append(((tile, wg), ev[0].elapsed_time(ev[1]) / sweeps))
         del b0, b1
     return sorted(res, key=lambda t: t[1])
 
 
-def best_tile(stencil, rows, cols, method="mfma", precision="fp16", **kw):
+def best_config(stencil, rows, cols, method="mfma", precision="fp16", **kw):
+    """(tile, wg_per_cu) of the fastest configuration."""
     return tune(stencil, rows, cols, method, precision, **kw)[0][0]
 
 
@@ -60,16 +63,17 @@ def main(argv=None):
     ap.add_argument("--precisions", nargs="+", default=["fp16"])
     ap.add_argument("--sizes", type=int, nargs="+", default=[16384])
     ap.add_argument("--sweeps", type=int, default=20)
+    ap.add_argument("--wgs", type=int, nargs="+", default=[0])
     a = ap.parse_args(argv)
     for stencil, method, precision, size in itertools.product(a.stencils, a.methods, a.precisions, a.sizes):
         if method == "direct" and precision != "fp16":
             continue
         r = 1 if stencil in ("star2d1r", "box2d1r") else 2
         pts = (size - 2 * r) ** 2
-        ranking = tune(stencil, size, size, method, precision, sweeps=a.sweeps)
+        ranking = tune(stencil, size, size, method, precision, sweeps=a.sweeps, wgs=a.wgs)
         print(json.dumps({"stencil": stencil, "method": method, "precision": precision, "size": size,
-                          "best": list(ranking[0][0]),
-                          "us": {"%dx%d" % t: round(ms * 1e3, 1) for t, ms in ranking},
+                          "best": {"tile": list(ranking[0][0][0]), "wg_per_cu": ranking[0][0][1]},
+                          "us": {"%dx%d/wg%d" % (t + (w,)): round(ms * 1e3, 1) for (t, w), ms in ranking},
                           "best_gstencil_s": round(pts / (ranking[0][1] * 1e-3) / 1e9, 1)}), flush=True)
     return 0
 

@@ -2,10 +2,11 @@
 // replacing layout16/kernel_<stencil>.cu::mma_run) and the direct-FMA kernel
 // (replacing baseline/kernel_<stencil>.cu::kernel_run).
 //
-// Work unit: a workgroup of 256 threads (4 wave64) owns a tile of TH x TW data
-// bricks (16*TH x 16*TW points).  It stages the tile plus a 2-point halo into
-// LDS once (16-byte global loads for the data bricks, 4-byte loads for the
-// halo frame), then
+// Work unit: a persistent workgroup of 256 threads (4 wave64) walks a run of
+// tiles of TH x TW data bricks (16*TH x 16*TW points).  For each tile it writes
+// the tile plus a 2-point halo into LDS from registers, issues the global loads
+// of its next tile (16-byte loads for the data bricks, 4-byte loads for the
+// halo frame) so they are in flight meanwhile, then
 //   MFMA:   each wave computes whole 16x16 bricks.  For a brick, with X the
 //           20x20 haloed input (X[0][0] = brick origin - (2,2)),
 //               out^T = sum_z  C_z * S_z,   C_z[j][k] = c[z][k-j] (0 <= k-j <= 4),
@@ -27,6 +28,18 @@
 
 namespace tcs {
 
+// The MFMA kernel stores each 16x16 output brick whole (512 contiguous bytes
+// per wave instruction) and never reads it again in the same sweep, so its
+// stores are nontemporal.  The direct kernel writes 32-byte pieces of a line
+// from different instructions; nontemporal stores measured 2x slower there.
+template <bool NT>
+__device__ __forceinline__ void store8(uint16_t *p, u32x2 w) {
+    if constexpr (NT)
+        __builtin_nontemporal_store(w, reinterpret_cast<u32x2 *>(p));
+    else
+        *reinterpret_cast<u32x2 *>(p) = w;
+}
+
 // ------------------------------------------------------------ tile staging
 template <int TH, int TW, int NPL>
 struct Tile {
@@ -41,15 +54,43 @@ struct Tile {
     static_assert(NCH % kWG == 0, "tile must hold a multiple of 8 bricks");
 };
 
-// LDS element (lr, lc) of plane p holds global point (gr0 - 2 + lr, gc0 - 2 + lc)
-// for lr < LR, lc < 16*TW + 4; columns [16*TW+4, LP) hold zeros.
+// One tile's global loads, held in registers between issue and the LDS write
+// so that the next tile's loads are in flight while the current one computes.
 template <int TH, int TW, int NPL>
-__device__ __forceinline__ void stage_tile(const SweepArgs &a, uint16_t *lds, int gr0, int gc0,
-                                           int tid) {
+struct TileRegs {
     using T = Tile<TH, TW, NPL>;
     u32x4 cv[NPL][T::CPT];
     uint32_t hv[NPL][T::HPT];
-    // Issue every global load before the first LDS write.
+};
+
+// Halo frame dword h -> (LDS row, LDS dword column): two rows above, two rows
+// below (HD dwords each), then the left and right 2-column strips.
+template <int TH, int TW>
+__device__ __forceinline__ void halo_coords(int h, int &lr, int &ld) {
+    constexpr int HD = 8 * TW + 2;
+    if (h < 2 * HD) {
+        lr = h / HD;
+        ld = h - lr * HD;
+    } else if (h < 4 * HD) {
+        const int q = h - 2 * HD;
+        const int rr = q / HD;
+        lr = 16 * TH + 2 + rr;
+        ld = q - rr * HD;
+    } else if (h < 4 * HD + 16 * TH) {
+        lr = 2 + (h - 4 * HD);
+        ld = 0;
+    } else {
+        lr = 2 + (h - 4 * HD - 16 * TH);
+        ld = HD - 1;
+    }
+}
+
+// LDS element (lr, lc) of plane p holds global point (gr0 - 2 + lr, gc0 - 2 + lc)
+// for lr < LR, lc < 16*TW + 4; columns [16*TW+4, LP) hold zeros.
+template <int TH, int TW, int NPL>
+__device__ __forceinline__ void load_tile(const SweepArgs &a, TileRegs<TH, TW, NPL> &R, int gr0, int gc0,
+                                          int tid) {
+    using T = Tile<TH, TW, NPL>;
 #pragma unroll
     for (int i = 0; i < T::CPT; ++i) {
         const int c = tid + i * kWG;
@@ -58,37 +99,25 @@ __device__ __forceinline__ void stage_tile(const SweepArgs &a, uint16_t *lds, in
         const int64_t s = stor_index(gr0 + 16 * br + (w >> 1), gc0 + 16 * bc + 8 * (w & 1), a.sbc);
 #pragma unroll
         for (int p = 0; p < NPL; ++p)
-            cv[p][i] = *reinterpret_cast<const u32x4 *>(a.in + p * a.plane + s);
+            R.cv[p][i] = *reinterpret_cast<const u32x4 *>(a.in + p * a.plane + s);
     }
-    int hrow[T::HPT], hcol[T::HPT];
 #pragma unroll
     for (int i = 0; i < T::HPT; ++i) {
         const int h = tid + i * kWG;
-        int lr, ld;
-        if (h < 2 * T::HD) {
-            lr = h / T::HD;
-            ld = h - lr * T::HD;
-        } else if (h < 4 * T::HD) {
-            const int q = h - 2 * T::HD;
-            const int rr = q / T::HD;
-            lr = 16 * TH + 2 + rr;
-            ld = q - rr * T::HD;
-        } else if (h < 4 * T::HD + 16 * TH) {
-            lr = 2 + (h - 4 * T::HD);
-            ld = 0;
-        } else {
-            lr = 2 + (h - 4 * T::HD - 16 * TH);
-            ld = T::HD - 1;
-        }
-        hrow[i] = lr;
-        hcol[i] = ld;
         if (h < T::NH) {
+            int lr, ld;
+            halo_coords<TH, TW>(h, lr, ld);
             const int64_t s = stor_index(gr0 - 2 + lr, gc0 - 2 + 2 * ld, a.sbc);
 #pragma unroll
             for (int p = 0; p < NPL; ++p)
-                hv[p][i] = *reinterpret_cast<const uint32_t *>(a.in + p * a.plane + s);
+                R.hv[p][i] = *reinterpret_cast<const uint32_t *>(a.in + p * a.plane + s);
         }
     }
+}
+
+template <int TH, int TW, int NPL>
+__device__ __forceinline__ void write_tile(const TileRegs<TH, TW, NPL> &R, uint16_t *lds, int tid) {
+    using T = Tile<TH, TW, NPL>;
     uint32_t *l32 = reinterpret_cast<uint32_t *>(lds);
 #pragma unroll
     for (int i = 0; i < T::CPT; ++i) {
@@ -101,18 +130,28 @@ __device__ __forceinline__ void stage_tile(const SweepArgs &a, uint16_t *lds, in
 #pragma unroll
         for (int p = 0; p < NPL; ++p) {
 #pragma unroll
-            for (int q = 0; q < 4; ++q) l32[p * (T::PLANE / 2) + d + q] = cv[p][i][q];
+            for (int q = 0; q < 4; ++q) l32[p * (T::PLANE / 2) + d + q] = R.cv[p][i][q];
         }
     }
 #pragma unroll
     for (int i = 0; i < T::HPT; ++i) {
         const int h = tid + i * kWG;
         if (h < T::NH) {
-            const int d = ((hrow[i] * T::LP) >> 1) + hcol[i];
+            int lr, ld;
+            halo_coords<TH, TW>(h, lr, ld);
+            const int d = ((lr * T::LP) >> 1) + ld;
 #pragma unroll
-            for (int p = 0; p < NPL; ++p) l32[p * (T::PLANE / 2) + d] = hv[p][i];
+            for (int p = 0; p < NPL; ++p) l32[p * (T::PLANE / 2) + d] = R.hv[p][i];
         }
     }
+}
+
+// Columns [16*TW+4, 16*TW+8) of every LDS row: read by the MFMA operand of the
+// last brick column (against zero coefficients), never written by write_tile.
+template <int TH, int TW, int NPL>
+__device__ __forceinline__ void zero_pad(uint16_t *lds, int tid) {
+    using T = Tile<TH, TW, NPL>;
+    uint32_t *l32 = reinterpret_cast<uint32_t *>(lds);
     for (int zc = tid; zc < 2 * T::LR; zc += kWG) {
         const int d = (((zc >> 1) * T::LP) >> 1) + 8 * TW + 2 + (zc & 1);
 #pragma unroll
@@ -128,11 +167,26 @@ __device__ __forceinline__ bool point_is_interior(const SweepArgs &a, int gr, in
     return gr >= a.rs && gr < a.rows - a.rs && gc >= a.rs && gc < a.cols - a.rs;
 }
 
-// XCD-aware tile order: workgroup w runs on XCD w % 8, so give each XCD a
-// contiguous run of tiles; vertically adjacent tiles then share an L2.
-__device__ __forceinline__ int xcd_tile(const SweepArgs &a) {
-    const int wg = blockIdx.x;
-    return (wg % kXCD) * a.tiles_per_xcd + wg / kXCD;
+// Persistent tile schedule.  Workgroup w runs on XCD w % 8; XCD x owns the
+// contiguous tile range [x*tpx, (x+1)*tpx), and its gridDim.x/8 workgroups step
+// through that range together, so the tiles in flight on one XCD are
+// neighbours and their halos meet in that XCD's L2.
+struct Schedule {
+    int tile, end, step;
+};
+
+__device__ __forceinline__ Schedule schedule(const SweepArgs &a) {
+    const int x = blockIdx.x % kXCD, k = blockIdx.x / kXCD;
+    const int begin = x * a.tiles_per_xcd;
+    const int end = begin + a.tiles_per_xcd < a.ntiles ? begin + a.tiles_per_xcd : a.ntiles;
+    return Schedule{begin + k, end, (int)gridDim.x / kXCD};
+}
+
+template <int TH, int TW>
+__device__ __forceinline__ void tile_origin(const SweepArgs &a, int tile, int &gr0, int &gc0) {
+    const int ty = tile / a.tiles_x, tx = tile - ty * a.tiles_x;
+    gr0 = 16 * TH * ty;
+    gc0 = 16 * TW * tx;
 }
 
 __device__ __forceinline__ uint16_t f16_bits(float v) {
@@ -165,7 +219,7 @@ struct Frag<TCS_PREC_BF16> {
 template <>
 struct Frag<TCS_PREC_SPLIT> : Frag<TCS_PREC_FP16> {};
 
-template <int TH, int TW, int ZLO, int ZHI, int PREC>
+template <int TH, int TW, int ZLO, int ZHI, int PREC, bool PERSIST>
 __global__ __launch_bounds__(kWG) void tcs_mfma_sweep(SweepArgs a) {
     constexpr int NPL = PREC == TCS_PREC_SPLIT ? 2 : 1;
     constexpr int NZ = ZHI - ZLO + 1;
@@ -174,13 +228,14 @@ __global__ __launch_bounds__(kWG) void tcs_mfma_sweep(SweepArgs a) {
     typedef typename F::type frag_t;
     __shared__ __attribute__((aligned(16))) uint16_t lds[NPL * T::PLANE];
 
-    const int tile = xcd_tile(a);
-    if (tile >= a.ntiles) return;
-    const int ty = tile / a.tiles_x, tx = tile - ty * a.tiles_x;
-    const int gr0 = 16 * TH * ty, gc0 = 16 * TW * tx;
+    Schedule sc = schedule(a);
+    if (sc.tile >= sc.end) return;
     const int tid = threadIdx.x;
-
-    stage_tile<TH, TW, NPL>(a, lds, gr0, gc0, tid);
+    int gr0, gc0;
+    tile_origin<TH, TW>(a, sc.tile, gr0, gc0);
+    TileRegs<TH, TW, NPL> R;
+    load_tile<TH, TW, NPL>(a, R, gr0, gc0, tid);
+    zero_pad<TH, TW, NPL>(lds, tid);
 
     // Coefficient fragments.  Lane l holds A[m][k] for m = l%16 and
     // k = 8*(l/16) + e, e = 0..7: the coefficient at row offset z and column
@@ -215,13 +270,21 @@ __global__ __launch_bounds__(kWG) void tcs_mfma_sweep(SweepArgs a) {
             }
         }
     }
-    __syncthreads();
-
-    const bool edge = !tile_is_interior(a, gr0, gc0, 16 * TH, 16 * TW);
     // Slot group 3 (k = 24..31) only meets zero coefficients: those lanes skip
     // the LDS read and feed zeros.
     const bool reads = g < 3;
     constexpr int NB = TH * TW / 4;
+
+    for (;;) {
+    write_tile<TH, TW, NPL>(R, lds, tid);
+    __syncthreads();
+    const int cgr0 = gr0, cgc0 = gc0;  // the tile in LDS
+    const int next = sc.tile + sc.step;
+    if (PERSIST && next < sc.end) {
+        tile_origin<TH, TW>(a, next, gr0, gc0);
+        load_tile<TH, TW, NPL>(a, R, gr0, gc0, tid);  // in flight during the compute below
+    }
+    const bool edge = !tile_is_interior(a, cgr0, cgc0, 16 * TH, 16 * TW);
 #pragma unroll
     for (int q = 0; q < NB; ++q) {
         const int b = wave + 4 * q;
@@ -244,8 +307,8 @@ __global__ __launch_bounds__(kWG) void tcs_mfma_sweep(SweepArgs a) {
             }
         }
         // Lane holds output row m, columns 4g..4g+3 of brick (br, bc).
-        const int gr = gr0 + 16 * br + m;
-        const int gcb = gc0 + 16 * bc + 4 * g;
+        const int gr = cgr0 + 16 * br + m;
+        const int gcb = cgc0 + 16 * bc + 4 * g;
         uint16_t o[NPL][4];
 #pragma unroll
         for (int v = 0; v < 4; ++v) {
@@ -273,8 +336,13 @@ __global__ __launch_bounds__(kWG) void tcs_mfma_sweep(SweepArgs a) {
 #pragma unroll
         for (int p = 0; p < NPL; ++p) {
             u32x2 w = {pack2(o[p][0], o[p][1]), pack2(o[p][2], o[p][3])};
-            *reinterpret_cast<u32x2 *>(a.out + p * a.plane + s) = w;
+            store8<true>(a.out + p * a.plane + s, w);
         }
+    }
+    // One-tile-per-workgroup launches cover every tile with the grid itself.
+    if (!PERSIST || next >= sc.end) break;
+    sc.tile = next;
+    __syncthreads();  // every wave is done reading LDS before it is overwritten
     }
 }
 
@@ -287,7 +355,7 @@ __host__ __device__ constexpr bool tap(int dz, int dj) {
     return true;
 }
 
-template <int TH, int TW, int STENCIL>
+template <int TH, int TW, int STENCIL, bool PERSIST>
 __global__ __launch_bounds__(kWG) void tcs_direct_sweep(SweepArgs a) {
     using T = Tile<TH, TW, 1>;
     constexpr int CG = 4 * TW;       // 4-column groups across the tile
@@ -296,17 +364,27 @@ __global__ __launch_bounds__(kWG) void tcs_direct_sweep(SweepArgs a) {
     static_assert(RG * CG == kWG && MR * RG == 16 * TH, "tile does not split into strips");
     __shared__ __attribute__((aligned(16))) uint16_t lds[T::PLANE];
 
-    const int tile = xcd_tile(a);
-    if (tile >= a.ntiles) return;
-    const int ty = tile / a.tiles_x, tx = tile - ty * a.tiles_x;
-    const int gr0 = 16 * TH * ty, gc0 = 16 * TW * tx;
+    Schedule sc = schedule(a);
+    if (sc.tile >= sc.end) return;
     const int tid = threadIdx.x;
-
-    stage_tile<TH, TW, 1>(a, lds, gr0, gc0, tid);
-    __syncthreads();
+    int gr0, gc0;
+    tile_origin<TH, TW>(a, sc.tile, gr0, gc0);
+    TileRegs<TH, TW, 1> R;
+    load_tile<TH, TW, 1>(a, R, gr0, gc0, tid);
+    zero_pad<TH, TW, 1>(lds, tid);
 
     const int cg = tid % CG, rg = tid / CG;
     const int r0 = rg * MR;  // first output row of this strip (tile coordinates)
+
+    for (;;) {
+    write_tile<TH, TW, 1>(R, lds, tid);
+    __syncthreads();
+    const int cgr0 = gr0, cgc0 = gc0;  // the tile in LDS
+    const int next = sc.tile + sc.step;
+    if (PERSIST && next < sc.end) {
+        tile_origin<TH, TW>(a, next, gr0, gc0);
+        load_tile<TH, TW, 1>(a, R, gr0, gc0, tid);  // in flight during the compute below
+    }
     float acc[MR][4];
 #pragma unroll
     for (int o = 0; o < MR; ++o)
@@ -348,12 +426,12 @@ __global__ __launch_bounds__(kWG) void tcs_direct_sweep(SweepArgs a) {
         }
     }
 
-    const bool edge = !tile_is_interior(a, gr0, gc0, 16 * TH, 16 * TW);
+    const bool edge = !tile_is_interior(a, cgr0, cgc0, 16 * TH, 16 * TW);
 #pragma unroll
     for (int o = 0; o < MR; ++o) {
         const int tr = r0 + o;                  // tile row
-        const int gr = gr0 + tr;
-        const int gcb = gc0 + 4 * cg;
+        const int gr = cgr0 + tr;
+        const int gcb = cgc0 + 4 * cg;
         uint16_t h[4];
 #pragma unroll
         for (int v = 0; v < 4; ++v) h[v] = f16_bits(acc[o][v]);
@@ -364,7 +442,12 @@ __global__ __launch_bounds__(kWG) void tcs_direct_sweep(SweepArgs a) {
                 if (!point_is_interior(a, gr, gcb + v)) h[v] = lds[cidx + v];
         }
         u32x2 w = {pack2(h[0], h[1]), pack2(h[2], h[3])};
-        *reinterpret_cast<u32x2 *>(a.out + stor_index(gr, gcb, a.sbc)) = w;
+        store8<false>(a.out + stor_index(gr, gcb, a.sbc), w);
+    }
+    // One-tile-per-workgroup launches cover every tile with the grid itself.
+    if (!PERSIST || next >= sc.end) break;
+    sc.tile = next;
+    __syncthreads();  // every wave is done reading LDS before it is overwritten
     }
 }
 
@@ -373,39 +456,45 @@ typedef void (*sweep_fn)(SweepArgs);
 
 struct KernelEntry {
     int method, prec, radius, stencil, th, tw;  // stencil = -1: any stencil of that radius
+    bool persist;
     sweep_fn fn;
     const char *name;  // as rocprofv3 reports it (demangled, without the namespace)
 };
 
-#define TCS_MFMA(TH, TW, ZLO, ZHI, P, R) \
-    {TCS_METHOD_MFMA, P, R, -1, TH, TW, tcs_mfma_sweep<TH, TW, ZLO, ZHI, P>, \
-     "tcs_mfma_sweep<" #TH ", " #TW ", " #ZLO ", " #ZHI ", " #P ">"}
-#define TCS_MFMA_TILES(ZLO, ZHI, P, R) \
-    TCS_MFMA(2, 4, ZLO, ZHI, P, R), TCS_MFMA(4, 4, ZLO, ZHI, P, R), TCS_MFMA(2, 8, ZLO, ZHI, P, R), \
-    TCS_MFMA(4, 8, ZLO, ZHI, P, R)
-#define TCS_DIRECT(TH, TW, S) \
-    {TCS_METHOD_DIRECT, TCS_PREC_FP16, -1, S, TH, TW, tcs_direct_sweep<TH, TW, S>, \
-     "tcs_direct_sweep<" #TH ", " #TW ", " #S ">"}
-#define TCS_DIRECT_TILES(S) \
-    TCS_DIRECT(2, 4, S), TCS_DIRECT(4, 4, S), TCS_DIRECT(2, 8, S), TCS_DIRECT(4, 8, S), TCS_DIRECT(8, 8, S)
+#define TCS_MFMA(TH, TW, ZLO, ZHI, P, R, PS) \
+    {TCS_METHOD_MFMA, P, R, -1, TH, TW, PS, tcs_mfma_sweep<TH, TW, ZLO, ZHI, P, PS>, \
+     "tcs_mfma_sweep<" #TH ", " #TW ", " #ZLO ", " #ZHI ", " #P ", " #PS ">"}
+#define TCS_MFMA_TILES(ZLO, ZHI, P, R, PS) \
+    TCS_MFMA(2, 4, ZLO, ZHI, P, R, PS), TCS_MFMA(4, 4, ZLO, ZHI, P, R, PS), TCS_MFMA(2, 8, ZLO, ZHI, P, R, PS), \
+    TCS_MFMA(4, 8, ZLO, ZHI, P, R, PS)
+#define TCS_MFMA_ALL(PS)                                                                  \
+    TCS_MFMA_TILES(0, 4, 0, 2, PS), TCS_MFMA(8, 8, 0, 4, 0, 2, PS), TCS_MFMA_TILES(1, 3, 0, 1, PS), \
+    TCS_MFMA(8, 8, 1, 3, 0, 1, PS), TCS_MFMA_TILES(0, 4, 1, 2, PS), TCS_MFMA(8, 8, 0, 4, 1, 2, PS), \
+    TCS_MFMA_TILES(1, 3, 1, 1, PS), TCS_MFMA(8, 8, 1, 3, 1, 1, PS), TCS_MFMA_TILES(0, 4, 2, 2, PS), \
+    TCS_MFMA_TILES(1, 3, 2, 1, PS)
+#define TCS_DIRECT(TH, TW, S, PS) \
+    {TCS_METHOD_DIRECT, TCS_PREC_FP16, -1, S, TH, TW, PS, tcs_direct_sweep<TH, TW, S, PS>, \
+     "tcs_direct_sweep<" #TH ", " #TW ", " #S ", " #PS ">"}
+#define TCS_DIRECT_TILES(S, PS) \
+    TCS_DIRECT(2, 4, S, PS), TCS_DIRECT(4, 4, S, PS), TCS_DIRECT(2, 8, S, PS), TCS_DIRECT(4, 8, S, PS), \
+    TCS_DIRECT(8, 8, S, PS)
 
-// Tile shapes offered (bricks, TH x TW).  SPLIT has no 8x8 tile: two planes
-// of 132 x 136 elements would need 71.8 KB of LDS.
+// Tile shapes offered (bricks, TH x TW), each one-tile-per-workgroup (false)
+// and persistent (true).  SPLIT has no 8x8 tile: two planes of 132 x 136
+// elements would need 71.8 KB of LDS.
 static const KernelEntry kKernels[] = {
-    TCS_MFMA_TILES(0, 4, 0, 2), TCS_MFMA(8, 8, 0, 4, 0, 2),
-    TCS_MFMA_TILES(1, 3, 0, 1), TCS_MFMA(8, 8, 1, 3, 0, 1),
-    TCS_MFMA_TILES(0, 4, 1, 2), TCS_MFMA(8, 8, 0, 4, 1, 2),
-    TCS_MFMA_TILES(1, 3, 1, 1), TCS_MFMA(8, 8, 1, 3, 1, 1),
-    TCS_MFMA_TILES(0, 4, 2, 2),
-    TCS_MFMA_TILES(1, 3, 2, 1),
-    TCS_DIRECT_TILES(0), TCS_DIRECT_TILES(1), TCS_DIRECT_TILES(2), TCS_DIRECT_TILES(3),
+    TCS_MFMA_ALL(false), TCS_MFMA_ALL(true),
+    TCS_DIRECT_TILES(0, false), TCS_DIRECT_TILES(1, false), TCS_DIRECT_TILES(2, false), TCS_DIRECT_TILES(3, false),
+    TCS_DIRECT_TILES(0, true), TCS_DIRECT_TILES(1, true), TCS_DIRECT_TILES(2, true), TCS_DIRECT_TILES(3, true),
 };
 static_assert(TCS_PREC_FP16 == 0 && TCS_PREC_BF16 == 1 && TCS_PREC_SPLIT == 2, "precision ids");
 static_assert(TCS_STAR2D1R == 0 && TCS_STAR2D2R == 1 && TCS_BOX2D1R == 2 && TCS_BOX2D2R == 3, "stencil ids");
 
 static const KernelEntry *select_kernel(const tcs_problem *p) {
     for (const KernelEntry &k : kKernels) {
-        if (k.method != p->method || k.prec != p->precision || k.th != p->tile_h || k.tw != p->tile_w) continue;
+        if (k.method != p->method || k.prec != p->precision || k.th != p->tile_h || k.tw != p->tile_w ||
+            k.persist != (p->wg_per_cu > 0))
+            continue;
         if (k.method == TCS_METHOD_MFMA ? k.radius == p->radius : k.stencil == p->stencil) return &k;
     }
     return nullptr;
@@ -448,7 +537,18 @@ int launch_sweep(const tcs_problem *p, const void *d_in, void *d_out, hipStream_
         a.ch[i] = p->coeff_hi[i];
         a.cl[i] = p->coeff_lo[i];
     }
-    const unsigned grid = (unsigned)(a.tiles_per_xcd * kXCD);
+    // One workgroup per tile, or (wg_per_cu > 0) a persistent grid of wg_per_cu
+    // workgroups per CU; either way a multiple of 8 so every XCD gets the same.
+    int per_xcd = a.tiles_per_xcd;
+    if (k->persist) {
+        int dev = 0, cus = 0;
+        TCS_HIP_CHECK(hipGetDevice(&dev));
+        TCS_HIP_CHECK(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+        per_xcd = cus * p->wg_per_cu / kXCD;
+        if (per_xcd < 1) per_xcd = 1;
+        if (per_xcd > a.tiles_per_xcd) per_xcd = a.tiles_per_xcd;
+    }
+    const unsigned grid = (unsigned)(per_xcd * kXCD);
     hipLaunchKernelGGL(k->fn, dim3(grid), dim3(kWG), 0, stream, a);
     TCS_HIP_CHECK(hipGetLastError());
     return TCS_OK;

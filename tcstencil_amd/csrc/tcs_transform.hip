@@ -115,17 +115,21 @@ __global__ void tcs_fill_kernel(u32x4 *__restrict__ p, int64_t n16) {
 
 // Streaming copy: the measured HBM ceiling next to the 8 TB/s spec figure, and
 // the known-size kernel that calibrates the FETCH_SIZE / WRITE_SIZE counters.
+// One pass, four 16-byte chunks per thread (all loads before the stores),
+// nontemporal loads and stores.
 __global__ void tcs_copy_kernel(const u32x4 *__restrict__ src, u32x4 *__restrict__ dst, int64_t n16) {
-    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
-    int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-    for (; t + 3 * stride < n16; t += 4 * stride) {
-        const u32x4 a = src[t], b = src[t + stride], c = src[t + 2 * stride], d = src[t + 3 * stride];
-        dst[t] = a;
-        dst[t + stride] = b;
-        dst[t + 2 * stride] = c;
-        dst[t + 3 * stride] = d;
+    const int64_t base = (int64_t)blockIdx.x * blockDim.x * 4 + threadIdx.x;
+    u32x4 v[4];
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+        const int64_t t = base + (int64_t)i * blockDim.x;
+        if (t < n16) v[i] = __builtin_nontemporal_load(src + t);
     }
-    for (; t < n16; t += stride) dst[t] = src[t];
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+        const int64_t t = base + (int64_t)i * blockDim.x;
+        if (t < n16) __builtin_nontemporal_store(v[i], dst + t);
+    }
 }
 
 __global__ void tcs_zero_u64_kernel(unsigned long long *p) { *p = 0ull; }
@@ -279,8 +283,8 @@ extern "C" int tcs_copy(void *d_dst, const void *d_src, int64_t bytes, void *str
         return set_error(TCS_ERR_INVALID, "copy: null pointer or size not a multiple of 16");
     const int64_t n16 = bytes / 16;
     if (n16 == 0) return TCS_OK;
-    int64_t g = (n16 + 4 * 256 - 1) / (4 * 256);
-    if (g > 8192) g = 8192;
+    const int64_t g = (n16 + 4 * 256 - 1) / (4 * 256);
+    if (g > 0x7fffffff) return set_error(TCS_ERR_INVALID, "copy: too large");
     hipLaunchKernelGGL(tcs_copy_kernel, dim3((unsigned)g), dim3(256), 0, (hipStream_t)stream, (const u32x4 *)d_src,
                        (u32x4 *)d_dst, n16);
     TCS_HIP_CHECK(hipGetLastError());

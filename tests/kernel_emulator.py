@@ -153,20 +153,25 @@ def stage_tile(inp: np.ndarray, L: Layout, G: TileGeom, gr0: int, gc0: int) -> n
     return lds
 
 
-def _tiles(L: Layout, TH: int, TW: int):
+def _tiles(L: Layout, TH: int, TW: int, per_xcd: int = 0):
+    """The persistent schedule of tcs_sweep.hip: grid = 8 * per_xcd workgroups;
+    workgroup w on XCD x = w % 8 walks tiles x*tpx + w//8, + per_xcd, ... below
+    min((x+1)*tpx, ntiles).  per_xcd = 0 means one workgroup per tile."""
     tiles_x = L.nbc // TW
     ntiles = (L.nbr // TH) * tiles_x
     tpx = -(-ntiles // XCD)
-    seen = set()
-    for wg in range(tpx * XCD):
-        tile = (wg % XCD) * tpx + wg // XCD
-        if tile >= ntiles:
-            continue
-        assert tile not in seen
-        seen.add(tile)
-        ty, tx = tile // tiles_x, tile % tiles_x
-        yield 16 * TH * ty, 16 * TW * tx
-    assert len(seen) == ntiles
+    g = tpx if per_xcd <= 0 else min(per_xcd, tpx)
+    seen = []
+    for wg in range(g * XCD):
+        x, k = wg % XCD, wg // XCD
+        end = min(x * tpx + tpx, ntiles)
+        tile = x * tpx + k
+        while tile < end:
+            seen.append(tile)
+            ty, tx = tile // tiles_x, tile % tiles_x
+            yield 16 * TH * ty, 16 * TW * tx
+            tile += g
+    assert sorted(seen) == list(range(ntiles)), "schedule does not cover every tile exactly once"
 
 
 def _interior(gr, gc, L: Layout, rs: int):
@@ -174,7 +179,7 @@ def _interior(gr, gc, L: Layout, rs: int):
 
 
 # ------------------------------------------------------------- MFMA sweep
-def mfma_sweep(inp, L: Layout, TH, TW, coeff5, rs, round_coeff=f16):
+def mfma_sweep(inp, L: Layout, TH, TW, coeff5, rs, round_coeff=f16, per_xcd=0):
     """tcs_mfma_sweep for FP16 (one plane).  coeff5: 5x5 FP32 embedding."""
     assert L.nbr % TH == 0 and L.nbc % TW == 0
     G = TileGeom(TH, TW)
@@ -191,7 +196,7 @@ def mfma_sweep(inp, L: Layout, TH, TW, coeff5, rs, round_coeff=f16):
             d = 8 * g + e - m
             ok = (d >= 0) & (d <= 4)
             afrag[zi, ok, e] = round_coeff(coeff5[ZLO + zi, d[ok]])
-    for gr0, gc0 in _tiles(L, TH, TW):
+    for gr0, gc0 in _tiles(L, TH, TW, per_xcd):
         lds = stage_tile(inp, L, G, gr0, gc0)
         edge = not (gr0 >= rs and gr0 + 16 * TH <= L.rows - rs and gc0 >= rs and gc0 + 16 * TW <= L.cols - rs)
         for wave in range(4):
@@ -244,7 +249,7 @@ def _taps(stencil):
     return t
 
 
-def direct_sweep(inp, L: Layout, TH, TW, coeff5, stencil, rs):
+def direct_sweep(inp, L: Layout, TH, TW, coeff5, stencil, rs, per_xcd=0):
     G = TileGeom(TH, TW)
     CG = 4 * TW
     RG = WG // CG
@@ -256,7 +261,7 @@ def direct_sweep(inp, L: Layout, TH, TW, coeff5, stencil, rs):
     tid = np.arange(WG)
     cg, rg = tid % CG, tid // CG
     r0 = rg * MR
-    for gr0, gc0 in _tiles(L, TH, TW):
+    for gr0, gc0 in _tiles(L, TH, TW, per_xcd):
         lds = stage_tile(inp, L, G, gr0, gc0)
         acc = np.zeros((WG, MR, 4))
         for ir in range(MR + 4):

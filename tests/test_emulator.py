@@ -39,10 +39,11 @@ def test_emulated_kernel_exact(oracle, stencil, case, method):
     A, B, c5 = dyadic_problem(stencil, rows, cols, seed=rows * 1000 + cols)
     L = E.Layout(rows, cols, max(TH, TW))
     st = E.pack(A, L)
+    per_xcd = (rows + cols) % 3  # 0 (one tile per workgroup), 1 or 2 persistent workgroups per XCD
     if method == "mfma":
-        out, written = E.mfma_sweep(st, L, TH, TW, c5, r)
+        out, written = E.mfma_sweep(st, L, TH, TW, c5, r, per_xcd=per_xcd)
     else:
-        out, written = E.direct_sweep(st, L, TH, TW, c5, stencil, r)
+        out, written = E.direct_sweep(st, L, TH, TW, c5, stencil, r, per_xcd=per_xcd)
     # every data brick is written, the ghost ring never
     data = np.zeros(L.plane, dtype=bool)
     for bi in range(L.nbr):
@@ -81,3 +82,12 @@ def test_storage_index_is_the_reference_brick_map():
     mine = E.stor_index(padded_r - 16, padded_c - 16, L.sbc)
     assert L.sbc == nb and L.plane == R * R
     assert np.array_equal(mine, ref_out)
+
+
+@pytest.mark.parametrize("ntile_shape", [(9, 4, 2, 4), (64, 64, 2, 4), (3, 200, 1, 8)])
+def test_persistent_schedule_covers_every_tile(ntile_shape):
+    rows_b, cols_b, TH, TW = ntile_shape
+    L = E.Layout(16 * rows_b, 16 * cols_b, max(TH, TW))
+    for per_xcd in (0, 1, 2, 3, 7, 1000):
+        origins = list(E._tiles(L, TH, TW, per_xcd))
+        assert len(origins) == len(set(origins)) == (L.nbr // TH) * (L.nbc // TW)
