@@ -138,16 +138,27 @@ static bool tile_offered(int th, int tw) {
            (th == 8 && tw == 8);
 }
 
-// Tuned default tile per grid size (bricks per workgroup, rows x cols).
-static void default_tile(int rows, int cols, int *th, int *tw) {
-    const int br = (rows + 15) / 16, bc = (cols + 15) / 16;
-    if (br >= 4 && bc >= 8) {
-        *th = 4;
-        *tw = 8;
-    } else {
-        *th = 2;
-        *tw = 4;
+// Default tile: the fastest shape measured on 16384^2 grids for this method,
+// precision and radius, stepped down until the grid has at least 1024 tiles
+// (4 per CU), so that small grids still fill the chip.
+static void default_tile(const tcs_problem *p, int *th, int *tw) {
+    static const int kPref[5][2] = {{8, 8}, {4, 8}, {4, 4}, {2, 8}, {2, 4}};
+    int first;
+    if (p->method == TCS_METHOD_MFMA)
+        first = (p->precision != TCS_PREC_SPLIT && p->radius == 2) ? 0 : 1;
+    else
+        first = p->stencil == TCS_BOX2D2R ? 1 : 4;
+    const long br = (p->layout.rows + 15) / 16, bc = (p->layout.cols + 15) / 16;
+    int pick = 4;
+    for (int i = first; i < 5; ++i) {
+        const long tiles = ((br + kPref[i][0] - 1) / kPref[i][0]) * ((bc + kPref[i][1] - 1) / kPref[i][1]);
+        if (tiles >= 1024) {
+            pick = i;
+            break;
+        }
     }
+    *th = kPref[pick][0];
+    *tw = kPref[pick][1];
 }
 
 extern "C" int tcs_problem_set_tile(tcs_problem *p, int tile_h, int tile_w) {
@@ -206,7 +217,7 @@ extern "C" int tcs_problem_init(tcs_problem *p, int stencil, int method, int pre
         }
     }
     int th, tw;
-    default_tile(rows, cols, &th, &tw);
+    default_tile(p, &th, &tw);
     return tcs_problem_set_tile(p, th, tw);
 }
 

@@ -204,11 +204,21 @@ def mfma_sweep(inp, L: Layout, TH, TW, coeff5, rs, round_coeff=f16, per_xcd=0):
                 b = wave + 4 * q
                 br, bc = b // TW, b % TW
                 D = np.zeros((16, 16))
+                bfrag = np.zeros((64, 8))
                 for zi in range(ZHI - ZLO + 1):
                     off = (16 * br + m + ZLO + zi) * G.LP + 16 * bc + 8 * g
-                    _check(off[reads], G.PLANE - 7, 8, "B operand LDS read (16 B)")
-                    bfrag = np.zeros((64, 8))
-                    bfrag[reads] = lds[off[reads][:, None] + np.arange(8)[None, :]]  # [lane][e]
+                    fetch = reads & ((zi == 0) | (m == 15))
+                    if zi > 0:  # DPP row_shl:1 within each 16-lane row; lane 15 gets 0
+                        shifted = np.zeros((64, 8))
+                        for l in range(64):
+                            if l % 16 != 15:
+                                shifted[l] = bfrag[l + 1]
+                        bfrag = shifted
+                    _check(off[fetch], G.PLANE - 7, 8, "B operand LDS read (16 B)")
+                    bfrag[fetch] = lds[off[fetch][:, None] + np.arange(8)[None, :]]  # [lane][e]
+                    direct = np.zeros((64, 8))
+                    direct[reads] = lds[off[reads][:, None] + np.arange(8)[None, :]]
+                    assert np.array_equal(bfrag, direct, equal_nan=True), "shifted operand differs from LDS rows"
                     Am = np.zeros((16, 32))
                     Bm = np.zeros((32, 16))
                     for l in range(64):
