@@ -82,6 +82,9 @@ typedef struct tcs_problem {
     int32_t coeff_exp;       /* SPLIT: coefficient scale exponent */
     int32_t wg_per_cu;       /* 0: one workgroup per tile; n > 0: a persistent grid
                                 of n workgroups per CU that prefetch their next tile */
+    int32_t fuse;            /* sweeps fused per launch by tcs_run (temporal blocking,
+                                MFMA only): 1..1+16/radius; 0 or 1 = unfused */
+    int32_t reserved;
 } tcs_problem;
 
 /* ---------------------------------------------------------- status, params */
@@ -166,8 +169,11 @@ int tcs_split_scale_exp(double maxabs);
 /* One sweep d_in -> d_out (distinct storage buffers of this problem; d_out's
  * ghost ring must be zero, e.g. from tcs_clear or a previous pack). */
 int tcs_sweep(const tcs_problem *p, const void *d_in, void *d_out, void *stream);
-/* `sweeps` ping-pong sweeps between d_buf0 and d_buf1: sweep s reads
- * buf[s % 2] and writes buf[(s+1) % 2], so the result is in buf[sweeps % 2]. */
+/* `sweeps` ping-pong sweeps between d_buf0 and d_buf1; the result is in
+ * buf[sweeps % 2].  Unfused, sweep s reads buf[s % 2] and writes buf[(s+1) % 2].
+ * With p->fuse = k > 1 the sweeps run as temporally blocked launches of at most
+ * k sweeps each (LDS-resident, bitwise equal to unfused sweeps), balanced so
+ * that the launch count has the parity of `sweeps`. */
 int tcs_run(const tcs_problem *p, void *d_buf0, void *d_buf1, int sweeps, void *stream);
 /* Capture tcs_run into a hipGraphExec (returned as void*) and replay it. */
 int tcs_graph_create(const tcs_problem *p, void *d_buf0, void *d_buf1, int sweeps, void *stream,
@@ -212,6 +218,8 @@ int tcs_gen_coeffs(int stencil, uint64_t seed, float *B, int n);
 /* ------------------------------------------------------------ introspection */
 /* Kernel name (as rocprofv3 reports it) used for this problem, or NULL. */
 const char *tcs_kernel_name(const tcs_problem *p);
+/* Kernel name of the fused (temporally blocked) launch for this problem, or NULL. */
+const char *tcs_fused_kernel_name(const tcs_problem *p);
 /* MFMA instructions issued per 16x16 brick (0 for the direct method). */
 int tcs_mfma_per_brick(const tcs_problem *p);
 

@@ -40,7 +40,8 @@ class ProblemStruct(ctypes.Structure):
                 ("coeff", ctypes.c_float * 25),
                 ("coeff_hi", ctypes.c_uint16 * 25),
                 ("coeff_lo", ctypes.c_uint16 * 25),
-                ("coeff_exp", ctypes.c_int32), ("wg_per_cu", ctypes.c_int32)]
+                ("coeff_exp", ctypes.c_int32), ("wg_per_cu", ctypes.c_int32),
+                ("fuse", ctypes.c_int32), ("reserved", ctypes.c_int32)]
 
 
 class Params(ctypes.Structure):
@@ -101,6 +102,7 @@ SIGNATURES = {
     "tcs_round_f32": (_i, [_vp, _i64, _i, _vp]),
     "tcs_gen_coeffs": (_i, [_i, _u64, _fp, _i]),
     "tcs_kernel_name": (ctypes.c_char_p, [_prob]),
+    "tcs_fused_kernel_name": (ctypes.c_char_p, [_prob]),
     "tcs_mfma_per_brick": (_i, [_prob]),
 }
 

@@ -50,7 +50,7 @@ class Problem:
     """A tcs_problem descriptor: stencil, method, precision, grid, coefficients."""
 
     def __init__(self, stencil, rows: int, cols: int, B, method="mfma", precision="fp16",
-                 tile=None, wg_per_cu: int = 0):
+                 tile=None, wg_per_cu: int = 0, fuse: int = 1):
         self.stencil = stencil_id(stencil)
         self.method = L.METHOD_ID[method] if isinstance(method, str) else int(method)
         self.precision = L.PREC_ID[precision] if isinstance(precision, str) else int(precision)
@@ -62,6 +62,7 @@ class Problem:
         if tile is not None:
             self.set_tile(*tile)
         self.s.wg_per_cu = wg_per_cu
+        self.s.fuse = fuse
 
     # -- descriptor ---------------------------------------------------------
     def set_tile(self, tile_h: int, tile_w: int) -> None:
@@ -90,6 +91,19 @@ class Problem:
     @wg_per_cu.setter
     def wg_per_cu(self, n: int) -> None:
         self.s.wg_per_cu = int(n)
+
+    @property
+    def fuse(self) -> int:
+        return self.s.fuse
+
+    @fuse.setter
+    def fuse(self, k: int) -> None:
+        self.s.fuse = int(k)
+
+    @property
+    def fused_kernel_name(self) -> str:
+        n = lib().tcs_fused_kernel_name(self.ref)
+        return n.decode() if n else ""
 
     @property
     def tile(self):

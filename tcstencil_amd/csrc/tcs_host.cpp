@@ -234,6 +234,7 @@ extern "C" int tcs_split_scale_exp(double maxabs) {
 }
 
 extern "C" const char *tcs_kernel_name(const tcs_problem *p) { return p ? sweep_kernel_name(p) : nullptr; }
+extern "C" const char *tcs_fused_kernel_name(const tcs_problem *p) { return p ? fused_kernel_name(p) : nullptr; }
 extern "C" int tcs_mfma_per_brick(const tcs_problem *p) { return p ? mfma_per_brick(p) : 0; }
 
 // ---------------------------------------------------------------- sweeping
@@ -245,10 +246,29 @@ extern "C" int tcs_sweep(const tcs_problem *p, const void *d_in, void *d_out, vo
 }
 
 extern "C" int tcs_run(const tcs_problem *p, void *d_buf0, void *d_buf1, int sweeps, void *stream) {
+    if (!p) return set_error(TCS_ERR_INVALID, "run: null problem");
     if (sweeps < 0) return set_error(TCS_ERR_INVALID, "run: negative sweep count");
     void *buf[2] = {d_buf0, d_buf1};
-    for (int s = 0; s < sweeps; ++s) {
-        int st = tcs_sweep(p, buf[s & 1], buf[(s + 1) & 1], stream);
+    const int fuse = p->fuse > 1 ? p->fuse : 1;
+    if (fuse == 1) {
+        for (int s = 0; s < sweeps; ++s) {
+            int st = tcs_sweep(p, buf[s & 1], buf[(s + 1) & 1], stream);
+            if (st) return st;
+        }
+        return TCS_OK;
+    }
+    if (p->method != TCS_METHOD_MFMA) return set_error(TCS_ERR_UNSUPPORTED, "run: fused sweeps need the MFMA method");
+    if (p->radius * (fuse - 1) > 16)
+        return set_error(TCS_ERR_INVALID, "run: fuse %d exceeds 1 + 16/radius = %d", fuse, 1 + 16 / p->radius);
+    if (!d_buf0 || !d_buf1 || d_buf0 == d_buf1) return set_error(TCS_ERR_INVALID, "run: need two distinct buffers");
+    if (sweeps == 0) return TCS_OK;
+    // Launch count with the parity of `sweeps` (result in buf[sweeps % 2]),
+    // sweeps spread evenly: each launch gets at most `fuse`.
+    int nl = (sweeps + fuse - 1) / fuse;
+    if ((nl & 1) != (sweeps & 1)) ++nl;
+    const int base = sweeps / nl, extra = sweeps % nl;
+    for (int i = 0; i < nl; ++i) {
+        int st = launch_fused(p, buf[i & 1], buf[(i + 1) & 1], base + (i < extra ? 1 : 0), (hipStream_t)stream);
         if (st) return st;
     }
     return TCS_OK;

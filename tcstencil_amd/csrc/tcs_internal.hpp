@@ -32,7 +32,7 @@ struct SweepArgs {
     uint16_t *out;
     int64_t plane;         // elements per storage plane (SPLIT: lo plane offset)
     int32_t rows, cols;    // logical grid
-    int32_t sbc;           // stored bricks per brick row
+    int32_t sbr, sbc;      // stored brick rows, stored bricks per brick row
     int32_t tiles_x;       // tiles per tile row
     int32_t ntiles;        // tiles in the grid
     int32_t tiles_per_xcd; // ceil(ntiles / 8)
@@ -44,7 +44,9 @@ struct SweepArgs {
 
 // Kernel launchers (tcs_sweep.hip).
 int launch_sweep(const tcs_problem *p, const void *d_in, void *d_out, hipStream_t stream);
+int launch_fused(const tcs_problem *p, const void *d_in, void *d_out, int nsweeps, hipStream_t stream);
 const char *sweep_kernel_name(const tcs_problem *p);
+const char *fused_kernel_name(const tcs_problem *p);
 int mfma_per_brick(const tcs_problem *p);
 
 // Error reporting (tcs_host.cpp).

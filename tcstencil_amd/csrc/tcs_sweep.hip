@@ -41,121 +41,130 @@ __device__ __forceinline__ void store8(uint16_t *p, u32x2 w) {
 }
 
 // ------------------------------------------------------------ tile staging
-template <int TH, int TW, int NPL>
-struct Tile {
-    static constexpr int LR = 16 * TH + 4;      // LDS rows (2 halo rows each side)
-    static constexpr int LP = 16 * TW + 8;      // LDS pitch in elements; 2*LP = 16 (mod 32) bytes
-    static constexpr int NCH = TH * TW * 32;    // 16-byte chunks of the data bricks
-    static constexpr int CPT = NCH / kWG;       // chunks per thread
-    static constexpr int HD = 8 * TW + 2;       // dwords in one top/bottom halo row
-    static constexpr int NH = 4 * HD + 32 * TH; // halo dwords: 2 top + 2 bottom rows, left, right
+// LDS image of BH x BW data bricks plus a 2-point frame on every side.
+template <int BH_, int BW_, int NPL_>
+struct Geom {
+    static constexpr int BH = BH_, BW = BW_, NPL = NPL_;
+    static constexpr int LR = 16 * BH + 4;      // LDS rows (2 frame rows each side)
+    static constexpr int LP = 16 * BW + 8;      // LDS pitch in elements; 2*LP = 16 (mod 32) bytes
+    static constexpr int NCH = BH * BW * 32;    // 16-byte chunks of the data bricks
+    static constexpr int CPT = (NCH + kWG - 1) / kWG;
+    static constexpr int HD = 8 * BW + 2;       // dwords in one top/bottom frame row
+    static constexpr int NH = 4 * HD + 32 * BH; // frame dwords: 2 top + 2 bottom rows, left, right
     static constexpr int HPT = (NH + kWG - 1) / kWG;
     static constexpr int PLANE = LR * LP;       // LDS elements per plane
-    static_assert(NCH % kWG == 0, "tile must hold a multiple of 8 bricks");
 };
+
+template <int TH, int TW, int NPL>
+using Tile = Geom<TH, TW, NPL>;
 
 // One tile's global loads, held in registers between issue and the LDS write
-// so that the next tile's loads are in flight while the current one computes.
-template <int TH, int TW, int NPL>
+// so that the next tile's loads can be in flight while the current one computes.
+template <class G>
 struct TileRegs {
-    using T = Tile<TH, TW, NPL>;
-    u32x4 cv[NPL][T::CPT];
-    uint32_t hv[NPL][T::HPT];
+    u32x4 cv[G::NPL][G::CPT];
+    uint32_t hv[G::NPL][G::HPT];
 };
 
-// Halo frame dword h -> (LDS row, LDS dword column): two rows above, two rows
-// below (HD dwords each), then the left and right 2-column strips.
-template <int TH, int TW>
+// Frame dword h -> (LDS row, LDS dword column): two rows above, two rows below
+// (HD dwords each), then the left and right 2-column strips.
+template <class G>
 __device__ __forceinline__ void halo_coords(int h, int &lr, int &ld) {
-    constexpr int HD = 8 * TW + 2;
+    constexpr int HD = G::HD;
     if (h < 2 * HD) {
         lr = h / HD;
         ld = h - lr * HD;
     } else if (h < 4 * HD) {
         const int q = h - 2 * HD;
         const int rr = q / HD;
-        lr = 16 * TH + 2 + rr;
+        lr = 16 * G::BH + 2 + rr;
         ld = q - rr * HD;
-    } else if (h < 4 * HD + 16 * TH) {
+    } else if (h < 4 * HD + 16 * G::BH) {
         lr = 2 + (h - 4 * HD);
         ld = 0;
     } else {
-        lr = 2 + (h - 4 * HD - 16 * TH);
+        lr = 2 + (h - 4 * HD - 16 * G::BH);
         ld = HD - 1;
     }
 }
 
-// LDS element (lr, lc) of plane p holds global point (gr0 - 2 + lr, gc0 - 2 + lc)
-// for lr < LR, lc < 16*TW + 4; columns [16*TW+4, LP) hold zeros.
-template <int TH, int TW, int NPL>
-__device__ __forceinline__ void load_tile(const SweepArgs &a, TileRegs<TH, TW, NPL> &R, int gr0, int gc0,
-                                          int tid) {
-    using T = Tile<TH, TW, NPL>;
+// LDS element (lr, lc) of plane p holds global point (r0 - 2 + lr, c0 - 2 + lc)
+// for lr < LR, lc < 16*BW + 4; columns [16*BW+4, LP) hold zeros.  (r0, c0) is
+// the first element of the first data brick.  With CHECK, frame points outside
+// the stored bricks (possible when the data bricks include the ghost ring) read
+// as zero.
+template <class G, bool CHECK>
+__device__ __forceinline__ void load_tile(const SweepArgs &a, TileRegs<G> &R, int r0, int c0, int tid) {
 #pragma unroll
-    for (int i = 0; i < T::CPT; ++i) {
+    for (int i = 0; i < G::CPT; ++i) {
         const int c = tid + i * kWG;
-        const int b = c >> 5, w = c & 31;
-        const int br = b / TW, bc = b - br * TW;
-        const int64_t s = stor_index(gr0 + 16 * br + (w >> 1), gc0 + 16 * bc + 8 * (w & 1), a.sbc);
+        if (G::NCH % kWG == 0 || c < G::NCH) {
+            const int b = c >> 5, w = c & 31;
+            const int br = b / G::BW, bc = b - br * G::BW;
+            const int64_t s = stor_index(r0 + 16 * br + (w >> 1), c0 + 16 * bc + 8 * (w & 1), a.sbc);
 #pragma unroll
-        for (int p = 0; p < NPL; ++p)
-            R.cv[p][i] = *reinterpret_cast<const u32x4 *>(a.in + p * a.plane + s);
+            for (int p = 0; p < G::NPL; ++p)
+                R.cv[p][i] = *reinterpret_cast<const u32x4 *>(a.in + p * a.plane + s);
+        }
     }
 #pragma unroll
-    for (int i = 0; i < T::HPT; ++i) {
+    for (int i = 0; i < G::HPT; ++i) {
         const int h = tid + i * kWG;
-        if (h < T::NH) {
+        if (h < G::NH) {
             int lr, ld;
-            halo_coords<TH, TW>(h, lr, ld);
-            const int64_t s = stor_index(gr0 - 2 + lr, gc0 - 2 + 2 * ld, a.sbc);
+            halo_coords<G>(h, lr, ld);
+            const int gr = r0 - 2 + lr, gc = c0 - 2 + 2 * ld;
+            const bool inside = !CHECK || ((unsigned)((gr + 16) >> 4) < (unsigned)a.sbr &&
+                                           (unsigned)((gc + 16) >> 4) < (unsigned)a.sbc);
+            const int64_t s = stor_index(gr, gc, a.sbc);
 #pragma unroll
-            for (int p = 0; p < NPL; ++p)
-                R.hv[p][i] = *reinterpret_cast<const uint32_t *>(a.in + p * a.plane + s);
+            for (int p = 0; p < G::NPL; ++p)
+                R.hv[p][i] = inside ? *reinterpret_cast<const uint32_t *>(a.in + p * a.plane + s) : 0u;
         }
     }
 }
 
-template <int TH, int TW, int NPL>
-__device__ __forceinline__ void write_tile(const TileRegs<TH, TW, NPL> &R, uint16_t *lds, int tid) {
-    using T = Tile<TH, TW, NPL>;
+template <class G>
+__device__ __forceinline__ void write_tile(const TileRegs<G> &R, uint16_t *lds, int tid) {
     uint32_t *l32 = reinterpret_cast<uint32_t *>(lds);
 #pragma unroll
-    for (int i = 0; i < T::CPT; ++i) {
+    for (int i = 0; i < G::CPT; ++i) {
         const int c = tid + i * kWG;
-        const int b = c >> 5, w = c & 31;
-        const int br = b / TW, bc = b - br * TW;
-        const int lr = 16 * br + (w >> 1) + 2;
-        const int lc = 16 * bc + 8 * (w & 1) + 2;
-        const int d = (lr * T::LP + lc) >> 1;  // lc even: dword aligned, not 16-byte aligned
+        if (G::NCH % kWG == 0 || c < G::NCH) {
+            const int b = c >> 5, w = c & 31;
+            const int br = b / G::BW, bc = b - br * G::BW;
+            const int lr = 16 * br + (w >> 1) + 2;
+            const int lc = 16 * bc + 8 * (w & 1) + 2;
+            const int d = (lr * G::LP + lc) >> 1;  // lc even: dword aligned, not 16-byte aligned
 #pragma unroll
-        for (int p = 0; p < NPL; ++p) {
+            for (int p = 0; p < G::NPL; ++p) {
 #pragma unroll
-            for (int q = 0; q < 4; ++q) l32[p * (T::PLANE / 2) + d + q] = R.cv[p][i][q];
+                for (int q = 0; q < 4; ++q) l32[p * (G::PLANE / 2) + d + q] = R.cv[p][i][q];
+            }
         }
     }
 #pragma unroll
-    for (int i = 0; i < T::HPT; ++i) {
+    for (int i = 0; i < G::HPT; ++i) {
         const int h = tid + i * kWG;
-        if (h < T::NH) {
+        if (h < G::NH) {
             int lr, ld;
-            halo_coords<TH, TW>(h, lr, ld);
-            const int d = ((lr * T::LP) >> 1) + ld;
+            halo_coords<G>(h, lr, ld);
+            const int d = ((lr * G::LP) >> 1) + ld;
 #pragma unroll
-            for (int p = 0; p < NPL; ++p) l32[p * (T::PLANE / 2) + d] = R.hv[p][i];
+            for (int p = 0; p < G::NPL; ++p) l32[p * (G::PLANE / 2) + d] = R.hv[p][i];
         }
     }
 }
 
-// Columns [16*TW+4, 16*TW+8) of every LDS row: read by the MFMA operand of the
+// Columns [16*BW+4, 16*BW+8) of every LDS row: read by the MFMA operand of the
 // last brick column (against zero coefficients), never written by write_tile.
-template <int TH, int TW, int NPL>
+template <class G>
 __device__ __forceinline__ void zero_pad(uint16_t *lds, int tid) {
-    using T = Tile<TH, TW, NPL>;
     uint32_t *l32 = reinterpret_cast<uint32_t *>(lds);
-    for (int zc = tid; zc < 2 * T::LR; zc += kWG) {
-        const int d = (((zc >> 1) * T::LP) >> 1) + 8 * TW + 2 + (zc & 1);
+    for (int zc = tid; zc < 2 * G::LR; zc += kWG) {
+        const int d = (((zc >> 1) * G::LP) >> 1) + 8 * G::BW + 2 + (zc & 1);
 #pragma unroll
-        for (int p = 0; p < NPL; ++p) l32[p * (T::PLANE / 2) + d] = 0u;
+        for (int p = 0; p < G::NPL; ++p) l32[p * (G::PLANE / 2) + d] = 0u;
     }
 }
 
@@ -235,13 +244,120 @@ struct Frag<TCS_PREC_BF16> {
 template <>
 struct Frag<TCS_PREC_SPLIT> : Frag<TCS_PREC_FP16> {};
 
+// Coefficient fragments.  Lane l holds A[m][k] for m = l%16 and
+// k = 8*(l/16) + e, e = 0..7: the coefficient at row offset z and column
+// offset k - m when 0 <= k - m <= 4, else 0.
+template <int ZLO, int ZHI, int PREC>
+struct Coef {
+    typedef typename Frag<PREC>::type frag_t;
+    static constexpr int NZ = ZHI - ZLO + 1;
+    frag_t hi[NZ], lo[NZ];
+
+    __device__ __forceinline__ void build(const SweepArgs &a, int m, int g) {
+#pragma unroll
+        for (int zi = 0; zi < NZ; ++zi) {
+#pragma unroll
+            for (int e = 0; e < 8; ++e) {
+                const int d = 8 * g + e - m;
+                if constexpr (PREC == TCS_PREC_SPLIT) {
+                    uint16_t hb = 0, lb = 0;
+#pragma unroll
+                    for (int dd = 0; dd < 5; ++dd)
+                        if (d == dd) {
+                            hb = a.ch[(ZLO + zi) * 5 + dd];
+                            lb = a.cl[(ZLO + zi) * 5 + dd];
+                        }
+                    hi[zi][e] = __builtin_bit_cast(_Float16, hb);
+                    lo[zi][e] = __builtin_bit_cast(_Float16, lb);
+                } else {
+                    float v = 0.0f;
+#pragma unroll
+                    for (int dd = 0; dd < 5; ++dd)
+                        if (d == dd) v = a.cf[(ZLO + zi) * 5 + dd];
+                    if constexpr (PREC == TCS_PREC_BF16)
+                        hi[zi][e] = static_cast<__bf16>(v);
+                    else
+                        hi[zi][e] = static_cast<_Float16>(v);
+                }
+            }
+        }
+    }
+};
+
+// One 16x16 output brick from the LDS image of geometry G: brick (br, bc) of
+// the image, whose haloed 20x20 window starts at LDS row 16*br, column 16*bc.
+// Lane (m, g) gets output row m, columns 4g..4g+3 in storage format (w[p]:
+// four 16-bit values of plane p).  gr, gcb: global row and first column of
+// those outputs; with `edge`, points outside the interior keep their input.
+template <class G, int ZLO, int ZHI, int PREC>
+__device__ __forceinline__ void mfma_brick(const SweepArgs &a, const uint16_t *lds, const Coef<ZLO, ZHI, PREC> &C,
+                                           int br, int bc, int m, int g, bool edge, int gr, int gcb,
+                                           u32x2 (&w)[G::NPL]) {
+    constexpr int NPL = G::NPL;
+    constexpr int NZ = ZHI - ZLO + 1;
+    using F = Frag<PREC>;
+    typedef typename F::type frag_t;
+    // Slot group 3 (k = 24..31) only meets zero coefficients: those lanes skip
+    // the LDS read and feed zeros.
+    const bool reads = g < 3;
+    f32x4 acc = {0.0f, 0.0f, 0.0f, 0.0f};
+    // Operand of row shift z: lane (m, g) holds haloed row m + z, columns
+    // 8g..8g+7.  With TCS_DPP_SHIFT, shift z+1 is shift z moved up one lane
+    // within each 16-lane row (DPP row_shl:1) and only lane 15 of each row
+    // reads a new LDS row.
+    u32x4 rh = {0u, 0u, 0u, 0u}, rl = {0u, 0u, 0u, 0u};
+#pragma unroll
+    for (int zi = 0; zi < NZ; ++zi) {
+        const int off = (16 * br + m + ZLO + zi) * G::LP + 16 * bc + 8 * g;
+        const bool fetch = reads && (TCS_DPP_SHIFT == 0 || zi == 0 || m == 15);
+        if (TCS_DPP_SHIFT && zi > 0) {
+            rh = rows_up(rh);
+            if constexpr (PREC == TCS_PREC_SPLIT) rl = rows_up(rl);
+        }
+        if (fetch) {
+            rh = *reinterpret_cast<const u32x4 *>(lds + off);
+            if constexpr (PREC == TCS_PREC_SPLIT) rl = *reinterpret_cast<const u32x4 *>(lds + G::PLANE + off);
+        }
+        const frag_t bh = __builtin_bit_cast(frag_t, rh);
+        acc = F::mma(C.hi[zi], bh, acc);
+        if constexpr (PREC == TCS_PREC_SPLIT) {
+            const frag_t bl = __builtin_bit_cast(frag_t, rl);
+            acc = F::mma(C.hi[zi], bl, acc);
+            acc = F::mma(C.lo[zi], bh, acc);
+            acc = F::mma(C.lo[zi], bl, acc);
+        }
+    }
+    uint16_t o[NPL][4];
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+        if constexpr (PREC == TCS_PREC_SPLIT) {
+            const float sv = acc[v] * a.out_scale;
+            const _Float16 hi = static_cast<_Float16>(sv);
+            o[0][v] = __builtin_bit_cast(uint16_t, hi);
+            o[1][v] = f16_bits(sv - static_cast<float>(hi));
+        } else if constexpr (PREC == TCS_PREC_BF16) {
+            o[0][v] = bf16_bits(acc[v]);
+        } else {
+            o[0][v] = f16_bits(acc[v]);
+        }
+    }
+    if (edge) {
+        const int cidx = (16 * br + m + 2) * G::LP + 16 * bc + 4 * g + 2;
+#pragma unroll
+        for (int v = 0; v < 4; ++v)
+            if (!point_is_interior(a, gr, gcb + v)) {
+#pragma unroll
+                for (int p = 0; p < NPL; ++p) o[p][v] = lds[p * G::PLANE + cidx + v];
+            }
+    }
+#pragma unroll
+    for (int p = 0; p < NPL; ++p) w[p] = u32x2{pack2(o[p][0], o[p][1]), pack2(o[p][2], o[p][3])};
+}
+
 template <int TH, int TW, int ZLO, int ZHI, int PREC, bool PERSIST>
 __global__ __launch_bounds__(kWG) void tcs_mfma_sweep(SweepArgs a) {
     constexpr int NPL = PREC == TCS_PREC_SPLIT ? 2 : 1;
-    constexpr int NZ = ZHI - ZLO + 1;
     using T = Tile<TH, TW, NPL>;
-    using F = Frag<PREC>;
-    typedef typename F::type frag_t;
     __shared__ __attribute__((aligned(16))) uint16_t lds[NPL * T::PLANE];
 
     Schedule sc = schedule(a);
@@ -249,126 +365,116 @@ __global__ __launch_bounds__(kWG) void tcs_mfma_sweep(SweepArgs a) {
     const int tid = threadIdx.x;
     int gr0, gc0;
     tile_origin<TH, TW>(a, sc.tile, gr0, gc0);
-    TileRegs<TH, TW, NPL> R;
-    load_tile<TH, TW, NPL>(a, R, gr0, gc0, tid);
-    zero_pad<TH, TW, NPL>(lds, tid);
+    TileRegs<T> R;
+    load_tile<T, false>(a, R, gr0, gc0, tid);
+    zero_pad<T>(lds, tid);
 
-    // Coefficient fragments.  Lane l holds A[m][k] for m = l%16 and
-    // k = 8*(l/16) + e, e = 0..7: the coefficient at row offset z and column
-    // offset k - m when 0 <= k - m <= 4, else 0.
     const int lane = tid & 63, wave = tid >> 6;
     const int m = lane & 15, g = lane >> 4;
-    frag_t ahi[NZ], alo[NZ];
-#pragma unroll
-    for (int zi = 0; zi < NZ; ++zi) {
-#pragma unroll
-        for (int e = 0; e < 8; ++e) {
-            const int d = 8 * g + e - m;
-            if constexpr (PREC == TCS_PREC_SPLIT) {
-                uint16_t hb = 0, lb = 0;
-#pragma unroll
-                for (int dd = 0; dd < 5; ++dd)
-                    if (d == dd) {
-                        hb = a.ch[(ZLO + zi) * 5 + dd];
-                        lb = a.cl[(ZLO + zi) * 5 + dd];
-                    }
-                ahi[zi][e] = __builtin_bit_cast(_Float16, hb);
-                alo[zi][e] = __builtin_bit_cast(_Float16, lb);
-            } else {
-                float v = 0.0f;
-#pragma unroll
-                for (int dd = 0; dd < 5; ++dd)
-                    if (d == dd) v = a.cf[(ZLO + zi) * 5 + dd];
-                if constexpr (PREC == TCS_PREC_BF16)
-                    ahi[zi][e] = static_cast<__bf16>(v);
-                else
-                    ahi[zi][e] = static_cast<_Float16>(v);
-            }
-        }
-    }
-    // Slot group 3 (k = 24..31) only meets zero coefficients: those lanes skip
-    // the LDS read and feed zeros.
-    const bool reads = g < 3;
+    Coef<ZLO, ZHI, PREC> C;
+    C.build(a, m, g);
     constexpr int NB = TH * TW / 4;
 
     for (;;) {
-    write_tile<TH, TW, NPL>(R, lds, tid);
-    __syncthreads();
-    const int cgr0 = gr0, cgc0 = gc0;  // the tile in LDS
-    const int next = sc.tile + sc.step;
-    if (PERSIST && next < sc.end) {
-        tile_origin<TH, TW>(a, next, gr0, gc0);
-        load_tile<TH, TW, NPL>(a, R, gr0, gc0, tid);  // in flight during the compute below
+        write_tile<T>(R, lds, tid);
+        __syncthreads();
+        const int cgr0 = gr0, cgc0 = gc0;  // the tile in LDS
+        const int next = sc.tile + sc.step;
+        if (PERSIST && next < sc.end) {
+            tile_origin<TH, TW>(a, next, gr0, gc0);
+            load_tile<T, false>(a, R, gr0, gc0, tid);  // in flight during the compute below
+        }
+        const bool edge = !tile_is_interior(a, cgr0, cgc0, 16 * TH, 16 * TW);
+#pragma unroll
+        for (int q = 0; q < NB; ++q) {
+            const int b = wave + 4 * q;
+            const int br = b / TW, bc = b - br * TW;
+            const int gr = cgr0 + 16 * br + m, gcb = cgc0 + 16 * bc + 4 * g;
+            u32x2 w[NPL];
+            mfma_brick<T, ZLO, ZHI, PREC>(a, lds, C, br, bc, m, g, edge, gr, gcb, w);
+            const int64_t s = stor_index(gr, gcb, a.sbc);
+#pragma unroll
+            for (int p = 0; p < NPL; ++p) store8<true>(a.out + p * a.plane + s, w[p]);
+        }
+        // One-tile-per-workgroup launches cover every tile with the grid itself.
+        if (!PERSIST || next >= sc.end) break;
+        sc.tile = next;
+        __syncthreads();  // every wave is done reading LDS before it is overwritten
     }
-    const bool edge = !tile_is_interior(a, cgr0, cgc0, 16 * TH, 16 * TW);
+}
+
+// Temporal blocking: `nsweeps` sweeps (1 <= nsweeps, radius * (nsweeps - 1) <= 16)
+// of one TH x TW tile per workgroup, fused in LDS.  The LDS image holds the tile
+// plus a one-brick ring and the 2-point frame; every sweep but the last
+// recomputes the ring too (each sweep's valid region shrinks by the radius, and
+// the ring is 16 points wide), computing into registers, then writing back in
+// place after a barrier.  Values are rounded to the storage format after every
+// sweep exactly as between unfused launches, so k fused sweeps are bitwise
+// equal to k single sweeps.  Values outside the valid region stay finite
+// (stencils of loaded data), which the MFMA form needs: a non-finite operand
+// anywhere in a brick row's window reaches all 16 outputs of that row.
+template <int TH, int TW, int ZLO, int ZHI, int PREC>
+__global__ __launch_bounds__(kWG) void tcs_mfma_fused(SweepArgs a, int nsweeps) {
+    constexpr int NPL = PREC == TCS_PREC_SPLIT ? 2 : 1;
+    using G = Geom<TH + 2, TW + 2, NPL>;
+    constexpr int QA = G::BH * G::BW / 4;  // bricks per wave, whole image
+    constexpr int QC = TH * TW / 4;        // bricks per wave, core only
+    static_assert(G::BH * G::BW % 4 == 0 && TH * TW % 4 == 0, "bricks must split over 4 waves");
+    __shared__ __attribute__((aligned(16))) uint16_t lds[NPL * G::PLANE];
+
+    Schedule sc = schedule(a);
+    if (sc.tile >= sc.end) return;
+    const int tid = threadIdx.x;
+    int gr0, gc0;
+    tile_origin<TH, TW>(a, sc.tile, gr0, gc0);
+    const int er0 = gr0 - 16, ec0 = gc0 - 16;  // global point of image brick (0, 0)
+    {
+        TileRegs<G> R;
+        load_tile<G, true>(a, R, er0, ec0, tid);
+        zero_pad<G>(lds, tid);
+        write_tile<G>(R, lds, tid);
+    }
+    const int lane = tid & 63, wave = tid >> 6;
+    const int m = lane & 15, g = lane >> 4;
+    Coef<ZLO, ZHI, PREC> C;
+    C.build(a, m, g);
+    const bool edge = !tile_is_interior(a, er0, ec0, 16 * G::BH, 16 * G::BW);
+    __syncthreads();
+
+    uint32_t *l32 = reinterpret_cast<uint32_t *>(lds);
+    for (int sw = 0; sw + 1 < nsweeps; ++sw) {
+        u32x2 res[QA][NPL];
 #pragma unroll
-    for (int q = 0; q < NB; ++q) {
+        for (int q = 0; q < QA; ++q) {
+            const int b = wave + 4 * q;
+            const int br = b / G::BW, bc = b - br * G::BW;
+            mfma_brick<G, ZLO, ZHI, PREC>(a, lds, C, br, bc, m, g, edge, er0 + 16 * br + m, ec0 + 16 * bc + 4 * g,
+                                          res[q]);
+        }
+        __syncthreads();  // every wave has read this sweep's input
+#pragma unroll
+        for (int q = 0; q < QA; ++q) {
+            const int b = wave + 4 * q;
+            const int br = b / G::BW, bc = b - br * G::BW;
+            const int d = ((16 * br + m + 2) * G::LP + 16 * bc + 4 * g + 2) >> 1;
+#pragma unroll
+            for (int p = 0; p < NPL; ++p) {
+                l32[p * (G::PLANE / 2) + d] = res[q][p][0];
+                l32[p * (G::PLANE / 2) + d + 1] = res[q][p][1];
+            }
+        }
+        __syncthreads();
+    }
+#pragma unroll
+    for (int q = 0; q < QC; ++q) {
         const int b = wave + 4 * q;
-        const int br = b / TW, bc = b - br * TW;
-        f32x4 acc = {0.0f, 0.0f, 0.0f, 0.0f};
-        // Operand of row shift z: lane (m, g) holds haloed row m + z, columns
-        // 8g..8g+7.  With TCS_DPP_SHIFT, shift z+1 is shift z moved up one lane
-        // within each 16-lane row (DPP row_shl:1) and only lane 15 of each row
-        // reads a new LDS row.
-        u32x4 rh = {0u, 0u, 0u, 0u}, rl = {0u, 0u, 0u, 0u};
-#pragma unroll
-        for (int zi = 0; zi < NZ; ++zi) {
-            const int off = (16 * br + m + ZLO + zi) * T::LP + 16 * bc + 8 * g;
-            const bool fetch = reads && (TCS_DPP_SHIFT == 0 || zi == 0 || m == 15);
-            if (TCS_DPP_SHIFT && zi > 0) {
-                rh = rows_up(rh);
-                if constexpr (PREC == TCS_PREC_SPLIT) rl = rows_up(rl);
-            }
-            if (fetch) {
-                rh = *reinterpret_cast<const u32x4 *>(lds + off);
-                if constexpr (PREC == TCS_PREC_SPLIT) rl = *reinterpret_cast<const u32x4 *>(lds + T::PLANE + off);
-            }
-            const frag_t bh = __builtin_bit_cast(frag_t, rh);
-            acc = F::mma(ahi[zi], bh, acc);
-            if constexpr (PREC == TCS_PREC_SPLIT) {
-                const frag_t bl = __builtin_bit_cast(frag_t, rl);
-                acc = F::mma(ahi[zi], bl, acc);
-                acc = F::mma(alo[zi], bh, acc);
-                acc = F::mma(alo[zi], bl, acc);
-            }
-        }
-        // Lane holds output row m, columns 4g..4g+3 of brick (br, bc).
-        const int gr = cgr0 + 16 * br + m;
-        const int gcb = cgc0 + 16 * bc + 4 * g;
-        uint16_t o[NPL][4];
-#pragma unroll
-        for (int v = 0; v < 4; ++v) {
-            if constexpr (PREC == TCS_PREC_SPLIT) {
-                const float s = acc[v] * a.out_scale;
-                const _Float16 hi = static_cast<_Float16>(s);
-                o[0][v] = __builtin_bit_cast(uint16_t, hi);
-                o[1][v] = f16_bits(s - static_cast<float>(hi));
-            } else if constexpr (PREC == TCS_PREC_BF16) {
-                o[0][v] = bf16_bits(acc[v]);
-            } else {
-                o[0][v] = f16_bits(acc[v]);
-            }
-        }
-        if (edge) {
-            const int cidx = (16 * br + m + 2) * T::LP + 16 * bc + 4 * g + 2;
-#pragma unroll
-            for (int v = 0; v < 4; ++v)
-                if (!point_is_interior(a, gr, gcb + v)) {
-#pragma unroll
-                    for (int p = 0; p < NPL; ++p) o[p][v] = lds[p * T::PLANE + cidx + v];
-                }
-        }
+        const int br = 1 + b / TW, bc = 1 + (b - (b / TW) * TW);
+        const int gr = er0 + 16 * br + m, gcb = ec0 + 16 * bc + 4 * g;
+        u32x2 w[NPL];
+        mfma_brick<G, ZLO, ZHI, PREC>(a, lds, C, br, bc, m, g, edge, gr, gcb, w);
         const int64_t s = stor_index(gr, gcb, a.sbc);
 #pragma unroll
-        for (int p = 0; p < NPL; ++p) {
-            u32x2 w = {pack2(o[p][0], o[p][1]), pack2(o[p][2], o[p][3])};
-            store8<true>(a.out + p * a.plane + s, w);
-        }
-    }
-    // One-tile-per-workgroup launches cover every tile with the grid itself.
-    if (!PERSIST || next >= sc.end) break;
-    sc.tile = next;
-    __syncthreads();  // every wave is done reading LDS before it is overwritten
+        for (int p = 0; p < NPL; ++p) store8<true>(a.out + p * a.plane + s, w[p]);
     }
 }
 
@@ -395,21 +501,21 @@ __global__ __launch_bounds__(kWG) void tcs_direct_sweep(SweepArgs a) {
     const int tid = threadIdx.x;
     int gr0, gc0;
     tile_origin<TH, TW>(a, sc.tile, gr0, gc0);
-    TileRegs<TH, TW, 1> R;
-    load_tile<TH, TW, 1>(a, R, gr0, gc0, tid);
-    zero_pad<TH, TW, 1>(lds, tid);
+    TileRegs<T> R;
+    load_tile<T, false>(a, R, gr0, gc0, tid);
+    zero_pad<T>(lds, tid);
 
     const int cg = tid % CG, rg = tid / CG;
     const int r0 = rg * MR;  // first output row of this strip (tile coordinates)
 
     for (;;) {
-    write_tile<TH, TW, 1>(R, lds, tid);
+    write_tile<T>(R, lds, tid);
     __syncthreads();
     const int cgr0 = gr0, cgc0 = gc0;  // the tile in LDS
     const int next = sc.tile + sc.step;
     if (PERSIST && next < sc.end) {
         tile_origin<TH, TW>(a, next, gr0, gc0);
-        load_tile<TH, TW, 1>(a, R, gr0, gc0, tid);  // in flight during the compute below
+        load_tile<T, false>(a, R, gr0, gc0, tid);  // in flight during the compute below
     }
     float acc[MR][4];
 #pragma unroll
@@ -526,6 +632,35 @@ static const KernelEntry *select_kernel(const tcs_problem *p) {
     return nullptr;
 }
 
+// Fused (temporally blocked) MFMA kernels: tiles 2x4, 4x4, 4x8, 8x8.
+typedef void (*fused_fn)(SweepArgs, int);
+struct FusedEntry {
+    int prec, radius, th, tw;
+    fused_fn fn;
+    const char *name;
+};
+#define TCS_FUSED(TH, TW, ZLO, ZHI, P, R) \
+    {P, R, TH, TW, tcs_mfma_fused<TH, TW, ZLO, ZHI, P>, "tcs_mfma_fused<" #TH ", " #TW ", " #ZLO ", " #ZHI ", " #P ">"}
+#define TCS_FUSED_TILES(ZLO, ZHI, P, R)                                                           \
+    TCS_FUSED(2, 4, ZLO, ZHI, P, R), TCS_FUSED(4, 4, ZLO, ZHI, P, R), TCS_FUSED(4, 8, ZLO, ZHI, P, R), \
+        TCS_FUSED(8, 8, ZLO, ZHI, P, R)
+static const FusedEntry kFused[] = {
+    TCS_FUSED_TILES(0, 4, 0, 2), TCS_FUSED_TILES(1, 3, 0, 1), TCS_FUSED_TILES(0, 4, 1, 2),
+    TCS_FUSED_TILES(1, 3, 1, 1), TCS_FUSED_TILES(0, 4, 2, 2), TCS_FUSED_TILES(1, 3, 2, 1),
+};
+
+static const FusedEntry *select_fused(const tcs_problem *p) {
+    if (p->method != TCS_METHOD_MFMA) return nullptr;
+    for (const FusedEntry &k : kFused)
+        if (k.prec == p->precision && k.radius == p->radius && k.th == p->tile_h && k.tw == p->tile_w) return &k;
+    return nullptr;
+}
+
+const char *fused_kernel_name(const tcs_problem *p) {
+    const FusedEntry *k = select_fused(p);
+    return k ? k->name : nullptr;
+}
+
 const char *sweep_kernel_name(const tcs_problem *p) {
     const KernelEntry *k = select_kernel(p);
     return k ? k->name : nullptr;
@@ -537,21 +672,17 @@ int mfma_per_brick(const tcs_problem *p) {
     return p->precision == TCS_PREC_SPLIT ? 4 * nz : nz;
 }
 
-int launch_sweep(const tcs_problem *p, const void *d_in, void *d_out, hipStream_t stream) {
-    const KernelEntry *k = select_kernel(p);
-    if (!k)
-        return set_error(TCS_ERR_UNSUPPORTED, "no kernel for stencil %d method %d precision %d tile %dx%d",
-                         p->stencil, p->method, p->precision, p->tile_h, p->tile_w);
+static int make_args(const tcs_problem *p, const void *d_in, void *d_out, SweepArgs &a) {
     const tcs_layout &L = p->layout;
     if (L.nbr % p->tile_h != 0 || L.nbc % p->tile_w != 0)
         return set_error(TCS_ERR_INVALID, "layout %dx%d bricks is not a multiple of tile %dx%d", L.nbr,
                          L.nbc, p->tile_h, p->tile_w);
-    SweepArgs a;
     a.in = static_cast<const uint16_t *>(d_in);
     a.out = static_cast<uint16_t *>(d_out);
     a.plane = L.plane_elems;
     a.rows = L.rows;
     a.cols = L.cols;
+    a.sbr = L.sbr;
     a.sbc = L.sbc;
     a.tiles_x = L.nbc / p->tile_w;
     a.ntiles = (L.nbr / p->tile_h) * a.tiles_x;
@@ -563,6 +694,33 @@ int launch_sweep(const tcs_problem *p, const void *d_in, void *d_out, hipStream_
         a.ch[i] = p->coeff_hi[i];
         a.cl[i] = p->coeff_lo[i];
     }
+    return TCS_OK;
+}
+
+int launch_fused(const tcs_problem *p, const void *d_in, void *d_out, int nsweeps, hipStream_t stream) {
+    const FusedEntry *k = select_fused(p);
+    if (!k)
+        return set_error(TCS_ERR_UNSUPPORTED, "no fused kernel for method %d precision %d tile %dx%d", p->method,
+                         p->precision, p->tile_h, p->tile_w);
+    if (nsweeps < 1 || p->radius * (nsweeps - 1) > 16)
+        return set_error(TCS_ERR_INVALID, "fused launch of %d sweeps: radius %d allows 1..%d", nsweeps, p->radius,
+                         1 + 16 / p->radius);
+    SweepArgs a;
+    int st = make_args(p, d_in, d_out, a);
+    if (st) return st;
+    hipLaunchKernelGGL(k->fn, dim3((unsigned)(a.tiles_per_xcd * kXCD)), dim3(kWG), 0, stream, a, nsweeps);
+    TCS_HIP_CHECK(hipGetLastError());
+    return TCS_OK;
+}
+
+int launch_sweep(const tcs_problem *p, const void *d_in, void *d_out, hipStream_t stream) {
+    const KernelEntry *k = select_kernel(p);
+    if (!k)
+        return set_error(TCS_ERR_UNSUPPORTED, "no kernel for stencil %d method %d precision %d tile %dx%d",
+                         p->stencil, p->method, p->precision, p->tile_h, p->tile_w);
+    SweepArgs a;
+    int st = make_args(p, d_in, d_out, a);
+    if (st) return st;
     // One workgroup per tile, or (wg_per_cu > 0) a persistent grid of wg_per_cu
     // workgroups per CU; either way a multiple of 8 so every XCD gets the same.
     int per_xcd = a.tiles_per_xcd;

@@ -33,19 +33,20 @@ import sys
 import tempfile
 
 HBM_PEAK_GBS = 8000.0
-BYTES_PER_UPDATE = {"fp16": 4.0, "bf16": 4.0, "split": 8.0}
+BYTES_PER_UPDATE = {"fp16": 4.0, "bf16": 4.0, "split": 8.0}  # per sweep, unfused; fused: / fuse
 TILES = [(2, 4), (4, 4), (2, 8), (4, 8), (8, 8)]
 
 
 def run_workload(stencil, rows, cols, method="mfma", precision="fp16", tile=None, sweeps=20,
-                 warmup=5, copy_mb=0, seed=1):
+                 warmup=5, copy_mb=0, seed=1, fuse=1, wg_per_cu=0):
     import numpy as np
     import torch
 
     from . import api as T
 
     B = T.gen_coeffs(stencil, seed)
-    p = T.Problem(stencil, rows, cols, B, method=method, precision=precision, tile=tile)
+    p = T.Problem(stencil, rows, cols, B, method=method, precision=precision, tile=tile, fuse=fuse,
+                  wg_per_cu=wg_per_cu)
     x = T.gen_uniform(rows, cols, seed=seed, dtype=torch.float32)
     e = T.split_scale_exp(1.0) if precision == "split" else 0
     b0, b1 = p.alloc(), p.alloc()
@@ -62,9 +63,10 @@ def run_workload(stencil, rows, cols, method="mfma", precision="fp16", tile=None
     ms = ev[0].elapsed_time(ev[1]) / sweeps
     pts = p.interior_points
     out = {"stencil": stencil, "rows": rows, "cols": cols, "method": method, "precision": precision,
-           "tile": list(p.tile), "kernel": p.kernel_name, "sweeps": sweeps, "ms_per_sweep": ms,
+           "tile": list(p.tile), "fuse": fuse,
+           "kernel": p.fused_kernel_name if fuse > 1 else p.kernel_name, "sweeps": sweeps, "ms_per_sweep": ms,
            "gstencil_s": pts / (ms * 1e-3) / 1e9,
-           "hbm_gbs": BYTES_PER_UPDATE[precision] * pts / (ms * 1e-3) / 1e9}
+           "hbm_gbs": BYTES_PER_UPDATE[precision] / max(fuse, 1) * pts / (ms * 1e-3) / 1e9}
     out["hbm_frac"] = out["hbm_gbs"] / HBM_PEAK_GBS
     if copy_mb:
         n = int(copy_mb * 2 ** 20) // 16 * 16
@@ -118,6 +120,8 @@ def main(argv=None):
     r.add_argument("--sweeps", type=int, default=20)
     r.add_argument("--warmup", type=int, default=5)
     r.add_argument("--copy-mb", type=float, default=0)
+    r.add_argument("--fuse", type=int, default=1)
+    r.add_argument("--wg-per-cu", type=int, default=0)
     s = sub.add_parser("sweep")
     s.add_argument("--stencils", nargs="+", default=["star2d1r", "star2d2r", "box2d1r", "box2d2r"])
     s.add_argument("--methods", nargs="+", default=["mfma", "direct"])
@@ -131,7 +135,8 @@ def main(argv=None):
 
     if a.cmd == "run":
         res = run_workload(a.stencil, a.size, a.cols or a.size, a.method, a.precision,
-                           tuple(a.tile) if a.tile else None, a.sweeps, a.warmup, a.copy_mb)
+                           tuple(a.tile) if a.tile else None, a.sweeps, a.warmup, a.copy_mb,
+                           fuse=a.fuse, wg_per_cu=a.wg_per_cu)
         print(json.dumps(res), flush=True)
         return 0
 

@@ -45,6 +45,8 @@ class Layout:
 
 def _check(idx, size, align=1, what=""):
     idx = np.asarray(idx)
+    if idx.size == 0:
+        return
     assert idx.min() >= 0 and idx.max() < size, f"{what}: index out of bounds [{idx.min()}, {idx.max()}] size {size}"
     assert np.all(idx % align == 0), f"{what}: misaligned access"
 
@@ -100,20 +102,22 @@ class TileGeom:
         self.LR = 16 * TH + 4
         self.LP = 16 * TW + 8
         self.NCH = TH * TW * 32
-        assert self.NCH % WG == 0
-        self.CPT = self.NCH // WG
+        self.CPT = -(-self.NCH // WG)
         self.HD = 8 * TW + 2
         self.NH = 4 * self.HD + 32 * TH
         self.HPT = -(-self.NH // WG)
         self.PLANE = self.LR * self.LP
 
 
-def stage_tile(inp: np.ndarray, L: Layout, G: TileGeom, gr0: int, gc0: int) -> np.ndarray:
+def stage_tile(inp: np.ndarray, L: Layout, G: TileGeom, gr0: int, gc0: int, check=False) -> np.ndarray:
+    """load_tile + write_tile + zero_pad.  check: frame points outside the
+    stored bricks read as zero (the fused kernel's image includes the ghost ring)."""
     lds = np.full(G.PLANE, np.nan)
     tid = np.arange(WG, dtype=np.int64)
     TW, TH = G.TW, G.TH
     for i in range(G.CPT):
         c = tid + i * WG
+        c = c[c < G.NCH]
         b, w = c >> 5, c & 31
         br, bc = b // TW, b % TW
         s = stor_index(gr0 + 16 * br + (w >> 1), gc0 + 16 * bc + 8 * (w & 1), L.sbc)
@@ -139,12 +143,18 @@ def stage_tile(inp: np.ndarray, L: Layout, G: TileGeom, gr0: int, gc0: int) -> n
         m4 = ~(m1 | m2 | m3)
         lr[m4], ld[m4] = 2 + (h[m4] - 4 * G.HD - 16 * TH), G.HD - 1
         act = h < G.NH
-        s = stor_index(gr0 - 2 + lr[act], gc0 - 2 + 2 * ld[act], L.sbc)
-        _check(s, L.plane - 1, 2, "halo load (4 B)")
+        gr, gc = gr0 - 2 + lr[act], gc0 - 2 + 2 * ld[act]
+        inside = np.ones(gr.shape, dtype=bool)
+        if check:
+            inside = (((gr + 16) >> 4) >= 0) & (((gr + 16) >> 4) < L.sbr) & (((gc + 16) >> 4) >= 0) & (((gc + 16) >> 4) < L.sbc)
+        s = stor_index(gr, gc, L.sbc)
+        _check(s[inside], L.plane - 1, 2, "halo load (4 B)")
         d = ((lr[act] * G.LP) >> 1) + ld[act]
         _check(2 * d, G.PLANE - 1, 2, "halo LDS write")
-        lds[2 * d] = inp[s]
-        lds[2 * d + 1] = inp[s + 1]
+        v0, v1 = np.zeros(gr.shape), np.zeros(gr.shape)
+        v0[inside], v1[inside] = inp[s[inside]], inp[s[inside] + 1]
+        lds[2 * d] = v0
+        lds[2 * d + 1] = v1
     for zc in range(2 * G.LR):
         d = (((zc >> 1) * G.LP) >> 1) + 8 * TW + 2 + (zc & 1)
         assert 0 <= 2 * d < G.PLANE
@@ -179,23 +189,74 @@ def _interior(gr, gc, L: Layout, rs: int):
 
 
 # ------------------------------------------------------------- MFMA sweep
-def mfma_sweep(inp, L: Layout, TH, TW, coeff5, rs, round_coeff=f16, per_xcd=0):
-    """tcs_mfma_sweep for FP16 (one plane).  coeff5: 5x5 FP32 embedding."""
-    assert L.nbr % TH == 0 and L.nbc % TW == 0
-    G = TileGeom(TH, TW)
+def _afrag(coeff5, rs, round_coeff=f16):
     ZLO, ZHI = (0, 4) if rs == 2 else (1, 3)
-    out = np.zeros(L.plane)
-    written = np.zeros(L.plane, dtype=bool)
     lane = np.arange(64)
     m, g = lane & 15, lane >> 4
-    reads = g < 3  # slot group 3 reads no LDS and feeds zeros
-    # A fragments: afrag[zi][lane][e]
     afrag = np.zeros((ZHI - ZLO + 1, 64, 8))
     for zi in range(ZHI - ZLO + 1):
         for e in range(8):
             d = 8 * g + e - m
             ok = (d >= 0) & (d <= 4)
             afrag[zi, ok, e] = round_coeff(coeff5[ZLO + zi, d[ok]])
+    return afrag, ZLO, ZHI
+
+
+def _mfma_brick(lds, G: TileGeom, afrag, ZLO, ZHI, br, bc, edge, gr, gcb, L: Layout, rs):
+    """mfma_brick: returns o[lane][v] (FP16 values) for LDS brick (br, bc)."""
+    lane = np.arange(64)
+    m, g = lane & 15, lane >> 4
+    reads = g < 3  # slot group 3 reads no LDS and feeds zeros
+    D = np.zeros((16, 16))
+    bfrag = np.zeros((64, 8))
+    for zi in range(ZHI - ZLO + 1):
+        off = (16 * br + m + ZLO + zi) * G.LP + 16 * bc + 8 * g
+        fetch = reads & ((zi == 0) | (m == 15))
+        if zi > 0:  # DPP row_shl:1 within each 16-lane row; lane 15 gets 0
+            shifted = np.zeros((64, 8))
+            for l in range(64):
+                if l % 16 != 15:
+                    shifted[l] = bfrag[l + 1]
+            bfrag = shifted
+        _check(off[fetch], G.PLANE - 7, 8, "B operand LDS read (16 B)")
+        bfrag[fetch] = lds[off[fetch][:, None] + np.arange(8)[None, :]]  # [lane][e]
+        direct = np.zeros((64, 8))
+        direct[reads] = lds[off[reads][:, None] + np.arange(8)[None, :]]
+        assert np.array_equal(bfrag, direct, equal_nan=True), "shifted operand differs from LDS rows"
+        Am = np.zeros((16, 32))
+        Bm = np.zeros((32, 16))
+        for l in range(64):
+            Am[l % 16, 8 * (l // 16):8 * (l // 16) + 8] = afrag[zi, l]
+            Bm[8 * (l // 16):8 * (l // 16) + 8, l % 16] = bfrag[l]
+        D += Am @ Bm
+    # lane l holds D[4g+v][n], n = l % 16: output row n, column 4g+v
+    o = np.stack([f16(D[4 * g + v, m]) for v in range(4)], axis=1)  # [lane][v]
+    if edge:
+        cidx = (16 * br + m + 2) * G.LP + 16 * bc + 4 * g + 2
+        _check(cidx, G.PLANE - 3, 1, "centre read")
+        for v in range(4):
+            keep = ~_interior(gr, gcb + v, L, rs)
+            o[keep, v] = lds[cidx[keep] + v]
+    return o
+
+
+def _store(out, written, L, gr, gcb, o):
+    s = stor_index(gr, gcb, L.sbc)
+    _check(s, L.plane - 3, 4, "output store (8 B)")
+    for v in range(4):
+        assert not written[s + v].any(), "two lanes store the same element"
+        out[s + v] = o[:, v]
+        written[s + v] = True
+
+
+def mfma_sweep(inp, L: Layout, TH, TW, coeff5, rs, round_coeff=f16, per_xcd=0):
+    """tcs_mfma_sweep for FP16 (one plane).  coeff5: 5x5 FP32 embedding."""
+    assert L.nbr % TH == 0 and L.nbc % TW == 0
+    G = TileGeom(TH, TW)
+    afrag, ZLO, ZHI = _afrag(coeff5, rs, round_coeff)
+    out = np.zeros(L.plane)
+    written = np.zeros(L.plane, dtype=bool)
+    m, g = np.arange(64) & 15, np.arange(64) >> 4
     for gr0, gc0 in _tiles(L, TH, TW, per_xcd):
         lds = stage_tile(inp, L, G, gr0, gc0)
         edge = not (gr0 >= rs and gr0 + 16 * TH <= L.rows - rs and gc0 >= rs and gc0 + 16 * TW <= L.cols - rs)
@@ -203,43 +264,48 @@ def mfma_sweep(inp, L: Layout, TH, TW, coeff5, rs, round_coeff=f16, per_xcd=0):
             for q in range(TH * TW // 4):
                 b = wave + 4 * q
                 br, bc = b // TW, b % TW
-                D = np.zeros((16, 16))
-                bfrag = np.zeros((64, 8))
-                for zi in range(ZHI - ZLO + 1):
-                    off = (16 * br + m + ZLO + zi) * G.LP + 16 * bc + 8 * g
-                    fetch = reads & ((zi == 0) | (m == 15))
-                    if zi > 0:  # DPP row_shl:1 within each 16-lane row; lane 15 gets 0
-                        shifted = np.zeros((64, 8))
-                        for l in range(64):
-                            if l % 16 != 15:
-                                shifted[l] = bfrag[l + 1]
-                        bfrag = shifted
-                    _check(off[fetch], G.PLANE - 7, 8, "B operand LDS read (16 B)")
-                    bfrag[fetch] = lds[off[fetch][:, None] + np.arange(8)[None, :]]  # [lane][e]
-                    direct = np.zeros((64, 8))
-                    direct[reads] = lds[off[reads][:, None] + np.arange(8)[None, :]]
-                    assert np.array_equal(bfrag, direct, equal_nan=True), "shifted operand differs from LDS rows"
-                    Am = np.zeros((16, 32))
-                    Bm = np.zeros((32, 16))
-                    for l in range(64):
-                        Am[l % 16, 8 * (l // 16):8 * (l // 16) + 8] = afrag[zi, l]
-                        Bm[8 * (l // 16):8 * (l // 16) + 8, l % 16] = bfrag[l]
-                    D += Am @ Bm
-                # lane l holds D[4g+v][n], n = l % 16: output row n, column 4g+v
-                gr = gr0 + 16 * br + m
-                gcb = gc0 + 16 * bc + 4 * g
-                o = np.stack([f16(D[4 * g + v, m]) for v in range(4)], axis=1)  # [lane][v]
-                if edge:
-                    cidx = (16 * br + m + 2) * G.LP + 16 * bc + 4 * g + 2
-                    for v in range(4):
-                        keep = ~_interior(gr, gcb + v, L, rs)
-                        o[keep, v] = lds[cidx[keep] + v]
-                s = stor_index(gr, gcb, L.sbc)
-                _check(s, L.plane - 3, 4, "output store (8 B)")
+                gr, gcb = gr0 + 16 * br + m, gc0 + 16 * bc + 4 * g
+                o = _mfma_brick(lds, G, afrag, ZLO, ZHI, br, bc, edge, gr, gcb, L, rs)
+                _store(out, written, L, gr, gcb, o)
+    return out, written
+
+
+def mfma_fused(inp, L: Layout, TH, TW, coeff5, rs, nsweeps, round_coeff=f16):
+    """tcs_mfma_fused: nsweeps sweeps per tile in an LDS image of the tile plus a
+    one-brick ring and the 2-point frame, rewritten in place between sweeps."""
+    assert L.nbr % TH == 0 and L.nbc % TW == 0 and 1 <= nsweeps and rs * (nsweeps - 1) <= 16
+    G = TileGeom(TH + 2, TW + 2)
+    afrag, ZLO, ZHI = _afrag(coeff5, rs, round_coeff)
+    out = np.zeros(L.plane)
+    written = np.zeros(L.plane, dtype=bool)
+    m, g = np.arange(64) & 15, np.arange(64) >> 4
+    BH, BW = TH + 2, TW + 2
+    for gr0, gc0 in _tiles(L, TH, TW, 0):
+        er0, ec0 = gr0 - 16, gc0 - 16
+        lds = stage_tile(inp, L, G, er0, ec0, check=True)
+        assert not np.isnan(lds).any(), "fused image has unwritten LDS words"
+        edge = not (er0 >= rs and er0 + 16 * BH <= L.rows - rs and ec0 >= rs and ec0 + 16 * BW <= L.cols - rs)
+        for sw in range(nsweeps - 1):
+            res = {}
+            for wave in range(4):
+                for q in range(BH * BW // 4):
+                    b = wave + 4 * q
+                    br, bc = b // BW, b % BW
+                    res[(br, bc)] = _mfma_brick(lds, G, afrag, ZLO, ZHI, br, bc, edge,
+                                                er0 + 16 * br + m, ec0 + 16 * bc + 4 * g, L, rs)
+            for (br, bc), o in res.items():
+                d = (16 * br + m + 2) * G.LP + 16 * bc + 4 * g + 2
+                assert np.all(d % 2 == 0)
+                _check(d, G.PLANE - 3, 2, "fused write-back")
                 for v in range(4):
-                    assert not written[s + v].any(), "two lanes store the same element"
-                    out[s + v] = o[:, v]
-                    written[s + v] = True
+                    lds[d + v] = o[:, v]
+        for wave in range(4):
+            for q in range(TH * TW // 4):
+                b = wave + 4 * q
+                br, bc = 1 + b // TW, 1 + b % TW
+                gr, gcb = er0 + 16 * br + m, ec0 + 16 * bc + 4 * g
+                o = _mfma_brick(lds, G, afrag, ZLO, ZHI, br, bc, edge, gr, gcb, L, rs)
+                _store(out, written, L, gr, gcb, o)
     return out, written
 
 

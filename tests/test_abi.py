@@ -48,7 +48,7 @@ def test_host_code_shims_export_host_code(stencil):
 def test_struct_sizes_match_c():
     from tcstencil_amd import _lib
     assert ctypes.sizeof(_lib.Layout) == 40
-    assert ctypes.sizeof(_lib.ProblemStruct) == 272
+    assert ctypes.sizeof(_lib.ProblemStruct) == 280
     assert ctypes.sizeof(_lib.Params) == 32
 
 

@@ -91,3 +91,40 @@ def test_persistent_schedule_covers_every_tile(ntile_shape):
     for per_xcd in (0, 1, 2, 3, 7, 1000):
         origins = list(E._tiles(L, TH, TW, per_xcd))
         assert len(origins) == len(set(origins)) == (L.nbr // TH) * (L.nbc // TW)
+
+
+@pytest.mark.parametrize("stencil", STENCILS)
+@pytest.mark.parametrize("case", [(70, 150, 2, 4, 3), (40, 70, 4, 4, 2), (33, 140, 4, 8, 4)],
+                         ids=lambda c: "%dx%d_t%dx%d_k%d" % c)
+def test_emulated_fused_equals_repeated_sweeps(oracle, stencil, case):
+    """k fused sweeps == k single sweeps, bitwise, and both equal the FP64 oracle
+    rounded to FP16 after every sweep (each sweep of dyadic FP16 data with
+    taps 1/2, 1/4, 1/4 is exact in FP64)."""
+    rows, cols, TH, TW, k = case
+    r = RADIUS[stencil]
+    A, _, _ = dyadic_problem(stencil, rows, cols, seed=rows + cols)
+    n = 2 * r + 1
+    B = np.zeros((n, n))
+    B[r, r] = 0.5
+    B[r, 0] = 0.25
+    B[0, r] = 0.25
+    c5 = np.zeros((5, 5))
+    c5[2 - r:3 + r, 2 - r:3 + r] = B
+    L = E.Layout(rows, cols, max(TH, TW))
+    st = E.pack(A, L)
+    fused, written = E.mfma_fused(st, L, TH, TW, c5, r, k)
+    data = np.zeros(L.plane, dtype=bool)
+    for bi in range(L.nbr):
+        base = ((bi + 1) * L.sbc + 1) * 256
+        data[base:base + L.nbc * 256] = True
+    assert np.array_equal(written, data)
+    x = st
+    for _ in range(k):
+        x, _ = E.mfma_sweep(x, L, TH, TW, c5, r)
+        x = np.where(data, x, 0.0)
+    assert np.array_equal(fused, x)
+    # each single sweep of FP16 data is exact in FP64, then rounded to FP16
+    ref = A
+    for _ in range(k):
+        ref = E.f16(oracle.sweep_f64(stencil, ref, B))
+    assert np.array_equal(E.unpack(fused, L), ref)

@@ -375,3 +375,68 @@ def test_full_size_box2d2r_16384(T, torch, method):
     assert res["rel_l2"] <= 2e-3
     # coefficient + output rounding, max|x| <= 1 + 1e-3 (sum of amplitudes + noise)
     assert res["max_abs"] <= (2.0 ** -11) * 2.0 * 1.001 + 1e-6
+
+
+# ------------------------------------------------------- temporal blocking
+FUSED_TILES = [(2, 4), (4, 4), (4, 8), (8, 8)]
+
+
+def run_buffers(T, torch, p, A, sweeps, scale_exp=0, graph=False):
+    src = torch.from_numpy(np.ascontiguousarray(A)).cuda()
+    b0, b1 = p.alloc(), p.alloc()
+    p.clear(b1)
+    p.pack(src, b0, scale_exp)
+    if graph:
+        g = p.graph(b0, b1, sweeps)
+        g.launch()
+        torch.cuda.synchronize()
+        g.destroy()
+        res = b1 if sweeps % 2 else b0
+    else:
+        res = p.run(b0, b1, sweeps)
+    torch.cuda.synchronize()
+    return res.clone()
+
+
+@pytest.mark.parametrize("stencil", STENCILS)
+@pytest.mark.parametrize("precision", ["fp16", "bf16", "split"])
+def test_fused_bitwise_equals_unfused(T, torch, stencil, precision):
+    """k sweeps fused in LDS give the same bits as k single-sweep launches."""
+    r = RADIUS[stencil]
+    B = T.gen_coeffs(stencil, 21)
+    x = T.gen_uniform(300, 517, seed=21, dtype=torch.float32).cpu().numpy()
+    e = T.split_scale_exp(1.0) if precision == "split" else 0
+    for tile in FUSED_TILES:
+        ref = {}
+        for fuse in (2, 3, 1 + 16 // r):
+            sweeps = 2 * fuse + 1  # several launches, uneven split, odd parity
+            say("case fused", stencil, precision, tile, fuse, sweeps)
+            if sweeps not in ref:
+                pu = T.Problem(stencil, 300, 517, B, precision=precision, tile=tile)
+                ref[sweeps] = run_buffers(T, torch, pu, x, sweeps, e)
+            pf = T.Problem(stencil, 300, 517, B, precision=precision, tile=tile, fuse=fuse)
+            got = run_buffers(T, torch, pf, x, sweeps, e)
+            assert torch.equal(got, ref[sweeps]), f"fuse {fuse}"
+
+
+def test_fused_graph_and_accuracy(T, torch, oracle):
+    say("case fused graph")
+    A, B = random_problem(T, torch, "box2d2r", 200, 333, seed=22)
+    p = T.Problem("box2d2r", 200, 333, B.astype(np.float32), fuse=4)
+    eager = run_buffers(T, torch, p, A, 10)
+    graphed = run_buffers(T, torch, p, A, 10, graph=True)
+    assert torch.equal(eager, graphed)
+    out = p.unpack(eager).cpu().numpy()
+    ref = oracle.sweeps_f64("box2d2r", A, B, 10)
+    assert oracle.error_norms(out, ref, 2)["rel_l2"] <= 10 * 2e-3
+
+
+def test_fused_limits(T, torch):
+    B = T.gen_coeffs("box2d2r", 1)
+    p = T.Problem("box2d2r", 64, 64, B, fuse=10)  # radius 2 allows at most 9
+    b0, b1 = p.alloc(), p.alloc()
+    with pytest.raises(T.TcsError, match="fuse"):
+        p.run(b0, b1, 10)
+    q = T.Problem("box2d2r", 64, 64, B, method="direct", fuse=2)
+    with pytest.raises(T.TcsError, match="MFMA"):
+        q.run(b0, b1, 4)
