@@ -165,8 +165,6 @@ extern "C" int tcs_problem_set_tile(tcs_problem *p, int tile_h, int tile_w) {
     if (!p) return set_error(TCS_ERR_INVALID, "set_tile: null problem");
     if (!tile_offered(tile_h, tile_w))
         return set_error(TCS_ERR_UNSUPPORTED, "tile %dx%d is not offered", tile_h, tile_w);
-    if (p->precision == TCS_PREC_SPLIT && tile_h == 8 && tile_w == 8)
-        return set_error(TCS_ERR_UNSUPPORTED, "SPLIT has no 8x8 tile (LDS)");
     const int align = tile_h > tile_w ? tile_h : tile_w;
     int st = tcs_layout_init(&p->layout, p->layout.rows, p->layout.cols, align);
     if (st) return st;
