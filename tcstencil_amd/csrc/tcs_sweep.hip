@@ -209,8 +209,9 @@ __device__ __forceinline__ uint32_t pack2(uint16_t lo, uint16_t hi) {
 }
 
 // ------------------------------------------------------------- MFMA kernel
-// Operand reuse across row shifts through DPP cuts LDS reads ~4x; measured
-// time-neutral on 16384^2 sweeps, so it is a build option, off by default.
+// Operand reuse across row shifts through DPP cuts LDS reads ~4x.  The fused
+// kernels, which are LDS-bound, always use it; for single sweeps (HBM-bound)
+// it measured time-neutral, so there it is a build option, off by default.
 #ifndef TCS_DPP_SHIFT
 #define TCS_DPP_SHIFT 0
 #endif
@@ -289,7 +290,7 @@ struct Coef {
 // Lane (m, g) gets output row m, columns 4g..4g+3 in storage format (w[p]:
 // four 16-bit values of plane p).  gr, gcb: global row and first column of
 // those outputs; with `edge`, points outside the interior keep their input.
-template <class G, int ZLO, int ZHI, int PREC>
+template <class G, int ZLO, int ZHI, int PREC, bool SHIFT>
 __device__ __forceinline__ void mfma_brick(const SweepArgs &a, const uint16_t *lds, const Coef<ZLO, ZHI, PREC> &C,
                                            int br, int bc, int m, int g, bool edge, int gr, int gcb,
                                            u32x2 (&w)[G::NPL]) {
@@ -302,15 +303,15 @@ __device__ __forceinline__ void mfma_brick(const SweepArgs &a, const uint16_t *l
     const bool reads = g < 3;
     f32x4 acc = {0.0f, 0.0f, 0.0f, 0.0f};
     // Operand of row shift z: lane (m, g) holds haloed row m + z, columns
-    // 8g..8g+7.  With TCS_DPP_SHIFT, shift z+1 is shift z moved up one lane
-    // within each 16-lane row (DPP row_shl:1) and only lane 15 of each row
-    // reads a new LDS row.
+    // 8g..8g+7.  With SHIFT, shift z+1 is shift z moved up one lane within
+    // each 16-lane row (DPP row_shl:1) and only lane 15 of each row reads a
+    // new LDS row: 4x fewer LDS bytes per brick.
     u32x4 rh = {0u, 0u, 0u, 0u}, rl = {0u, 0u, 0u, 0u};
 #pragma unroll
     for (int zi = 0; zi < NZ; ++zi) {
         const int off = (16 * br + m + ZLO + zi) * G::LP + 16 * bc + 8 * g;
-        const bool fetch = reads && (TCS_DPP_SHIFT == 0 || zi == 0 || m == 15);
-        if (TCS_DPP_SHIFT && zi > 0) {
+        const bool fetch = reads && (!SHIFT || zi == 0 || m == 15);
+        if (SHIFT && zi > 0) {
             rh = rows_up(rh);
             if constexpr (PREC == TCS_PREC_SPLIT) rl = rows_up(rl);
         }
@@ -391,7 +392,7 @@ __global__ __launch_bounds__(kWG) void tcs_mfma_sweep(SweepArgs a) {
             const int br = b / TW, bc = b - br * TW;
             const int gr = cgr0 + 16 * br + m, gcb = cgc0 + 16 * bc + 4 * g;
             u32x2 w[NPL];
-            mfma_brick<T, ZLO, ZHI, PREC>(a, lds, C, br, bc, m, g, edge, gr, gcb, w);
+            mfma_brick<T, ZLO, ZHI, PREC, TCS_DPP_SHIFT != 0>(a, lds, C, br, bc, m, g, edge, gr, gcb, w);
             const int64_t s = stor_index(gr, gcb, a.sbc);
 #pragma unroll
             for (int p = 0; p < NPL; ++p) store8<true>(a.out + p * a.plane + s, w[p]);
@@ -448,8 +449,8 @@ __global__ __launch_bounds__(kWG) void tcs_mfma_fused(SweepArgs a, int nsweeps) 
         for (int q = 0; q < QA; ++q) {
             const int b = wave + 4 * q;
             const int br = b / G::BW, bc = b - br * G::BW;
-            mfma_brick<G, ZLO, ZHI, PREC>(a, lds, C, br, bc, m, g, edge, er0 + 16 * br + m, ec0 + 16 * bc + 4 * g,
-                                          res[q]);
+            mfma_brick<G, ZLO, ZHI, PREC, true>(a, lds, C, br, bc, m, g, edge, er0 + 16 * br + m,
+                                                ec0 + 16 * bc + 4 * g, res[q]);
         }
         __syncthreads();  // every wave has read this sweep's input
 #pragma unroll
@@ -471,7 +472,7 @@ __global__ __launch_bounds__(kWG) void tcs_mfma_fused(SweepArgs a, int nsweeps) 
         const int br = 1 + b / TW, bc = 1 + (b - (b / TW) * TW);
         const int gr = er0 + 16 * br + m, gcb = ec0 + 16 * bc + 4 * g;
         u32x2 w[NPL];
-        mfma_brick<G, ZLO, ZHI, PREC>(a, lds, C, br, bc, m, g, edge, gr, gcb, w);
+        mfma_brick<G, ZLO, ZHI, PREC, true>(a, lds, C, br, bc, m, g, edge, gr, gcb, w);
         const int64_t s = stor_index(gr, gcb, a.sbc);
 #pragma unroll
         for (int p = 0; p < NPL; ++p) store8<true>(a.out + p * a.plane + s, w[p]);
