@@ -13,7 +13,7 @@
 //                                           S_z[k][i] = X[i+z][k]
 //           is one v_mfma_f32_16x16x32_{f16,bf16} per row shift z: M = output
 //           column j, N = output row i, K = haloed column k (20 of 32 slots
-//           used, the rest multiply zero coefficients; slots 24..31 read no LDS).  The banded Toeplitz
+//           used, the rest multiply zero coefficients).  The banded Toeplitz
 //           C_z is the reference's init_coe matrix (layout16/host_box2d2r.cu:29-50)
 //           transposed and widened to cover the halo columns, so no scalar
 //           halo fix-ups are needed.  C_z lives in registers for the whole
@@ -298,34 +298,60 @@ __device__ __forceinline__ void mfma_brick(const SweepArgs &a, const uint16_t *l
     constexpr int NZ = ZHI - ZLO + 1;
     using F = Frag<PREC>;
     typedef typename F::type frag_t;
-    // Slot group 3 (k = 24..31) only meets zero coefficients: those lanes skip
-    // the LDS read and feed zeros.
-    const bool reads = g < 3;
     f32x4 acc = {0.0f, 0.0f, 0.0f, 0.0f};
-    // Operand of row shift z: lane (m, g) holds haloed row m + z, columns
-    // 8g..8g+7.  With SHIFT, shift z+1 is shift z moved up one lane within
-    // each 16-lane row (DPP row_shl:1) and only lane 15 of each row reads a
-    // new LDS row: 4x fewer LDS bytes per brick.
-    u32x4 rh = {0u, 0u, 0u, 0u}, rl = {0u, 0u, 0u, 0u};
+    if constexpr (!SHIFT) {
+        // Operand of row shift z: lane (m, g) holds haloed row m + z, columns
+        // 8g..8g+7.  Slot group 3 (k = 24..31) only meets zero coefficients; it
+        // re-reads group 2's (finite) columns so that every lane reads and all
+        // NZ reads issue back to back, ahead of the MFMAs.
+        const int gs = g < 2 ? g : 2;
+        u32x4 rh[NZ], rl[NZ];
 #pragma unroll
-    for (int zi = 0; zi < NZ; ++zi) {
-        const int off = (16 * br + m + ZLO + zi) * G::LP + 16 * bc + 8 * g;
-        const bool fetch = reads && (!SHIFT || zi == 0 || m == 15);
-        if (SHIFT && zi > 0) {
-            rh = rows_up(rh);
-            if constexpr (PREC == TCS_PREC_SPLIT) rl = rows_up(rl);
+        for (int zi = 0; zi < NZ; ++zi) {
+            const int off = (16 * br + m + ZLO + zi) * G::LP + 16 * bc + 8 * gs;
+            rh[zi] = *reinterpret_cast<const u32x4 *>(lds + off);
+            if constexpr (PREC == TCS_PREC_SPLIT) rl[zi] = *reinterpret_cast<const u32x4 *>(lds + G::PLANE + off);
         }
-        if (fetch) {
-            rh = *reinterpret_cast<const u32x4 *>(lds + off);
-            if constexpr (PREC == TCS_PREC_SPLIT) rl = *reinterpret_cast<const u32x4 *>(lds + G::PLANE + off);
+        // Keep the reads ahead of the MFMAs (the scheduler would otherwise
+        // reuse one register quad and wait out each LDS round trip in turn).
+        __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+        for (int zi = 0; zi < NZ; ++zi) {
+            const frag_t bh = __builtin_bit_cast(frag_t, rh[zi]);
+            acc = F::mma(C.hi[zi], bh, acc);
+            if constexpr (PREC == TCS_PREC_SPLIT) {
+                const frag_t bl = __builtin_bit_cast(frag_t, rl[zi]);
+                acc = F::mma(C.hi[zi], bl, acc);
+                acc = F::mma(C.lo[zi], bh, acc);
+                acc = F::mma(C.lo[zi], bl, acc);
+            }
         }
-        const frag_t bh = __builtin_bit_cast(frag_t, rh);
-        acc = F::mma(C.hi[zi], bh, acc);
-        if constexpr (PREC == TCS_PREC_SPLIT) {
-            const frag_t bl = __builtin_bit_cast(frag_t, rl);
-            acc = F::mma(C.hi[zi], bl, acc);
-            acc = F::mma(C.lo[zi], bh, acc);
-            acc = F::mma(C.lo[zi], bl, acc);
+    } else {
+        // Shift z+1 is shift z moved up one lane within each 16-lane row (DPP
+        // row_shl:1); only lane 15 of each row reads a new LDS row (4x fewer
+        // LDS bytes).  Slot group 3 reads nothing and feeds zeros.
+        const bool reads = g < 3;
+        u32x4 rh = {0u, 0u, 0u, 0u}, rl = {0u, 0u, 0u, 0u};
+#pragma unroll
+        for (int zi = 0; zi < NZ; ++zi) {
+            const int off = (16 * br + m + ZLO + zi) * G::LP + 16 * bc + 8 * g;
+            const bool fetch = reads && (zi == 0 || m == 15);
+            if (zi > 0) {
+                rh = rows_up(rh);
+                if constexpr (PREC == TCS_PREC_SPLIT) rl = rows_up(rl);
+            }
+            if (fetch) {
+                rh = *reinterpret_cast<const u32x4 *>(lds + off);
+                if constexpr (PREC == TCS_PREC_SPLIT) rl = *reinterpret_cast<const u32x4 *>(lds + G::PLANE + off);
+            }
+            const frag_t bh = __builtin_bit_cast(frag_t, rh);
+            acc = F::mma(C.hi[zi], bh, acc);
+            if constexpr (PREC == TCS_PREC_SPLIT) {
+                const frag_t bl = __builtin_bit_cast(frag_t, rl);
+                acc = F::mma(C.hi[zi], bl, acc);
+                acc = F::mma(C.lo[zi], bh, acc);
+                acc = F::mma(C.lo[zi], bl, acc);
+            }
         }
     }
     uint16_t o[NPL][4];
@@ -449,7 +475,7 @@ __global__ __launch_bounds__(kWG) void tcs_mfma_fused(SweepArgs a, int nsweeps) 
         for (int q = 0; q < QA; ++q) {
             const int b = wave + 4 * q;
             const int br = b / G::BW, bc = b - br * G::BW;
-            mfma_brick<G, ZLO, ZHI, PREC, true>(a, lds, C, br, bc, m, g, edge, er0 + 16 * br + m,
+            mfma_brick<G, ZLO, ZHI, PREC, false>(a, lds, C, br, bc, m, g, edge, er0 + 16 * br + m,
                                                 ec0 + 16 * bc + 4 * g, res[q]);
         }
         __syncthreads();  // every wave has read this sweep's input
@@ -472,7 +498,7 @@ __global__ __launch_bounds__(kWG) void tcs_mfma_fused(SweepArgs a, int nsweeps) 
         const int br = 1 + b / TW, bc = 1 + (b - (b / TW) * TW);
         const int gr = er0 + 16 * br + m, gcb = ec0 + 16 * bc + 4 * g;
         u32x2 w[NPL];
-        mfma_brick<G, ZLO, ZHI, PREC, true>(a, lds, C, br, bc, m, g, edge, gr, gcb, w);
+        mfma_brick<G, ZLO, ZHI, PREC, false>(a, lds, C, br, bc, m, g, edge, gr, gcb, w);
         const int64_t s = stor_index(gr, gcb, a.sbc);
 #pragma unroll
         for (int p = 0; p < NPL; ++p) store8<true>(a.out + p * a.plane + s, w[p]);
