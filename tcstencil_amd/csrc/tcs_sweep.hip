@@ -17,7 +17,8 @@
 //           C_z is the reference's init_coe matrix (layout16/host_box2d2r.cu:29-50)
 //           transposed and widened to cover the halo columns, so no scalar
 //           halo fix-ups are needed.  C_z lives in registers for the whole
-//           kernel; S_z is one ds_read_b128 per lane.  The FP32 accumulator of
+//           kernel; S_z is one ds_read_b128 per lane, and a brick's reads
+//           are issued while the previous brick's MFMAs run.  The FP32 accumulator of
 //           lane l holds output row l%16, columns 4*(l/16)..+3, which is one
 //           8-byte store into the row-major brick.
 //   DIRECT: each thread computes a 4-column x MR-row strip with FP32 FMAs on
@@ -209,23 +210,6 @@ __device__ __forceinline__ uint32_t pack2(uint16_t lo, uint16_t hi) {
 }
 
 // ------------------------------------------------------------- MFMA kernel
-// Operand reuse across row shifts through DPP cuts LDS reads ~4x.  The fused
-// kernels, which are LDS-bound, always use it; for single sweeps (HBM-bound)
-// it measured time-neutral, so there it is a build option, off by default.
-#ifndef TCS_DPP_SHIFT
-#define TCS_DPP_SHIFT 0
-#endif
-
-// Lane n of each 16-lane row takes lane n+1's value (DPP row_shl:1); lane 15
-// of each row receives 0.
-__device__ __forceinline__ u32x4 rows_up(u32x4 v) {
-    u32x4 r;
-#pragma unroll
-    for (int q = 0; q < 4; ++q)
-        r[q] = (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v[q], 0x101, 0xf, 0xf, true);
-    return r;
-}
-
 template <int PREC>
 struct Frag;
 template <>
@@ -285,73 +269,50 @@ struct Coef {
     }
 };
 
-// One 16x16 output brick from the LDS image of geometry G: brick (br, bc) of
-// the image, whose haloed 20x20 window starts at LDS row 16*br, column 16*bc.
-// Lane (m, g) gets output row m, columns 4g..4g+3 in storage format (w[p]:
-// four 16-bit values of plane p).  gr, gcb: global row and first column of
-// those outputs; with `edge`, points outside the interior keep their input.
-template <class G, int ZLO, int ZHI, int PREC, bool SHIFT>
+// Operands of one 16x16 output brick: brick (br, bc) of the LDS image of
+// geometry G, whose haloed 20x20 window starts at LDS row 16*br, column 16*bc.
+// Row shift z: lane (m, g) holds haloed row m + z, columns 8g..8g+7.  Slot
+// group 3 (k = 24..31) only meets zero coefficients; it re-reads group 2's
+// (finite) columns so that every lane reads and the reads issue back to back.
+template <class G, int ZLO, int ZHI, int PREC>
+struct Ops {
+    static constexpr int NZ = ZHI - ZLO + 1;
+    u32x4 h[NZ];
+    u32x4 l[PREC == TCS_PREC_SPLIT ? NZ : 1];
+
+    __device__ __forceinline__ void load(const uint16_t *lds, int br, int bc, int m, int g) {
+        const int gs = g < 2 ? g : 2;
+#pragma unroll
+        for (int zi = 0; zi < NZ; ++zi) {
+            const int off = (16 * br + m + ZLO + zi) * G::LP + 16 * bc + 8 * gs;
+            h[zi] = *reinterpret_cast<const u32x4 *>(lds + off);
+            if constexpr (PREC == TCS_PREC_SPLIT) l[zi] = *reinterpret_cast<const u32x4 *>(lds + G::PLANE + off);
+        }
+    }
+};
+
+// The brick's MFMAs and epilogue: lane (m, g) gets output row m, columns
+// 4g..4g+3 in storage format (w[p]: four 16-bit values of plane p).  gr, gcb:
+// global row and first column of those outputs; with `edge`, points outside
+// the interior keep their input value (read from the LDS image).
+template <class G, int ZLO, int ZHI, int PREC>
 __device__ __forceinline__ void mfma_brick(const SweepArgs &a, const uint16_t *lds, const Coef<ZLO, ZHI, PREC> &C,
-                                           int br, int bc, int m, int g, bool edge, int gr, int gcb,
-                                           u32x2 (&w)[G::NPL]) {
+                                           const Ops<G, ZLO, ZHI, PREC> &op, int br, int bc, int m, int g,
+                                           bool edge, int gr, int gcb, u32x2 (&w)[G::NPL]) {
     constexpr int NPL = G::NPL;
     constexpr int NZ = ZHI - ZLO + 1;
     using F = Frag<PREC>;
     typedef typename F::type frag_t;
     f32x4 acc = {0.0f, 0.0f, 0.0f, 0.0f};
-    if constexpr (!SHIFT) {
-        // Operand of row shift z: lane (m, g) holds haloed row m + z, columns
-        // 8g..8g+7.  Slot group 3 (k = 24..31) only meets zero coefficients; it
-        // re-reads group 2's (finite) columns so that every lane reads and all
-        // NZ reads issue back to back, ahead of the MFMAs.
-        const int gs = g < 2 ? g : 2;
-        u32x4 rh[NZ], rl[NZ];
 #pragma unroll
-        for (int zi = 0; zi < NZ; ++zi) {
-            const int off = (16 * br + m + ZLO + zi) * G::LP + 16 * bc + 8 * gs;
-            rh[zi] = *reinterpret_cast<const u32x4 *>(lds + off);
-            if constexpr (PREC == TCS_PREC_SPLIT) rl[zi] = *reinterpret_cast<const u32x4 *>(lds + G::PLANE + off);
-        }
-        // Keep the reads ahead of the MFMAs (the scheduler would otherwise
-        // reuse one register quad and wait out each LDS round trip in turn).
-        __builtin_amdgcn_sched_barrier(0);
-#pragma unroll
-        for (int zi = 0; zi < NZ; ++zi) {
-            const frag_t bh = __builtin_bit_cast(frag_t, rh[zi]);
-            acc = F::mma(C.hi[zi], bh, acc);
-            if constexpr (PREC == TCS_PREC_SPLIT) {
-                const frag_t bl = __builtin_bit_cast(frag_t, rl[zi]);
-                acc = F::mma(C.hi[zi], bl, acc);
-                acc = F::mma(C.lo[zi], bh, acc);
-                acc = F::mma(C.lo[zi], bl, acc);
-            }
-        }
-    } else {
-        // Shift z+1 is shift z moved up one lane within each 16-lane row (DPP
-        // row_shl:1); only lane 15 of each row reads a new LDS row (4x fewer
-        // LDS bytes).  Slot group 3 reads nothing and feeds zeros.
-        const bool reads = g < 3;
-        u32x4 rh = {0u, 0u, 0u, 0u}, rl = {0u, 0u, 0u, 0u};
-#pragma unroll
-        for (int zi = 0; zi < NZ; ++zi) {
-            const int off = (16 * br + m + ZLO + zi) * G::LP + 16 * bc + 8 * g;
-            const bool fetch = reads && (zi == 0 || m == 15);
-            if (zi > 0) {
-                rh = rows_up(rh);
-                if constexpr (PREC == TCS_PREC_SPLIT) rl = rows_up(rl);
-            }
-            if (fetch) {
-                rh = *reinterpret_cast<const u32x4 *>(lds + off);
-                if constexpr (PREC == TCS_PREC_SPLIT) rl = *reinterpret_cast<const u32x4 *>(lds + G::PLANE + off);
-            }
-            const frag_t bh = __builtin_bit_cast(frag_t, rh);
-            acc = F::mma(C.hi[zi], bh, acc);
-            if constexpr (PREC == TCS_PREC_SPLIT) {
-                const frag_t bl = __builtin_bit_cast(frag_t, rl);
-                acc = F::mma(C.hi[zi], bl, acc);
-                acc = F::mma(C.lo[zi], bh, acc);
-                acc = F::mma(C.lo[zi], bl, acc);
-            }
+    for (int zi = 0; zi < NZ; ++zi) {
+        const frag_t bh = __builtin_bit_cast(frag_t, op.h[zi]);
+        acc = F::mma(C.hi[zi], bh, acc);
+        if constexpr (PREC == TCS_PREC_SPLIT) {
+            const frag_t bl = __builtin_bit_cast(frag_t, op.l[zi]);
+            acc = F::mma(C.hi[zi], bl, acc);
+            acc = F::mma(C.lo[zi], bh, acc);
+            acc = F::mma(C.lo[zi], bl, acc);
         }
     }
     uint16_t o[NPL][4];
@@ -379,6 +340,13 @@ __device__ __forceinline__ void mfma_brick(const SweepArgs &a, const uint16_t *l
     }
 #pragma unroll
     for (int p = 0; p < NPL; ++p) w[p] = u32x2{pack2(o[p][0], o[p][1]), pack2(o[p][2], o[p][3])};
+}
+
+// Brick q of a wave's list -> (br, bc) in the LDS image.
+template <int BW, int BASE>
+__device__ __forceinline__ void brick_of(int b, int &br, int &bc) {
+    br = BASE + b / BW;
+    bc = BASE + (b - (b / BW) * BW);
 }
 
 template <int TH, int TW, int ZLO, int ZHI, int PREC, bool PERSIST>
@@ -412,16 +380,30 @@ __global__ __launch_bounds__(kWG) void tcs_mfma_sweep(SweepArgs a) {
             load_tile<T, false>(a, R, gr0, gc0, tid);  // in flight during the compute below
         }
         const bool edge = !tile_is_interior(a, cgr0, cgc0, 16 * TH, 16 * TW);
+        // Operand reads run one brick ahead of the MFMAs.
+        Ops<T, ZLO, ZHI, PREC> cur, nxt;
+        {
+            int br, bc;
+            brick_of<TW, 0>(wave, br, bc);
+            cur.load(lds, br, bc, m, g);
+        }
 #pragma unroll
         for (int q = 0; q < NB; ++q) {
-            const int b = wave + 4 * q;
-            const int br = b / TW, bc = b - br * TW;
+            int br, bc;
+            brick_of<TW, 0>(wave + 4 * q, br, bc);
+            if (q + 1 < NB) {
+                int nbr, nbc;
+                brick_of<TW, 0>(wave + 4 * (q + 1), nbr, nbc);
+                nxt.load(lds, nbr, nbc, m, g);
+            }
+            __builtin_amdgcn_sched_barrier(0);
             const int gr = cgr0 + 16 * br + m, gcb = cgc0 + 16 * bc + 4 * g;
             u32x2 w[NPL];
-            mfma_brick<T, ZLO, ZHI, PREC, TCS_DPP_SHIFT != 0>(a, lds, C, br, bc, m, g, edge, gr, gcb, w);
+            mfma_brick<T, ZLO, ZHI, PREC>(a, lds, C, cur, br, bc, m, g, edge, gr, gcb, w);
             const int64_t s = stor_index(gr, gcb, a.sbc);
 #pragma unroll
             for (int p = 0; p < NPL; ++p) store8<true>(a.out + p * a.plane + s, w[p]);
+            cur = nxt;
         }
         // One-tile-per-workgroup launches cover every tile with the grid itself.
         if (!PERSIST || next >= sc.end) break;
@@ -471,12 +453,25 @@ __global__ __launch_bounds__(kWG) void tcs_mfma_fused(SweepArgs a, int nsweeps) 
     uint32_t *l32 = reinterpret_cast<uint32_t *>(lds);
     for (int sw = 0; sw + 1 < nsweeps; ++sw) {
         u32x2 res[QA][NPL];
+        Ops<G, ZLO, ZHI, PREC> cur, nxt;
+        {
+            int br, bc;
+            brick_of<G::BW, 0>(wave, br, bc);
+            cur.load(lds, br, bc, m, g);
+        }
 #pragma unroll
         for (int q = 0; q < QA; ++q) {
-            const int b = wave + 4 * q;
-            const int br = b / G::BW, bc = b - br * G::BW;
-            mfma_brick<G, ZLO, ZHI, PREC, false>(a, lds, C, br, bc, m, g, edge, er0 + 16 * br + m,
-                                                ec0 + 16 * bc + 4 * g, res[q]);
+            int br, bc;
+            brick_of<G::BW, 0>(wave + 4 * q, br, bc);
+            if (q + 1 < QA) {
+                int nbr, nbc;
+                brick_of<G::BW, 0>(wave + 4 * (q + 1), nbr, nbc);
+                nxt.load(lds, nbr, nbc, m, g);
+            }
+            __builtin_amdgcn_sched_barrier(0);
+            mfma_brick<G, ZLO, ZHI, PREC>(a, lds, C, cur, br, bc, m, g, edge, er0 + 16 * br + m,
+                                          ec0 + 16 * bc + 4 * g, res[q]);
+            cur = nxt;
         }
         __syncthreads();  // every wave has read this sweep's input
 #pragma unroll
@@ -492,16 +487,29 @@ __global__ __launch_bounds__(kWG) void tcs_mfma_fused(SweepArgs a, int nsweeps) 
         }
         __syncthreads();
     }
+    Ops<G, ZLO, ZHI, PREC> cur, nxt;
+    {
+        int br, bc;
+        brick_of<TW, 1>(wave, br, bc);
+        cur.load(lds, br, bc, m, g);
+    }
 #pragma unroll
     for (int q = 0; q < QC; ++q) {
-        const int b = wave + 4 * q;
-        const int br = 1 + b / TW, bc = 1 + (b - (b / TW) * TW);
+        int br, bc;
+        brick_of<TW, 1>(wave + 4 * q, br, bc);
+        if (q + 1 < QC) {
+            int nbr, nbc;
+            brick_of<TW, 1>(wave + 4 * (q + 1), nbr, nbc);
+            nxt.load(lds, nbr, nbc, m, g);
+        }
+        __builtin_amdgcn_sched_barrier(0);
         const int gr = er0 + 16 * br + m, gcb = ec0 + 16 * bc + 4 * g;
         u32x2 w[NPL];
-        mfma_brick<G, ZLO, ZHI, PREC, false>(a, lds, C, br, bc, m, g, edge, gr, gcb, w);
+        mfma_brick<G, ZLO, ZHI, PREC>(a, lds, C, cur, br, bc, m, g, edge, gr, gcb, w);
         const int64_t s = stor_index(gr, gcb, a.sbc);
 #pragma unroll
         for (int p = 0; p < NPL; ++p) store8<true>(a.out + p * a.plane + s, w[p]);
+        cur = nxt;
     }
 }
 

@@ -202,35 +202,17 @@ def _afrag(coeff5, rs, round_coeff=f16):
     return afrag, ZLO, ZHI
 
 
-def _mfma_brick(lds, G: TileGeom, afrag, ZLO, ZHI, br, bc, edge, gr, gcb, L: Layout, rs, shift=False):
-    """mfma_brick: returns o[lane][v] (FP16 values) for LDS brick (br, bc).
-    shift=False: every lane reads (slot group 3 re-reads group 2's columns);
-    shift=True: the DPP row_shl:1 path (group 3 reads nothing, feeds zeros)."""
+def _mfma_brick(lds, G: TileGeom, afrag, ZLO, ZHI, br, bc, edge, gr, gcb, L: Layout, rs):
+    """Ops::load + mfma_brick: returns o[lane][v] (FP16 values) for LDS brick
+    (br, bc).  Every lane reads; slot group 3 re-reads group 2's columns."""
     lane = np.arange(64)
     m, g = lane & 15, lane >> 4
     D = np.zeros((16, 16))
-    bfrag = np.zeros((64, 8))
     for zi in range(ZHI - ZLO + 1):
-        if not shift:
-            gs = np.minimum(g, 2)
-            off = (16 * br + m + ZLO + zi) * G.LP + 16 * bc + 8 * gs
-            _check(off, G.PLANE - 7, 8, "B operand LDS read (16 B)")
-            bfrag = lds[off[:, None] + np.arange(8)[None, :]]  # [lane][e]
-        else:
-            reads = g < 3
-            off = (16 * br + m + ZLO + zi) * G.LP + 16 * bc + 8 * g
-            fetch = reads & ((zi == 0) | (m == 15))
-            if zi > 0:  # DPP row_shl:1 within each 16-lane row; lane 15 gets 0
-                shifted = np.zeros((64, 8))
-                for l in range(64):
-                    if l % 16 != 15:
-                        shifted[l] = bfrag[l + 1]
-                bfrag = shifted
-            _check(off[fetch], G.PLANE - 7, 8, "B operand LDS read (16 B)")
-            bfrag[fetch] = lds[off[fetch][:, None] + np.arange(8)[None, :]]
-            direct = np.zeros((64, 8))
-            direct[reads] = lds[off[reads][:, None] + np.arange(8)[None, :]]
-            assert np.array_equal(bfrag, direct, equal_nan=True), "shifted operand differs from LDS rows"
+        gs = np.minimum(g, 2)
+        off = (16 * br + m + ZLO + zi) * G.LP + 16 * bc + 8 * gs
+        _check(off, G.PLANE - 7, 8, "B operand LDS read (16 B)")
+        bfrag = lds[off[:, None] + np.arange(8)[None, :]]  # [lane][e]
         Am = np.zeros((16, 32))
         Bm = np.zeros((32, 16))
         for l in range(64):
@@ -257,7 +239,7 @@ def _store(out, written, L, gr, gcb, o):
         written[s + v] = True
 
 
-def mfma_sweep(inp, L: Layout, TH, TW, coeff5, rs, round_coeff=f16, per_xcd=0, shift=False):
+def mfma_sweep(inp, L: Layout, TH, TW, coeff5, rs, round_coeff=f16, per_xcd=0):
     """tcs_mfma_sweep for FP16 (one plane).  coeff5: 5x5 FP32 embedding."""
     assert L.nbr % TH == 0 and L.nbc % TW == 0
     G = TileGeom(TH, TW)
@@ -273,7 +255,7 @@ def mfma_sweep(inp, L: Layout, TH, TW, coeff5, rs, round_coeff=f16, per_xcd=0, s
                 b = wave + 4 * q
                 br, bc = b // TW, b % TW
                 gr, gcb = gr0 + 16 * br + m, gc0 + 16 * bc + 4 * g
-                o = _mfma_brick(lds, G, afrag, ZLO, ZHI, br, bc, edge, gr, gcb, L, rs, shift)
+                o = _mfma_brick(lds, G, afrag, ZLO, ZHI, br, bc, edge, gr, gcb, L, rs)
                 _store(out, written, L, gr, gcb, o)
     return out, written
 

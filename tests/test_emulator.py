@@ -41,7 +41,7 @@ def test_emulated_kernel_exact(oracle, stencil, case, method):
     st = E.pack(A, L)
     per_xcd = (rows + cols) % 3  # 0 (one tile per workgroup), 1 or 2 persistent workgroups per XCD
     if method == "mfma":
-        out, written = E.mfma_sweep(st, L, TH, TW, c5, r, per_xcd=per_xcd, shift=bool(rows % 2))
+        out, written = E.mfma_sweep(st, L, TH, TW, c5, r, per_xcd=per_xcd)
     else:
         out, written = E.direct_sweep(st, L, TH, TW, c5, stencil, r, per_xcd=per_xcd)
     # every data brick is written, the ghost ring never
