@@ -349,9 +349,16 @@ __device__ __forceinline__ void brick_of(int b, int &br, int &bc) {
     bc = BASE + (b - (b / BW) * BW);
 }
 
+// Operand look-ahead across bricks costs 4*NZ registers (8*NZ for SPLIT); it
+// pays for FP16/BF16 and measured slower for SPLIT, whose four MFMA passes per
+// shift already hide the LDS latency.
+template <int PREC>
+constexpr bool kLookahead = PREC != TCS_PREC_SPLIT;
+
 template <int TH, int TW, int ZLO, int ZHI, int PREC, bool PERSIST>
 __global__ __launch_bounds__(kWG) void tcs_mfma_sweep(SweepArgs a) {
     constexpr int NPL = PREC == TCS_PREC_SPLIT ? 2 : 1;
+    constexpr bool LOOKAHEAD = kLookahead<PREC>;
     using T = Tile<TH, TW, NPL>;
     __shared__ __attribute__((aligned(16))) uint16_t lds[NPL * T::PLANE];
 
@@ -380,9 +387,9 @@ __global__ __launch_bounds__(kWG) void tcs_mfma_sweep(SweepArgs a) {
             load_tile<T, false>(a, R, gr0, gc0, tid);  // in flight during the compute below
         }
         const bool edge = !tile_is_interior(a, cgr0, cgc0, 16 * TH, 16 * TW);
-        // Operand reads run one brick ahead of the MFMAs.
+        // With LOOKAHEAD, operand reads run one brick ahead of the MFMAs.
         Ops<T, ZLO, ZHI, PREC> cur, nxt;
-        {
+        if constexpr (LOOKAHEAD) {
             int br, bc;
             brick_of<TW, 0>(wave, br, bc);
             cur.load(lds, br, bc, m, g);
@@ -391,10 +398,14 @@ __global__ __launch_bounds__(kWG) void tcs_mfma_sweep(SweepArgs a) {
         for (int q = 0; q < NB; ++q) {
             int br, bc;
             brick_of<TW, 0>(wave + 4 * q, br, bc);
-            if (q + 1 < NB) {
-                int nbr, nbc;
-                brick_of<TW, 0>(wave + 4 * (q + 1), nbr, nbc);
-                nxt.load(lds, nbr, nbc, m, g);
+            if constexpr (LOOKAHEAD) {
+                if (q + 1 < NB) {
+                    int nbr, nbc;
+                    brick_of<TW, 0>(wave + 4 * (q + 1), nbr, nbc);
+                    nxt.load(lds, nbr, nbc, m, g);
+                }
+            } else {
+                cur.load(lds, br, bc, m, g);
             }
             __builtin_amdgcn_sched_barrier(0);
             const int gr = cgr0 + 16 * br + m, gcb = cgc0 + 16 * bc + 4 * g;
@@ -403,7 +414,7 @@ __global__ __launch_bounds__(kWG) void tcs_mfma_sweep(SweepArgs a) {
             const int64_t s = stor_index(gr, gcb, a.sbc);
 #pragma unroll
             for (int p = 0; p < NPL; ++p) store8<true>(a.out + p * a.plane + s, w[p]);
-            cur = nxt;
+            if constexpr (LOOKAHEAD) cur = nxt;
         }
         // One-tile-per-workgroup launches cover every tile with the grid itself.
         if (!PERSIST || next >= sc.end) break;
@@ -425,6 +436,7 @@ __global__ __launch_bounds__(kWG) void tcs_mfma_sweep(SweepArgs a) {
 template <int TH, int TW, int ZLO, int ZHI, int PREC>
 __global__ __launch_bounds__(kWG) void tcs_mfma_fused(SweepArgs a, int nsweeps) {
     constexpr int NPL = PREC == TCS_PREC_SPLIT ? 2 : 1;
+    constexpr bool LOOKAHEAD = kLookahead<PREC>;
     using G = Geom<TH + 2, TW + 2, NPL>;
     constexpr int QA = G::BH * G::BW / 4;  // bricks per wave, whole image
     constexpr int QC = TH * TW / 4;        // bricks per wave, core only
@@ -454,7 +466,7 @@ __global__ __launch_bounds__(kWG) void tcs_mfma_fused(SweepArgs a, int nsweeps) 
     for (int sw = 0; sw + 1 < nsweeps; ++sw) {
         u32x2 res[QA][NPL];
         Ops<G, ZLO, ZHI, PREC> cur, nxt;
-        {
+        if constexpr (LOOKAHEAD) {
             int br, bc;
             brick_of<G::BW, 0>(wave, br, bc);
             cur.load(lds, br, bc, m, g);
@@ -463,15 +475,19 @@ __global__ __launch_bounds__(kWG) void tcs_mfma_fused(SweepArgs a, int nsweeps) 
         for (int q = 0; q < QA; ++q) {
             int br, bc;
             brick_of<G::BW, 0>(wave + 4 * q, br, bc);
-            if (q + 1 < QA) {
-                int nbr, nbc;
-                brick_of<G::BW, 0>(wave + 4 * (q + 1), nbr, nbc);
-                nxt.load(lds, nbr, nbc, m, g);
+            if constexpr (LOOKAHEAD) {
+                if (q + 1 < QA) {
+                    int nbr, nbc;
+                    brick_of<G::BW, 0>(wave + 4 * (q + 1), nbr, nbc);
+                    nxt.load(lds, nbr, nbc, m, g);
+                }
+            } else {
+                cur.load(lds, br, bc, m, g);
             }
             __builtin_amdgcn_sched_barrier(0);
             mfma_brick<G, ZLO, ZHI, PREC>(a, lds, C, cur, br, bc, m, g, edge, er0 + 16 * br + m,
                                           ec0 + 16 * bc + 4 * g, res[q]);
-            cur = nxt;
+            if constexpr (LOOKAHEAD) cur = nxt;
         }
         __syncthreads();  // every wave has read this sweep's input
 #pragma unroll
@@ -488,7 +504,7 @@ __global__ __launch_bounds__(kWG) void tcs_mfma_fused(SweepArgs a, int nsweeps) 
         __syncthreads();
     }
     Ops<G, ZLO, ZHI, PREC> cur, nxt;
-    {
+    if constexpr (LOOKAHEAD) {
         int br, bc;
         brick_of<TW, 1>(wave, br, bc);
         cur.load(lds, br, bc, m, g);
@@ -497,10 +513,14 @@ __global__ __launch_bounds__(kWG) void tcs_mfma_fused(SweepArgs a, int nsweeps) 
     for (int q = 0; q < QC; ++q) {
         int br, bc;
         brick_of<TW, 1>(wave + 4 * q, br, bc);
-        if (q + 1 < QC) {
-            int nbr, nbc;
-            brick_of<TW, 1>(wave + 4 * (q + 1), nbr, nbc);
-            nxt.load(lds, nbr, nbc, m, g);
+        if constexpr (LOOKAHEAD) {
+            if (q + 1 < QC) {
+                int nbr, nbc;
+                brick_of<TW, 1>(wave + 4 * (q + 1), nbr, nbc);
+                nxt.load(lds, nbr, nbc, m, g);
+            }
+        } else {
+            cur.load(lds, br, bc, m, g);
         }
         __builtin_amdgcn_sched_barrier(0);
         const int gr = er0 + 16 * br + m, gcb = ec0 + 16 * bc + 4 * g;
@@ -509,7 +529,7 @@ __global__ __launch_bounds__(kWG) void tcs_mfma_fused(SweepArgs a, int nsweeps) 
         const int64_t s = stor_index(gr, gcb, a.sbc);
 #pragma unroll
         for (int p = 0; p < NPL; ++p) store8<true>(a.out + p * a.plane + s, w[p]);
-        cur = nxt;
+        if constexpr (LOOKAHEAD) cur = nxt;
     }
 }
 
