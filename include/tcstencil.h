@@ -183,6 +183,24 @@ int tcs_graph_create(const tcs_problem *p, void *d_buf0, void *d_buf1, int sweep
 int tcs_graph_launch(void *graph_exec, void *stream);
 int tcs_graph_destroy(void *graph_exec);
 
+/* ---------------------------------------------------- batched uneven grids */
+/* Many independent grids (own size, stencil, coefficients and buffers) swept
+ * together: one launch per sweep for all problems that share a tile shape, so a
+ * sweep of the whole batch is a handful of launches instead of one per grid.
+ * All problems: MFMA, the same precision (FP16 or BF16), tiles 2x4, 4x4 or 4x8
+ * (per problem).  d_buf0[i] / d_buf1[i] are problem i's two storage buffers
+ * (d_buf1 ghost ring zero); sweep s reads buf[s % 2], so after tcs_batch_run the
+ * results are in buf[sweeps % 2].  The batch owns device copies of the
+ * descriptors (allocated here, freed by tcs_batch_destroy). */
+typedef struct tcs_batch tcs_batch;
+int tcs_batch_create(tcs_batch **b, int nprob, const tcs_problem *probs, void *const *d_buf0,
+                     void *const *d_buf1);
+int tcs_batch_run(tcs_batch *b, int sweeps, void *stream);
+int tcs_batch_graph_create(tcs_batch *b, int sweeps, void *stream, void **graph_exec);
+int tcs_batch_destroy(tcs_batch *b);
+int64_t tcs_batch_interior_points(const tcs_batch *b);
+int tcs_batch_launches_per_sweep(const tcs_batch *b);
+
 /* ------------------------------------------------ FP64 oracle and error norms */
 /* One FP64 sweep on row-major grids (ld = cols), gold's loop and accumulation
  * order with no fused multiply-add (common/gold_*.cpp); the ring of d_out is not

@@ -93,6 +93,12 @@ SIGNATURES = {
     "tcs_graph_create": (_i, [_prob, _vp, _vp, _i, _vp, _P(_vp)]),
     "tcs_graph_launch": (_i, [_vp, _vp]),
     "tcs_graph_destroy": (_i, [_vp]),
+    "tcs_batch_create": (_i, [_P(_vp), _i, _prob, _P(_vp), _P(_vp)]),
+    "tcs_batch_run": (_i, [_vp, _i, _vp]),
+    "tcs_batch_graph_create": (_i, [_vp, _i, _vp, _P(_vp)]),
+    "tcs_batch_destroy": (_i, [_vp]),
+    "tcs_batch_interior_points": (_i64, [_vp]),
+    "tcs_batch_launches_per_sweep": (_i, [_vp]),
     "tcs_oracle_sweep_f64": (_i, [_i, _vp, _dp, _i, _vp, _i, _i, _vp]),
     "tcs_error_norms": (_i, [_vp, _vp, _i, _i, _i, _dp, _vp]),
     "tcs_gen_uniform_f32": (_i, [_vp, _i, _i, _u64, _u64, ctypes.c_float, ctypes.c_float, _vp]),

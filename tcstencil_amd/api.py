@@ -194,6 +194,49 @@ class Graph:
             pass
 
 
+class Batch:
+    """Batched uneven grids (tcs_batch): problems[i] swept between bufs0[i] and
+    bufs1[i] (device tensors from problems[i].alloc())."""
+
+    def __init__(self, problems, bufs0, bufs1):
+        n = len(problems)
+        if not (n == len(bufs0) == len(bufs1)) or n == 0:
+            raise ValueError("batch: need one problem and two buffers per grid")
+        arr = (L.ProblemStruct * n)(*[p.s for p in problems])
+        p0 = (ctypes.c_void_p * n)(*[_ptr(b).value for b in bufs0])
+        p1 = (ctypes.c_void_p * n)(*[_ptr(b).value for b in bufs1])
+        self.h = ctypes.c_void_p()
+        check(lib().tcs_batch_create(ctypes.byref(self.h), n, arr, p0, p1))
+        self._keep = (problems, bufs0, bufs1)
+
+    @property
+    def interior_points(self) -> int:
+        return int(lib().tcs_batch_interior_points(self.h))
+
+    @property
+    def launches_per_sweep(self) -> int:
+        return int(lib().tcs_batch_launches_per_sweep(self.h))
+
+    def run(self, sweeps: int, stream=None) -> None:
+        check(lib().tcs_batch_run(self.h, sweeps, _stream(stream)))
+
+    def graph(self, sweeps: int, stream=None) -> Graph:
+        h = ctypes.c_void_p()
+        check(lib().tcs_batch_graph_create(self.h, sweeps, _stream(stream), ctypes.byref(h)))
+        return Graph(h)
+
+    def destroy(self) -> None:
+        if self.h:
+            check(lib().tcs_batch_destroy(self.h))
+            self.h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.destroy()
+        except Exception:
+            pass
+
+
 # ---------------------------------------------------------------- utilities
 def oracle_sweep_f64(stencil, src, B, dst, stream=None) -> None:
     """One FP64 sweep (gold's order, no FMA) on row-major device grids."""

@@ -1,0 +1,160 @@
+// Batched uneven grids: many independent problems swept by one launch per
+// sweep (per tile shape).  The batch owns the device-resident problem
+// descriptors and tile lists; the caller owns the grids' storage buffers.
+#include <stdlib.h>
+#include <string.h>
+
+#include <new>
+#include <vector>
+
+#include "tcs_internal.hpp"
+
+using namespace tcs;
+
+struct tcs_batch {
+    int precision = 0;
+    int nprob = 0;
+    int64_t interior = 0;
+    BatchProblem *d_probs = nullptr;
+    struct Group {
+        int th, tw, ntiles;
+        BatchTile *d_tiles;
+    };
+    std::vector<Group> groups;
+};
+
+static bool batch_tile_offered(int th, int tw) {
+    return (th == 2 && tw == 4) || (th == 4 && tw == 4) || (th == 4 && tw == 8);
+}
+
+extern "C" int tcs_batch_destroy(tcs_batch *b) {
+    if (!b) return TCS_OK;
+    for (auto &g : b->groups) (void)hipFree(g.d_tiles);
+    (void)hipFree(b->d_probs);
+    delete b;
+    return TCS_OK;
+}
+
+extern "C" int tcs_batch_create(tcs_batch **out, int nprob, const tcs_problem *probs, void *const *d_buf0,
+                                void *const *d_buf1) {
+    if (!out) return set_error(TCS_ERR_INVALID, "batch: null output");
+    *out = nullptr;
+    if (nprob < 1 || !probs || !d_buf0 || !d_buf1) return set_error(TCS_ERR_INVALID, "batch: empty or null input");
+    const int prec = probs[0].precision;
+    if (prec != TCS_PREC_FP16 && prec != TCS_PREC_BF16)
+        return set_error(TCS_ERR_UNSUPPORTED, "batch: FP16 or BF16 storage only");
+    std::vector<BatchProblem> hp(nprob);
+    std::vector<std::vector<BatchTile>> tiles;
+    std::vector<std::pair<int, int>> shapes;
+    int64_t interior = 0;
+    for (int i = 0; i < nprob; ++i) {
+        const tcs_problem &p = probs[i];
+        if (p.method != TCS_METHOD_MFMA || p.precision != prec)
+            return set_error(TCS_ERR_INVALID, "batch: problem %d is not MFMA with the batch's precision", i);
+        if (!batch_tile_offered(p.tile_h, p.tile_w))
+            return set_error(TCS_ERR_UNSUPPORTED, "batch: problem %d tile %dx%d (offered 2x4, 4x4, 4x8)", i,
+                             p.tile_h, p.tile_w);
+        const tcs_layout &L = p.layout;
+        if (L.nbr % p.tile_h || L.nbc % p.tile_w)
+            return set_error(TCS_ERR_INVALID, "batch: problem %d layout is not a multiple of its tile", i);
+        if (!d_buf0[i] || !d_buf1[i] || d_buf0[i] == d_buf1[i])
+            return set_error(TCS_ERR_INVALID, "batch: problem %d needs two distinct buffers", i);
+        BatchProblem &q = hp[i];
+        memset(&q, 0, sizeof q);
+        q.buf[0] = static_cast<uint16_t *>(d_buf0[i]);
+        q.buf[1] = static_cast<uint16_t *>(d_buf1[i]);
+        q.plane = L.plane_elems;
+        q.rows = L.rows;
+        q.cols = L.cols;
+        q.sbr = L.sbr;
+        q.sbc = L.sbc;
+        q.rs = p.radius;
+        q.tiles_x = L.nbc / p.tile_w;
+        q.out_scale = ldexpf(1.0f, -p.coeff_exp);
+        for (int k = 0; k < 25; ++k) {
+            q.cf[k] = p.coeff[k];
+            q.ch[k] = p.coeff_hi[k];
+            q.cl[k] = p.coeff_lo[k];
+        }
+        int gi = -1;
+        for (size_t s = 0; s < shapes.size(); ++s)
+            if (shapes[s].first == p.tile_h && shapes[s].second == p.tile_w) gi = (int)s;
+        if (gi < 0) {
+            shapes.emplace_back(p.tile_h, p.tile_w);
+            tiles.emplace_back();
+            gi = (int)shapes.size() - 1;
+        }
+        const int nt = (L.nbr / p.tile_h) * q.tiles_x;
+        for (int t = 0; t < nt; ++t) tiles[gi].push_back(BatchTile{i, t});
+        const int r = p.radius;
+        if (L.rows > 2 * r && L.cols > 2 * r) interior += (int64_t)(L.rows - 2 * r) * (L.cols - 2 * r);
+    }
+    tcs_batch *b = new (std::nothrow) tcs_batch;
+    if (!b) return set_error(TCS_ERR_ALLOC, "batch: out of host memory");
+    b->precision = prec;
+    b->nprob = nprob;
+    b->interior = interior;
+    hipError_t e = hipMalloc(&b->d_probs, sizeof(BatchProblem) * nprob);
+    if (e == hipSuccess) e = hipMemcpy(b->d_probs, hp.data(), sizeof(BatchProblem) * nprob, hipMemcpyHostToDevice);
+    for (size_t s = 0; e == hipSuccess && s < shapes.size(); ++s) {
+        tcs_batch::Group g{shapes[s].first, shapes[s].second, (int)tiles[s].size(), nullptr};
+        e = hipMalloc(&g.d_tiles, sizeof(BatchTile) * tiles[s].size());
+        if (e == hipSuccess) {
+            b->groups.push_back(g);
+            e = hipMemcpy(g.d_tiles, tiles[s].data(), sizeof(BatchTile) * tiles[s].size(), hipMemcpyHostToDevice);
+        }
+    }
+    if (e != hipSuccess) {
+        tcs_batch_destroy(b);
+        return hip_error(e, "batch: device set-up");
+    }
+    *out = b;
+    return TCS_OK;
+}
+
+extern "C" int tcs_batch_run(tcs_batch *b, int sweeps, void *stream) {
+    if (!b) return set_error(TCS_ERR_INVALID, "batch: null batch");
+    if (sweeps < 0) return set_error(TCS_ERR_INVALID, "batch: negative sweep count");
+    for (int s = 0; s < sweeps; ++s)
+        for (auto &g : b->groups) {
+            int st = launch_batch(b->precision, g.th, g.tw, b->d_probs, g.d_tiles, g.ntiles, s & 1,
+                                  (hipStream_t)stream);
+            if (st) return st;
+        }
+    return TCS_OK;
+}
+
+extern "C" int tcs_batch_graph_create(tcs_batch *b, int sweeps, void *stream, void **graph_exec) {
+    if (!graph_exec) return set_error(TCS_ERR_INVALID, "batch graph: null output");
+    *graph_exec = nullptr;
+    hipStream_t cs = (hipStream_t)stream;
+    bool own = false;
+    if (!cs) {
+        TCS_HIP_CHECK(hipStreamCreateWithFlags(&cs, hipStreamNonBlocking));
+        own = true;
+    }
+    hipError_t e = hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal);
+    if (e != hipSuccess) {
+        if (own) (void)hipStreamDestroy(cs);
+        return hip_error(e, "hipStreamBeginCapture");
+    }
+    const int st = tcs_batch_run(b, sweeps, cs);
+    hipGraph_t g = nullptr;
+    e = hipStreamEndCapture(cs, &g);
+    if (own) (void)hipStreamDestroy(cs);
+    if (st) {
+        if (g) (void)hipGraphDestroy(g);
+        return st;
+    }
+    if (e != hipSuccess) return hip_error(e, "hipStreamEndCapture");
+    hipGraphExec_t ge = nullptr;
+    e = hipGraphInstantiate(&ge, g, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(g);
+    if (e != hipSuccess) return hip_error(e, "hipGraphInstantiate");
+    *graph_exec = ge;
+    return TCS_OK;
+}
+
+extern "C" int64_t tcs_batch_interior_points(const tcs_batch *b) { return b ? b->interior : 0; }
+
+extern "C" int tcs_batch_launches_per_sweep(const tcs_batch *b) { return b ? (int)b->groups.size() : 0; }

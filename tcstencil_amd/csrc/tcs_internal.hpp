@@ -42,7 +42,27 @@ struct SweepArgs {
     uint16_t ch[25], cl[25];
 };
 
+// One problem of a batch, device-resident (tcs_batch.cpp builds them).
+struct BatchProblem {
+    uint16_t *buf[2];      // ping-pong storage buffers
+    int64_t plane;
+    int32_t rows, cols, sbr, sbc;
+    int32_t rs;            // radius
+    int32_t tiles_x;       // tiles per tile row of this problem
+    float out_scale;
+    float cf[25];
+    uint16_t ch[25], cl[25];
+};
+
+// One tile of a batch launch: problem index and tile index within it.
+struct BatchTile {
+    int32_t prob, tile;
+};
+
 // Kernel launchers (tcs_sweep.hip).
+int launch_batch(int precision, int tile_h, int tile_w, const BatchProblem *d_probs, const BatchTile *d_tiles,
+                 int ntiles, int parity, hipStream_t stream);
+const char *batch_kernel_name(int precision, int tile_h, int tile_w);
 int launch_sweep(const tcs_problem *p, const void *d_in, void *d_out, hipStream_t stream);
 int launch_fused(const tcs_problem *p, const void *d_in, void *d_out, int nsweeps, hipStream_t stream);
 const char *sweep_kernel_name(const tcs_problem *p);

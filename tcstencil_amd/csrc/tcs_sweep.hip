@@ -533,6 +533,95 @@ __global__ __launch_bounds__(kWG) void tcs_mfma_fused(SweepArgs a, int nsweeps) 
     }
 }
 
+// Batched uneven grids: one launch sweeps every tile of a list drawn from many
+// problems (each with its own grid, buffers, radius and coefficients).  The
+// tile list is problem-major, and tiles are handed to XCDs in contiguous runs,
+// so neighbouring tiles of one problem share an L2.  The radius selects one of
+// two instantiations of the same brick code with a workgroup-uniform branch.
+template <int TH, int TW, int ZLO, int ZHI, int PREC>
+__device__ __forceinline__ void batch_tile(const SweepArgs &a, uint16_t *lds, int gr0, int gc0) {
+    constexpr int NPL = PREC == TCS_PREC_SPLIT ? 2 : 1;
+    constexpr bool LOOKAHEAD = kLookahead<PREC>;
+    using T = Tile<TH, TW, NPL>;
+    const int tid = threadIdx.x;
+    {
+        TileRegs<T> R;
+        load_tile<T, false>(a, R, gr0, gc0, tid);
+        zero_pad<T>(lds, tid);
+        write_tile<T>(R, lds, tid);
+    }
+    const int lane = tid & 63, wave = tid >> 6;
+    const int m = lane & 15, g = lane >> 4;
+    Coef<ZLO, ZHI, PREC> C;
+    C.build(a, m, g);
+    __syncthreads();
+    const bool edge = !tile_is_interior(a, gr0, gc0, 16 * TH, 16 * TW);
+    constexpr int NB = TH * TW / 4;
+    Ops<T, ZLO, ZHI, PREC> cur, nxt;
+    if constexpr (LOOKAHEAD) {
+        int br, bc;
+        brick_of<TW, 0>(wave, br, bc);
+        cur.load(lds, br, bc, m, g);
+    }
+#pragma unroll
+    for (int q = 0; q < NB; ++q) {
+        int br, bc;
+        brick_of<TW, 0>(wave + 4 * q, br, bc);
+        if constexpr (LOOKAHEAD) {
+            if (q + 1 < NB) {
+                int nbr, nbc;
+                brick_of<TW, 0>(wave + 4 * (q + 1), nbr, nbc);
+                nxt.load(lds, nbr, nbc, m, g);
+            }
+        } else {
+            cur.load(lds, br, bc, m, g);
+        }
+        __builtin_amdgcn_sched_barrier(0);
+        const int gr = gr0 + 16 * br + m, gcb = gc0 + 16 * bc + 4 * g;
+        u32x2 w[NPL];
+        mfma_brick<T, ZLO, ZHI, PREC>(a, lds, C, cur, br, bc, m, g, edge, gr, gcb, w);
+        const int64_t s = stor_index(gr, gcb, a.sbc);
+#pragma unroll
+        for (int p = 0; p < NPL; ++p) store8<true>(a.out + p * a.plane + s, w[p]);
+        if constexpr (LOOKAHEAD) cur = nxt;
+    }
+}
+
+template <int TH, int TW, int PREC>
+__global__ __launch_bounds__(kWG) void tcs_mfma_batch(const BatchProblem *__restrict__ probs,
+                                                       const BatchTile *__restrict__ tiles, int ntiles,
+                                                       int tiles_per_xcd, int parity) {
+    constexpr int NPL = PREC == TCS_PREC_SPLIT ? 2 : 1;
+    __shared__ __attribute__((aligned(16))) uint16_t lds[NPL * Tile<TH, TW, NPL>::PLANE];
+    const int idx = (blockIdx.x % kXCD) * tiles_per_xcd + blockIdx.x / kXCD;
+    if (idx >= ntiles) return;
+    const BatchTile bt = tiles[idx];
+    const BatchProblem &P = probs[bt.prob];
+    SweepArgs a;
+    a.in = P.buf[parity];
+    a.out = P.buf[parity ^ 1];
+    a.plane = P.plane;
+    a.rows = P.rows;
+    a.cols = P.cols;
+    a.sbr = P.sbr;
+    a.sbc = P.sbc;
+    a.tiles_x = P.tiles_x;
+    a.rs = P.rs;
+    a.out_scale = P.out_scale;
+#pragma unroll
+    for (int i = 0; i < 25; ++i) {
+        a.cf[i] = P.cf[i];
+        a.ch[i] = P.ch[i];
+        a.cl[i] = P.cl[i];
+    }
+    int gr0, gc0;
+    tile_origin<TH, TW>(a, bt.tile, gr0, gc0);
+    if (a.rs == 2)
+        batch_tile<TH, TW, 0, 4, PREC>(a, lds, gr0, gc0);
+    else
+        batch_tile<TH, TW, 1, 3, PREC>(a, lds, gr0, gc0);
+}
+
 // ----------------------------------------------------------- direct kernel
 template <int STENCIL>
 __host__ __device__ constexpr bool tap(int dz, int dj) {
@@ -709,6 +798,43 @@ static const FusedEntry *select_fused(const tcs_problem *p) {
     for (const FusedEntry &k : kFused)
         if (k.prec == p->precision && k.radius == p->radius && k.th == p->tile_h && k.tw == p->tile_w) return &k;
     return nullptr;
+}
+
+// Batched kernels: FP16 and BF16, tiles 2x4, 4x4, 4x8.
+typedef void (*batch_fn)(const BatchProblem *, const BatchTile *, int, int, int);
+struct BatchEntry {
+    int prec, th, tw;
+    batch_fn fn;
+    const char *name;
+};
+#define TCS_BATCH(TH, TW, P) {P, TH, TW, tcs_mfma_batch<TH, TW, P>, "tcs_mfma_batch<" #TH ", " #TW ", " #P ">"}
+static const BatchEntry kBatch[] = {
+    TCS_BATCH(2, 4, 0), TCS_BATCH(4, 4, 0), TCS_BATCH(4, 8, 0),
+    TCS_BATCH(2, 4, 1), TCS_BATCH(4, 4, 1), TCS_BATCH(4, 8, 1),
+};
+
+static const BatchEntry *select_batch(int prec, int th, int tw) {
+    for (const BatchEntry &k : kBatch)
+        if (k.prec == prec && k.th == th && k.tw == tw) return &k;
+    return nullptr;
+}
+
+const char *batch_kernel_name(int precision, int tile_h, int tile_w) {
+    const BatchEntry *k = select_batch(precision, tile_h, tile_w);
+    return k ? k->name : nullptr;
+}
+
+int launch_batch(int precision, int tile_h, int tile_w, const BatchProblem *d_probs, const BatchTile *d_tiles,
+                 int ntiles, int parity, hipStream_t stream) {
+    const BatchEntry *k = select_batch(precision, tile_h, tile_w);
+    if (!k) return set_error(TCS_ERR_UNSUPPORTED, "no batched kernel for precision %d tile %dx%d", precision,
+                             tile_h, tile_w);
+    if (ntiles <= 0) return TCS_OK;
+    const int tpx = (ntiles + kXCD - 1) / kXCD;
+    hipLaunchKernelGGL(k->fn, dim3((unsigned)(tpx * kXCD)), dim3(kWG), 0, stream, d_probs, d_tiles, ntiles, tpx,
+                       parity);
+    TCS_HIP_CHECK(hipGetLastError());
+    return TCS_OK;
 }
 
 const char *fused_kernel_name(const tcs_problem *p) {

@@ -139,3 +139,21 @@ def test_gen_coeffs_normalised():
             off[:, r] = 0
             assert not off.any()
     assert not np.array_equal(T.gen_coeffs("box2d2r", 1), T.gen_coeffs("box2d2r", 2))
+
+
+def test_batch_validation_needs_no_gpu():
+    import tcstencil_amd as T
+    L = T.lib()
+    B = np.ones((3, 3), np.float32)
+    ok = T.Problem("box2d1r", 40, 40, B, tile=(2, 4))
+    bad_tile = T.Problem("box2d1r", 40, 40, B, tile=(2, 8))
+    direct = T.Problem("box2d1r", 40, 40, B, method="direct", tile=(2, 4))
+    fake = (ctypes.c_void_p * 1)(0x1000)
+    fake2 = (ctypes.c_void_p * 1)(0x2000)
+    h = ctypes.c_void_p()
+    for p, want in ((bad_tile, T._lib.ERR_UNSUPPORTED), (direct, T._lib.ERR_INVALID)):
+        arr = (T._lib.ProblemStruct * 1)(p.s)
+        assert L.tcs_batch_create(ctypes.byref(h), 1, arr, fake, fake2) == want
+    arr = (T._lib.ProblemStruct * 1)(ok.s)
+    assert L.tcs_batch_create(ctypes.byref(h), 1, arr, fake, fake) == T._lib.ERR_INVALID  # same buffer twice
+    assert L.tcs_batch_run(None, 1, None) == T._lib.ERR_INVALID
