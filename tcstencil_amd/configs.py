@@ -1,0 +1,263 @@
+"""Measure the five benchmark configurations of the stencil path on one GPU.
+
+  python -m tcstencil_amd.configs [c1 c2 c3 c4 c5]
+
+prints one JSON line per measurement.  Throughput is interior updates per
+second, (rows-2r)(cols-2r) * sweeps / t, with t from HIP events around the
+sweep loop (inputs resident, after warm-up).  Accuracy is max-abs and
+relative-L2 against the library's FP64 oracle sweep on the GPU, run on the
+same input values (the FP16/BF16 inputs are FP16/BF16-representable; the SPLIT
+input is the FP32 or FP64 grid).  HBM fractions use the algorithmic bytes per
+update (4 B FP16/BF16, 8 B SPLIT, divided by the sweeps fused per launch)
+against 8 TB/s.
+
+  c1  star2d1r 1024^2, FP16 U[-1,1), one sweep, MFMA and direct
+  c2  box2d1r 10240^2, FP16 U[-1,1), 100 sweeps, MFMA vs direct, errors after 1/10/100
+  c3  star2d2r and box2d2r 16384^2, 4 sinusoids + noise, 10 sweeps, FP16/BF16/SPLIT MFMA
+  c4  512 batched uneven grids (box2d1r/star2d2r, 64..2048 per side), 20 sweeps in a
+      hipGraph, vs one launch per grid
+  c5  box2d2r 32768x16384, 256 sweeps, SPLIT, 2/4/8 sweeps fused vs unfused MFMA and
+      direct; rel-L2 after 256 sweeps
+"""
+from __future__ import annotations
+
+import json
+import math
+import sys
+
+import numpy as np
+
+HBM_PEAK_GBS = 8000.0
+BYTES = {"fp16": 4.0, "bf16": 4.0, "split": 8.0}
+SEED = 1
+
+
+def emit(**kw):
+    print(json.dumps(kw), flush=True)
+
+
+def _torch():
+    import torch
+    return torch
+
+
+def _interior(stencil, rows, cols):
+    from . import api as T
+    r = T.radius(stencil)
+    return (rows - 2 * r) * (cols - 2 * r)
+
+
+def _zero_ring(x, r):
+    for v in (x[:r], x[-r:], x[:, :r], x[:, -r:]):
+        v.zero_()
+
+
+def _fp64_refs(stencil, x64, B, counts):
+    """FP64 oracle results after each sweep count in `counts`."""
+    from . import api as T
+    torch = _torch()
+    a, b = x64.clone(), x64.clone()
+    out, done = {}, 0
+    for k in sorted(counts):
+        while done < k:
+            T.oracle_sweep_f64(stencil, a, B.astype(np.float64), b)
+            a, b = b, a
+            done += 1
+        out[k] = a.clone()
+    del a, b
+    torch.cuda.synchronize()
+    return out
+
+
+def _timed_run(p, b0, b1, sweeps, warmup=2):
+    torch = _torch()
+    p.run(b0, b1, warmup + warmup % 2)
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    res = p.run(b0, b1, sweeps)
+    e1.record()
+    torch.cuda.synchronize()
+    return e0.elapsed_time(e1), res
+
+
+def _errors(p, x, refs, scale_exp=0):
+    """Errors of the problem's result after each count in refs (fresh runs)."""
+    from . import api as T
+    b0, b1 = p.alloc(), p.alloc()
+    out = {}
+    for k in sorted(refs):
+        p.clear(b1)
+        p.pack(x, b0, scale_exp)
+        y = p.unpack(p.run(b0, b1, k), scale_exp=scale_exp)
+        out[k] = T.error_norms(y, refs[k], p.radius)
+    return out
+
+
+def _measure(stencil, rows, cols, B, x, method, precision, sweeps, fuse=1, scale_exp=0, tile=None):
+    from . import api as T
+    p = T.Problem(stencil, rows, cols, B, method=method, precision=precision, fuse=fuse, tile=tile)
+    b0, b1 = p.alloc(), p.alloc()
+    p.clear(b1)
+    p.pack(x, b0, scale_exp)
+    ms, _ = _timed_run(p, b0, b1, sweeps)
+    pts = p.interior_points * sweeps
+    gst = pts / (ms * 1e-3) / 1e9
+    return p, {"stencil": stencil, "grid": [rows, cols], "method": method, "precision": precision,
+               "fuse": fuse, "tile": list(p.tile), "sweeps": sweeps, "ms": round(ms, 3),
+               "gstencil_s": round(gst, 1),
+               "hbm_gbs": round(gst * BYTES[precision] / fuse, 1),
+               "hbm_frac": round(gst * BYTES[precision] / fuse / HBM_PEAK_GBS, 4),
+               "kernel": p.fused_kernel_name if fuse > 1 else p.kernel_name}
+
+
+def c1():
+    from . import api as T
+    torch = _torch()
+    stencil, N = "star2d1r", 1024
+    B = T.gen_coeffs(stencil, SEED)
+    x = T.gen_uniform(N, N, seed=SEED, dtype=torch.float64)
+    T.round_to(x, "fp16")
+    _zero_ring(x, 1)
+    refs = _fp64_refs(stencil, x, B, [1])
+    for method in ("mfma", "direct"):
+        p, rec = _measure(stencil, N, N, B, x, method, "fp16", 200)
+        rec["errors_vs_fp64"] = {1: _errors(p, x, refs)[1]}
+        emit(config="c1", **rec)
+
+
+def c2():
+    from . import api as T
+    torch = _torch()
+    stencil, N = "box2d1r", 10240
+    B = T.gen_coeffs(stencil, SEED)
+    x = T.gen_uniform(N, N, seed=SEED, dtype=torch.float64)
+    T.round_to(x, "fp16")
+    _zero_ring(x, 1)
+    refs = _fp64_refs(stencil, x, B, [1, 10, 100])
+    res = {}
+    for method in ("mfma", "direct"):
+        p, rec = _measure(stencil, N, N, B, x, method, "fp16", 100)
+        rec["errors_vs_fp64"] = _errors(p, x, refs)
+        res[method] = rec["gstencil_s"]
+        emit(config="c2", **rec)
+    p, rec = _measure(stencil, N, N, B, x, "mfma", "fp16", 100, fuse=8)
+    emit(config="c2", **rec)
+    emit(config="c2", mfma_vs_direct=round(res["mfma"] / res["direct"], 3))
+
+
+def c3():
+    from . import api as T
+    torch = _torch()
+    N = 16384
+    for stencil in ("star2d2r", "box2d2r"):
+        B = T.gen_coeffs(stencil, SEED)
+        x64 = T.gen_sinusoid(N, N, seed=SEED, noise=1e-3)
+        _zero_ring(x64, 2)
+        for precision in ("fp16", "bf16", "split"):
+            x = x64.clone()
+            e = 0
+            if precision == "split":
+                e = T.split_scale_exp(T.maxabs(x))
+            else:
+                T.round_to(x, precision)
+            refs = _fp64_refs(stencil, x, B, [1, 10])
+            p, rec = _measure(stencil, N, N, B, x, "mfma", precision, 10, scale_exp=e)
+            rec["errors_vs_fp64"] = _errors(p, x, refs, e)
+            rec["input"] = "4 sinusoids + U[-1e-3,1e-3)" + ("" if precision == "split" else f", rounded to {precision}")
+            emit(config="c3", **rec)
+            del x, refs
+        p, rec = _measure(stencil, N, N, B, x64, "direct", "fp16", 10)
+        emit(config="c3", **rec)
+        del x64
+
+
+def c4(nprob=512, sweeps=20):
+    from . import api as T
+    torch = _torch()
+    rng = np.random.default_rng(SEED)
+    kinds = rng.integers(0, 2, nprob)
+    rows = np.exp(rng.uniform(math.log(64), math.log(2048), nprob)).astype(int)
+    cols = np.exp(rng.uniform(math.log(64), math.log(2048), nprob)).astype(int)
+    probs, b0s, b1s, xs = [], [], [], []
+    for i in range(nprob):
+        stencil = ("box2d1r", "star2d2r")[kinds[i]]
+        B = T.gen_coeffs(stencil, SEED + i)
+        br, bc = (rows[i] + 15) // 16, (cols[i] + 15) // 16
+        # tile per size bucket (measured): small grids keep many workgroups busy
+        tile = (2, 4) if min(br, bc) < 16 else ((4, 4) if min(br, bc) < 64 else (4, 8))
+        p = T.Problem(stencil, int(rows[i]), int(cols[i]), B, tile=tile)
+        x = T.gen_uniform(int(rows[i]), int(cols[i]), seed=SEED + i, dtype=torch.float32)
+        b0, b1 = p.alloc(), p.alloc()
+        p.clear(b1)
+        p.pack(x, b0)
+        probs.append(p), b0s.append(b0), b1s.append(b1), xs.append(x)
+    batch = T.Batch(probs, b0s, b1s)
+    pts = batch.interior_points * sweeps
+    g = batch.graph(sweeps)
+    g.launch()
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    reps = 5
+    e0.record()
+    for _ in range(reps):
+        g.launch()
+    e1.record()
+    torch.cuda.synchronize()
+    ms_batch = e0.elapsed_time(e1) / reps
+    # one launch per grid per sweep, the same work
+    for p, b0, b1 in zip(probs, b0s, b1s):
+        p.run(b0, b1, 2)
+    torch.cuda.synchronize()
+    e0.record()
+    for p, b0, b1 in zip(probs, b0s, b1s):
+        p.run(b0, b1, sweeps)
+    e1.record()
+    torch.cuda.synchronize()
+    ms_each = e0.elapsed_time(e1)
+    gst_b = pts / (ms_batch * 1e-3) / 1e9
+    gst_e = pts / (ms_each * 1e-3) / 1e9
+    emit(config="c4", problems=nprob, sweeps=sweeps, interior_points_per_sweep=batch.interior_points,
+         launches_per_sweep=batch.launches_per_sweep, batched_graph_ms=round(ms_batch, 3),
+         batched_gstencil_s=round(gst_b, 1), batched_hbm_frac=round(gst_b * 4 / HBM_PEAK_GBS, 4),
+         one_launch_per_grid_ms=round(ms_each, 3), one_launch_per_grid_gstencil_s=round(gst_e, 1),
+         speedup=round(ms_each / ms_batch, 2))
+    g.destroy()
+    batch.destroy()
+
+
+def c5(sweeps=256):
+    from . import api as T
+    torch = _torch()
+    stencil, R, C = "box2d2r", 32768, 16384
+    B = T.gen_coeffs(stencil, SEED)
+    x = T.gen_uniform(R, C, seed=SEED, dtype=torch.float64)
+    _zero_ring(x, 2)
+    e = T.split_scale_exp(T.maxabs(x))
+    ref = _fp64_refs(stencil, x, B, [sweeps])[sweeps]
+    recs = []
+    for fuse in (1, 2, 4, 8):
+        p, rec = _measure(stencil, R, C, B, x, "mfma", "split", sweeps, fuse=fuse, scale_exp=e)
+        if fuse == 8:
+            rec["errors_vs_fp64"] = {sweeps: _errors(p, x, {sweeps: ref}, e)[sweeps]}
+        recs.append(rec)
+        emit(config="c5", **rec)
+    x16 = x.clone()
+    T.round_to(x16, "fp16")
+    for method in ("mfma", "direct"):
+        p, rec = _measure(stencil, R, C, B, x16, method, "fp16", 32)
+        emit(config="c5", reference_point=True, **rec)
+
+
+def main(argv=None):
+    names = (argv if argv is not None else sys.argv[1:]) or ["c1", "c2", "c3", "c4", "c5"]
+    torch = _torch()
+    torch.cuda.set_device(0)
+    for n in names:
+        globals()[n]()
+        torch.cuda.empty_cache()
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

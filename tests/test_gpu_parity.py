@@ -483,3 +483,23 @@ def test_batch_bitwise_equals_per_grid(T, torch):
         assert torch.equal(p.unpack(b1), ref)
     g.destroy()
     batch.destroy()
+
+
+def test_c1_star2d1r_1024_vs_gold(T, torch, oracle):
+    """Config C1 as the reference's driver would check it: star2d1r on 1024^2,
+    FP16 input U[-1,1), normalised coefficients, zero ring, one sweep, compared
+    with gold_star2d1r (FP32, CPU) on the same input."""
+    N = 1024
+    x = T.gen_uniform(N, N, seed=1, dtype=torch.float32)
+    T.round_to(x, "fp16")
+    A = x.cpu().numpy()
+    A[0, :] = A[-1, :] = A[:, 0] = A[:, -1] = 0.0
+    B = T.gen_coeffs("star2d1r", 1)
+    gold = oracle.gold("star2d1r", A, B)
+    for method in ("mfma", "direct"):
+        C, _ = T.host_sweep("star2d1r", A, B, method=method)
+        err = np.abs(C.astype(np.float64) - gold)
+        say("case c1", method, err.max())
+        assert np.array_equal(C[0], gold[0]) and np.array_equal(C[:, -1], gold[:, -1])
+        # coefficient rounding (MFMA) + output rounding, u = 2^-11, |x| <= 1
+        assert err.max() <= (2.0 if method == "mfma" else 1.0) * 2.0 ** -11 + 1e-6
