@@ -84,7 +84,9 @@ typedef struct tcs_problem {
                                 of n workgroups per CU that prefetch their next tile */
     int32_t fuse;            /* sweeps fused per launch by tcs_run (temporal blocking,
                                 MFMA only): 1..1+16/radius; 0 or 1 = unfused */
-    int32_t reserved;
+    int32_t frag;            /* MFMA fragment edge: 16 (v_mfma_f32_16x16x32, one brick per
+                                product; the default) or 32 (v_mfma_f32_32x32x16 over
+                                2x2-brick blocks, FP16/BF16); set with tcs_problem_set_frag */
 } tcs_problem;
 
 /* ---------------------------------------------------------- status, params */
@@ -140,6 +142,10 @@ int tcs_radius(int stencil);
  * (tcs_kernel_name / tcs_fused_kernel_name return NULL for it).  Re-derives the
  * layout's alignment, so buffers packed for another tile must be packed again. */
 int tcs_problem_set_tile(tcs_problem *p, int tile_h, int tile_w);
+/* Choose the MFMA fragment (16 or 32, see tcs_problem.frag) and re-derive the
+ * default tile for it (32: 4x4, 2x8, 4x8 or 8x8 single-sweep tiles; 4x4, 4x8, 8x8
+ * fused).  32 needs method MFMA and FP16 or BF16. */
+int tcs_problem_set_frag(tcs_problem *p, int frag);
 /* Bytes of one storage buffer for this problem (2 B/elem; SPLIT: two planes). */
 int64_t tcs_buffer_bytes(const tcs_problem *p);
 
@@ -185,13 +191,19 @@ int tcs_graph_destroy(void *graph_exec);
 
 /* ---------------------------------------------------- batched uneven grids */
 /* Many independent grids (own size, stencil, coefficients and buffers) swept
- * together: one launch per sweep for all problems that share a tile shape, so a
- * sweep of the whole batch is a handful of launches instead of one per grid.
- * All problems: MFMA, the same precision (FP16 or BF16), tiles 2x4, 4x4 or 4x8
- * (per problem).  d_buf0[i] / d_buf1[i] are problem i's two storage buffers
- * (d_buf1 ghost ring zero); sweep s reads buf[s % 2], so after tcs_batch_run the
- * results are in buf[sweeps % 2].  The batch owns device copies of the
- * descriptors (allocated here, freed by tcs_batch_destroy). */
+ * together.  Problems that share a tile shape, fragment (frag), sweeps per launch
+ * (fuse) and wg_per_cu form a group; a group advances all its problems with one
+ * launch per sweep, or per `fuse` sweeps fused in LDS (balanced like tcs_run),
+ * on a grid of one workgroup per tile or, with wg_per_cu > 0, a persistent grid
+ * of wg_per_cu workgroups per CU.  A sweep of the whole batch is a handful of
+ * launches instead of one per grid.  All problems: MFMA, the same precision
+ * (FP16 or BF16); tiles 2x4, 4x4 or 4x8 (16x16 fragments), 4x4 or 4x8 (32x32).
+ * d_buf0[i] / d_buf1[i] are problem i's two storage buffers (d_buf1 ghost ring
+ * zero); after tcs_batch_run the results are in buf[sweeps % 2].  The batch owns
+ * device copies of the descriptors (allocated here, freed by tcs_batch_destroy).
+ * tcs_batch_graph_create captures each group's launches as an independent
+ * chain, so groups run concurrently on replay; tcs_batch_run issues them in
+ * order on one stream.  tcs_batch_launches_per_sweep returns the group count. */
 typedef struct tcs_batch tcs_batch;
 int tcs_batch_create(tcs_batch **b, int nprob, const tcs_problem *probs, void *const *d_buf0,
                      void *const *d_buf1);
@@ -240,8 +252,11 @@ int tcs_gen_coeffs(int stencil, uint64_t seed, float *B, int n);
 const char *tcs_kernel_name(const tcs_problem *p);
 /* Kernel name of the fused (temporally blocked) launch for this problem, or NULL. */
 const char *tcs_fused_kernel_name(const tcs_problem *p);
-/* MFMA instructions issued per 16x16 brick (0 for the direct method). */
+/* MFMA instructions issued per 16x16 brick, or per 2x2-brick block with 32x32
+ * fragments (0 for the direct method). */
 int tcs_mfma_per_brick(const tcs_problem *p);
+/* MFMA flops issued per 16x16 output brick by one sweep (0 for the direct method). */
+int64_t tcs_mfma_flops_per_brick(const tcs_problem *p);
 
 #ifdef __cplusplus
 }

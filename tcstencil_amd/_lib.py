@@ -41,7 +41,7 @@ class ProblemStruct(ctypes.Structure):
                 ("coeff_hi", ctypes.c_uint16 * 25),
                 ("coeff_lo", ctypes.c_uint16 * 25),
                 ("coeff_exp", ctypes.c_int32), ("wg_per_cu", ctypes.c_int32),
-                ("fuse", ctypes.c_int32), ("reserved", ctypes.c_int32)]
+                ("fuse", ctypes.c_int32), ("frag", ctypes.c_int32)]
 
 
 class Params(ctypes.Structure):
@@ -79,6 +79,7 @@ SIGNATURES = {
     "tcs_problem_init": (_i, [_prob, _i, _i, _i, _i, _i, _fp, _i]),
     "tcs_radius": (_i, [_i]),
     "tcs_problem_set_tile": (_i, [_prob, _i, _i]),
+    "tcs_problem_set_frag": (_i, [_prob, _i]),
     "tcs_buffer_bytes": (_i64, [_prob]),
     "tcs_pack_f32": (_i, [_prob, _vp, _i64, _vp, _i, _vp]),
     "tcs_pack_f64": (_i, [_prob, _vp, _i64, _vp, _i, _vp]),
@@ -110,6 +111,7 @@ SIGNATURES = {
     "tcs_kernel_name": (ctypes.c_char_p, [_prob]),
     "tcs_fused_kernel_name": (ctypes.c_char_p, [_prob]),
     "tcs_mfma_per_brick": (_i, [_prob]),
+    "tcs_mfma_flops_per_brick": (_i64, [_prob]),
 }
 
 _lib = None

@@ -50,7 +50,7 @@ class Problem:
     """A tcs_problem descriptor: stencil, method, precision, grid, coefficients."""
 
     def __init__(self, stencil, rows: int, cols: int, B, method="mfma", precision="fp16",
-                 tile=None, wg_per_cu: int = 0, fuse: int = 1):
+                 tile=None, wg_per_cu: int = 0, fuse: int = 1, frag: int = 16):
         self.stencil = stencil_id(stencil)
         self.method = L.METHOD_ID[method] if isinstance(method, str) else int(method)
         self.precision = L.PREC_ID[precision] if isinstance(precision, str) else int(precision)
@@ -59,6 +59,8 @@ class Problem:
         self.s = L.ProblemStruct()
         check(lib().tcs_problem_init(ctypes.byref(self.s), self.stencil, self.method,
                                      self.precision, rows, cols, _fp(self.B), n))
+        if frag != 16:
+            check(lib().tcs_problem_set_frag(ctypes.byref(self.s), frag))
         if tile is not None:
             self.set_tile(*tile)
         self.s.wg_per_cu = wg_per_cu
@@ -119,8 +121,16 @@ class Problem:
         return n.decode() if n else ""
 
     @property
+    def frag(self) -> int:
+        return self.s.frag
+
+    @property
     def mfma_per_brick(self) -> int:
         return int(lib().tcs_mfma_per_brick(self.ref))
+
+    @property
+    def mfma_flops_per_brick(self) -> int:
+        return int(lib().tcs_mfma_flops_per_brick(self.ref))
 
     @property
     def interior_points(self) -> int:

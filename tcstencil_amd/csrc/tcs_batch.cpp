@@ -1,6 +1,8 @@
 // Batched uneven grids: many independent problems swept by one launch per
-// sweep (per tile shape).  The batch owns the device-resident problem
-// descriptors and tile lists; the caller owns the grids' storage buffers.
+// sweep, or per `fuse` sweeps (temporal blocking), for each group of problems
+// that share a tile shape, MFMA fragment, sweeps per launch and grid size.  The
+// batch owns the device-resident problem descriptors and tile lists; the caller
+// owns the grids' storage buffers.
 #include <stdlib.h>
 #include <string.h>
 
@@ -17,13 +19,16 @@ struct tcs_batch {
     int64_t interior = 0;
     BatchProblem *d_probs = nullptr;
     struct Group {
-        int th, tw, ntiles;
+        int th, tw, frag, fuse, wg_per_cu, ntiles;
         BatchTile *d_tiles;
     };
     std::vector<Group> groups;
 };
 
-static bool batch_tile_offered(int th, int tw) {
+static int frag_of(const tcs_problem &p) { return p.frag == 32 ? 32 : 16; }
+
+static bool batch_tile_offered(int th, int tw, int frag) {
+    if (frag == 32) return (th == 4 && tw == 4) || (th == 4 && tw == 8);
     return (th == 2 && tw == 4) || (th == 4 && tw == 4) || (th == 4 && tw == 8);
 }
 
@@ -45,15 +50,28 @@ extern "C" int tcs_batch_create(tcs_batch **out, int nprob, const tcs_problem *p
         return set_error(TCS_ERR_UNSUPPORTED, "batch: FP16 or BF16 storage only");
     std::vector<BatchProblem> hp(nprob);
     std::vector<std::vector<BatchTile>> tiles;
-    std::vector<std::pair<int, int>> shapes;
+    struct Shape {
+        int th, tw, frag, fuse, wg;
+        bool operator==(const Shape &o) const {
+            return th == o.th && tw == o.tw && frag == o.frag && fuse == o.fuse && wg == o.wg;
+        }
+    };
+    std::vector<Shape> shapes;
     int64_t interior = 0;
     for (int i = 0; i < nprob; ++i) {
         const tcs_problem &p = probs[i];
         if (p.method != TCS_METHOD_MFMA || p.precision != prec)
             return set_error(TCS_ERR_INVALID, "batch: problem %d is not MFMA with the batch's precision", i);
-        if (!batch_tile_offered(p.tile_h, p.tile_w))
-            return set_error(TCS_ERR_UNSUPPORTED, "batch: problem %d tile %dx%d (offered 2x4, 4x4, 4x8)", i,
-                             p.tile_h, p.tile_w);
+        const int frag = frag_of(p);
+        if (!batch_tile_offered(p.tile_h, p.tile_w, frag))
+            return set_error(TCS_ERR_UNSUPPORTED,
+                             "batch: problem %d tile %dx%d fragment %d (offered 2x4, 4x4, 4x8; 32x32: 4x4, 4x8)", i,
+                             p.tile_h, p.tile_w, frag);
+        const int fuse = p.fuse > 1 ? p.fuse : 1;
+        if (p.radius * (fuse - 1) > 16)
+            return set_error(TCS_ERR_INVALID, "batch: problem %d fuses %d sweeps, radius %d allows 1..%d", i, fuse,
+                             p.radius, 1 + 16 / p.radius);
+        if (p.wg_per_cu < 0) return set_error(TCS_ERR_INVALID, "batch: problem %d has wg_per_cu < 0", i);
         const tcs_layout &L = p.layout;
         if (L.nbr % p.tile_h || L.nbc % p.tile_w)
             return set_error(TCS_ERR_INVALID, "batch: problem %d layout is not a multiple of its tile", i);
@@ -76,11 +94,12 @@ extern "C" int tcs_batch_create(tcs_batch **out, int nprob, const tcs_problem *p
             q.ch[k] = p.coeff_hi[k];
             q.cl[k] = p.coeff_lo[k];
         }
+        const Shape shape{p.tile_h, p.tile_w, frag, fuse, p.wg_per_cu};
         int gi = -1;
         for (size_t s = 0; s < shapes.size(); ++s)
-            if (shapes[s].first == p.tile_h && shapes[s].second == p.tile_w) gi = (int)s;
+            if (shapes[s] == shape) gi = (int)s;
         if (gi < 0) {
-            shapes.emplace_back(p.tile_h, p.tile_w);
+            shapes.push_back(shape);
             tiles.emplace_back();
             gi = (int)shapes.size() - 1;
         }
@@ -97,7 +116,8 @@ extern "C" int tcs_batch_create(tcs_batch **out, int nprob, const tcs_problem *p
     hipError_t e = hipMalloc(&b->d_probs, sizeof(BatchProblem) * nprob);
     if (e == hipSuccess) e = hipMemcpy(b->d_probs, hp.data(), sizeof(BatchProblem) * nprob, hipMemcpyHostToDevice);
     for (size_t s = 0; e == hipSuccess && s < shapes.size(); ++s) {
-        tcs_batch::Group g{shapes[s].first, shapes[s].second, (int)tiles[s].size(), nullptr};
+        tcs_batch::Group g{shapes[s].th, shapes[s].tw, shapes[s].frag, shapes[s].fuse, shapes[s].wg,
+                           (int)tiles[s].size(), nullptr};
         e = hipMalloc(&g.d_tiles, sizeof(BatchTile) * tiles[s].size());
         if (e == hipSuccess) {
             b->groups.push_back(g);
@@ -112,16 +132,69 @@ extern "C" int tcs_batch_create(tcs_batch **out, int nprob, const tcs_problem *p
     return TCS_OK;
 }
 
+// All `sweeps` sweeps of one group: launches of at most g.fuse sweeps,
+// balanced, with a launch count of the same parity as `sweeps`, so that the
+// result lands in buf[sweeps % 2] of every problem (as tcs_run does).
+static int run_group(const tcs_batch *b, const tcs_batch::Group &g, int sweeps, hipStream_t stream) {
+    if (sweeps <= 0) return TCS_OK;
+    int per_xcd = 0;
+    if (g.wg_per_cu > 0) {
+        int dev = 0, cus = 0;
+        TCS_HIP_CHECK(hipGetDevice(&dev));
+        TCS_HIP_CHECK(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+        per_xcd = cus * g.wg_per_cu / 8;
+        if (per_xcd < 1) per_xcd = 1;
+    }
+    int nl = (sweeps + g.fuse - 1) / g.fuse;
+    if ((nl & 1) != (sweeps & 1)) ++nl;
+    const int base = sweeps / nl, extra = sweeps % nl;
+    for (int i = 0; i < nl; ++i) {
+        const int k = base + (i < extra ? 1 : 0);
+        const int st = launch_batch(b->precision, g.th, g.tw, g.frag, b->d_probs, g.d_tiles, g.ntiles, i & 1, k,
+                                    per_xcd, stream);
+        if (st) return st;
+    }
+    return TCS_OK;
+}
+
 extern "C" int tcs_batch_run(tcs_batch *b, int sweeps, void *stream) {
     if (!b) return set_error(TCS_ERR_INVALID, "batch: null batch");
     if (sweeps < 0) return set_error(TCS_ERR_INVALID, "batch: negative sweep count");
-    for (int s = 0; s < sweeps; ++s)
-        for (auto &g : b->groups) {
-            int st = launch_batch(b->precision, g.th, g.tw, b->d_probs, g.d_tiles, g.ntiles, s & 1,
-                                  (hipStream_t)stream);
-            if (st) return st;
-        }
+    for (auto &g : b->groups) {
+        const int st = run_group(b, g, sweeps, (hipStream_t)stream);
+        if (st) return st;
+    }
     return TCS_OK;
+}
+
+// Captures every group's launch chain on its own forked stream, so the graph
+// holds one independent chain per group and the groups can run concurrently.
+static int capture_groups(tcs_batch *b, int sweeps, hipStream_t cs) {
+    if (b->groups.size() <= 1) return tcs_batch_run(b, sweeps, cs);
+    std::vector<hipStream_t> side(b->groups.size(), nullptr);
+    std::vector<hipEvent_t> done(b->groups.size(), nullptr);
+    hipEvent_t fork = nullptr;
+    int st = TCS_OK;
+    hipError_t e = hipEventCreateWithFlags(&fork, hipEventDisableTiming);
+    if (e == hipSuccess) e = hipEventRecord(fork, cs);
+    for (size_t i = 0; e == hipSuccess && i < side.size(); ++i) {
+        e = hipStreamCreateWithFlags(&side[i], hipStreamNonBlocking);
+        if (e == hipSuccess) e = hipEventCreateWithFlags(&done[i], hipEventDisableTiming);
+        if (e == hipSuccess) e = hipStreamWaitEvent(side[i], fork, 0);
+        if (e == hipSuccess) {
+            st = run_group(b, b->groups[i], sweeps, side[i]);
+            if (st) break;
+        }
+        if (e == hipSuccess) e = hipEventRecord(done[i], side[i]);
+        if (e == hipSuccess) e = hipStreamWaitEvent(cs, done[i], 0);
+    }
+    if (e != hipSuccess && !st) st = hip_error(e, "batch graph: fork/join");
+    for (size_t i = 0; i < side.size(); ++i) {
+        if (done[i]) (void)hipEventDestroy(done[i]);
+        if (side[i]) (void)hipStreamDestroy(side[i]);
+    }
+    if (fork) (void)hipEventDestroy(fork);
+    return st;
 }
 
 extern "C" int tcs_batch_graph_create(tcs_batch *b, int sweeps, void *stream, void **graph_exec) {
@@ -138,7 +211,7 @@ extern "C" int tcs_batch_graph_create(tcs_batch *b, int sweeps, void *stream, vo
         if (own) (void)hipStreamDestroy(cs);
         return hip_error(e, "hipStreamBeginCapture");
     }
-    const int st = tcs_batch_run(b, sweeps, cs);
+    const int st = b ? capture_groups(b, sweeps, cs) : set_error(TCS_ERR_INVALID, "batch: null batch");
     hipGraph_t g = nullptr;
     e = hipStreamEndCapture(cs, &g);
     if (own) (void)hipStreamDestroy(cs);

@@ -138,19 +138,22 @@ static bool tile_offered(int th, int tw) {
            (th == 8 && tw == 8);
 }
 
+static bool frag32(const tcs_problem *p) { return p->frag == 32; }
+
 // Default tile: the fastest shape measured on 16384^2 grids for this method,
 // precision and radius, stepped down until the grid has at least 1024 tiles
 // (4 per CU), so that small grids still fill the chip.
 static void default_tile(const tcs_problem *p, int *th, int *tw) {
     static const int kPref[5][2] = {{8, 8}, {4, 8}, {4, 4}, {2, 8}, {2, 4}};
+    const int last = frag32(p) ? 3 : 4;  // 32x32 fragments need 2x2-brick blocks, 4 per tile
     int first;
     if (p->method == TCS_METHOD_MFMA)
         first = (p->precision != TCS_PREC_SPLIT && p->radius == 2) ? 0 : 1;
     else
         first = p->stencil == TCS_BOX2D2R ? 1 : 4;
     const long br = (p->layout.rows + 15) / 16, bc = (p->layout.cols + 15) / 16;
-    int pick = 4;
-    for (int i = first; i < 5; ++i) {
+    int pick = last;
+    for (int i = first; i <= last; ++i) {
         const long tiles = ((br + kPref[i][0] - 1) / kPref[i][0]) * ((bc + kPref[i][1] - 1) / kPref[i][1]);
         if (tiles >= 1024) {
             pick = i;
@@ -188,6 +191,7 @@ extern "C" int tcs_problem_init(tcs_problem *p, int stencil, int method, int pre
     if (n != 2 * r + 1) return set_error(TCS_ERR_INVALID, "problem: n = %d, stencil needs n = %d", n, 2 * r + 1);
     if (rows < 1 || cols < 1) return set_error(TCS_ERR_INVALID, "problem: grid %dx%d is empty", rows, cols);
     memset(p, 0, sizeof *p);
+    p->frag = 16;
     p->stencil = stencil;
     p->method = method;
     p->precision = precision;
@@ -219,6 +223,17 @@ extern "C" int tcs_problem_init(tcs_problem *p, int stencil, int method, int pre
     return tcs_problem_set_tile(p, th, tw);
 }
 
+extern "C" int tcs_problem_set_frag(tcs_problem *p, int frag) {
+    if (!p) return set_error(TCS_ERR_INVALID, "set_frag: null problem");
+    if (frag != 16 && frag != 32) return set_error(TCS_ERR_INVALID, "set_frag: fragment %d not in {16, 32}", frag);
+    if (frag == 32 && (p->method != TCS_METHOD_MFMA || p->precision == TCS_PREC_SPLIT))
+        return set_error(TCS_ERR_UNSUPPORTED, "set_frag: 32x32 fragments need method MFMA with FP16 or BF16");
+    p->frag = frag;
+    int th, tw;
+    default_tile(p, &th, &tw);
+    return tcs_problem_set_tile(p, th, tw);
+}
+
 extern "C" int64_t tcs_buffer_bytes(const tcs_problem *p) {
     if (!p) return 0;
     return p->layout.plane_elems * 2 * (p->precision == TCS_PREC_SPLIT ? 2 : 1);
@@ -234,6 +249,7 @@ extern "C" int tcs_split_scale_exp(double maxabs) {
 extern "C" const char *tcs_kernel_name(const tcs_problem *p) { return p ? sweep_kernel_name(p) : nullptr; }
 extern "C" const char *tcs_fused_kernel_name(const tcs_problem *p) { return p ? fused_kernel_name(p) : nullptr; }
 extern "C" int tcs_mfma_per_brick(const tcs_problem *p) { return p ? mfma_per_brick(p) : 0; }
+extern "C" int64_t tcs_mfma_flops_per_brick(const tcs_problem *p) { return p ? mfma_flops_per_brick(p) : 0; }
 
 // ---------------------------------------------------------------- sweeping
 extern "C" int tcs_sweep(const tcs_problem *p, const void *d_in, void *d_out, void *stream) {

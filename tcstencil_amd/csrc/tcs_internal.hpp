@@ -11,6 +11,7 @@ namespace tcs {
 typedef _Float16 half8 __attribute__((ext_vector_type(8)));
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef float f32x4 __attribute__((ext_vector_type(4)));
+typedef float f32x16 __attribute__((ext_vector_type(16)));
 typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
 typedef uint32_t u32x2 __attribute__((ext_vector_type(2)));
 
@@ -60,14 +61,18 @@ struct BatchTile {
 };
 
 // Kernel launchers (tcs_sweep.hip).
-int launch_batch(int precision, int tile_h, int tile_w, const BatchProblem *d_probs, const BatchTile *d_tiles,
-                 int ntiles, int parity, hipStream_t stream);
-const char *batch_kernel_name(int precision, int tile_h, int tile_w);
+// Batched launch: nsweeps > 1 fuses that many sweeps per tile; grid_per_xcd > 0
+// caps the workgroups per XCD (persistent grid), else one workgroup per tile.
+int launch_batch(int precision, int tile_h, int tile_w, int frag, const BatchProblem *d_probs,
+                 const BatchTile *d_tiles, int ntiles, int parity, int nsweeps, int grid_per_xcd,
+                 hipStream_t stream);
+const char *batch_kernel_name(int precision, int tile_h, int tile_w, int frag, int nsweeps);
 int launch_sweep(const tcs_problem *p, const void *d_in, void *d_out, hipStream_t stream);
 int launch_fused(const tcs_problem *p, const void *d_in, void *d_out, int nsweeps, hipStream_t stream);
 const char *sweep_kernel_name(const tcs_problem *p);
 const char *fused_kernel_name(const tcs_problem *p);
 int mfma_per_brick(const tcs_problem *p);
+int64_t mfma_flops_per_brick(const tcs_problem *p);
 
 // Error reporting (tcs_host.cpp).
 int set_error(int status, const char *fmt, ...);

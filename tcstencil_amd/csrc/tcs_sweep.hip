@@ -433,21 +433,18 @@ __global__ __launch_bounds__(kWG) void tcs_mfma_sweep(SweepArgs a) {
 // equal to k single sweeps.  Values outside the valid region stay finite
 // (stencils of loaded data), which the MFMA form needs: a non-finite operand
 // anywhere in a brick row's window reaches all 16 outputs of that row.
+template <int TH, int TW, int NPL>
+using FusedGeom = Geom<TH + 2, TW + 2, NPL>;
+
 template <int TH, int TW, int ZLO, int ZHI, int PREC>
-__global__ __launch_bounds__(kWG) void tcs_mfma_fused(SweepArgs a, int nsweeps) {
+__device__ __forceinline__ void fused_tile(const SweepArgs &a, uint16_t *lds, int gr0, int gc0, int nsweeps) {
     constexpr int NPL = PREC == TCS_PREC_SPLIT ? 2 : 1;
     constexpr bool LOOKAHEAD = kLookahead<PREC>;
-    using G = Geom<TH + 2, TW + 2, NPL>;
+    using G = FusedGeom<TH, TW, NPL>;
     constexpr int QA = G::BH * G::BW / 4;  // bricks per wave, whole image
     constexpr int QC = TH * TW / 4;        // bricks per wave, core only
     static_assert(G::BH * G::BW % 4 == 0 && TH * TW % 4 == 0, "bricks must split over 4 waves");
-    __shared__ __attribute__((aligned(16))) uint16_t lds[NPL * G::PLANE];
-
-    Schedule sc = schedule(a);
-    if (sc.tile >= sc.end) return;
     const int tid = threadIdx.x;
-    int gr0, gc0;
-    tile_origin<TH, TW>(a, sc.tile, gr0, gc0);
     const int er0 = gr0 - 16, ec0 = gc0 - 16;  // global point of image brick (0, 0)
     {
         TileRegs<G> R;
@@ -533,6 +530,312 @@ __global__ __launch_bounds__(kWG) void tcs_mfma_fused(SweepArgs a, int nsweeps) 
     }
 }
 
+template <int TH, int TW, int ZLO, int ZHI, int PREC>
+__global__ __launch_bounds__(kWG) void tcs_mfma_fused(SweepArgs a, int nsweeps) {
+    constexpr int NPL = PREC == TCS_PREC_SPLIT ? 2 : 1;
+    __shared__ __attribute__((aligned(16))) uint16_t lds[NPL * FusedGeom<TH, TW, NPL>::PLANE];
+    const Schedule sc = schedule(a);
+    if (sc.tile >= sc.end) return;
+    int gr0, gc0;
+    tile_origin<TH, TW>(a, sc.tile, gr0, gc0);
+    fused_tile<TH, TW, ZLO, ZHI, PREC>(a, lds, gr0, gc0, nsweeps);
+}
+
+// ----------------------------------------------- MFMA kernels, 32x32 fragments
+// A wave computes a block of 2x2 bricks (32x32 points) with
+// v_mfma_f32_32x32x16_{f16,bf16}, the row shift z folded into K.  With X the
+// 36x36 haloed input of the block (X[0][0] = block origin - (2,2)),
+//     out^T[j][i] = sum_(z,k) C[j][(z,k)] * S[(z,k)][i],
+//     C[j][(z,k)] = c[z][k-j] for 0 <= k-j <= 4 (else 0),   S[(z,k)][i] = X[i+z][k].
+// The (z, k) slots are packed into groups of 8: per shift four full groups
+// (k = 0..7, 8..15, 16..23, 24..31; one ds_read_b128 each) and the half group
+// k = 32..35, two shifts sharing one group (two ds_read_b64).  Lane half h
+// of MFMA mf holds group 2*mf + h.  Radius 2 (shifts 0..4): 23 groups, 12 MFMAs
+// per block; radius 1 (shifts 1..3): 14 groups, 7 MFMAs.  Against the 16x16x32
+// form (5 MFMAs and 5 KB of operand reads per brick) this issues 1.2x the MFMA
+// flops and reads 3 KB per brick.  Slots without a real (z, k) re-read a half
+// row of the block's own window against zero coefficients, so every lane reads
+// and every operand is a value of the window.
+// Output columns are assigned to the M index so that lane (i, h) = (l%32, l/32)
+// ends with row i, columns 8h..8h+7 of the block's left brick in accumulator
+// registers 0..7 and of its right brick in 8..15: two 16-byte stores per lane,
+// each store instruction writing two whole bricks.
+template <int PREC>
+struct Frag32;
+template <>
+struct Frag32<TCS_PREC_FP16> {
+    typedef half8 type;
+    static __device__ __forceinline__ f32x16 mma(type x, type y, f32x16 c) {
+        return __builtin_amdgcn_mfma_f32_32x32x16_f16(x, y, c, 0, 0, 0);
+    }
+};
+template <>
+struct Frag32<TCS_PREC_BF16> {
+    typedef bf16x8 type;
+    static __device__ __forceinline__ f32x16 mma(type x, type y, f32x16 c) {
+        return __builtin_amdgcn_mfma_f32_32x32x16_bf16(x, y, c, 0, 0, 0);
+    }
+};
+
+template <int ZLO, int ZHI>
+struct Fold32 {
+    static constexpr int NZ = ZHI - ZLO + 1;
+    static constexpr int NFULL = 4 * NZ;              // full groups
+    static constexpr int NG = NFULL + (NZ + 1) / 2;   // + half-group pairs
+    static constexpr int NMF = (NG + 1) / 2;          // MFMAs per block
+    // Half-group pair of MFMA mf >= 2*NZ for lane half h: shift indices of its
+    // two halves (clamped to a real shift when the half is empty).
+    static __device__ __forceinline__ int pair(int mf, int h) { return 2 * (mf - 2 * NZ) + h; }
+    static __device__ __forceinline__ int half_shift(int p, int s) {
+        const int z = 2 * p + s;
+        return z < NZ ? z : NZ - 1;
+    }
+};
+
+// Output column (within the block) of M index mu.
+__host__ __device__ constexpr int jmap32(int mu) {
+    return 16 * (mu >> 4) + 8 * ((mu >> 2) & 1) + 4 * ((mu >> 3) & 1) + (mu & 3);
+}
+
+// Coefficient fragments: lane (mrow, h) = (l%32, l/32) holds A[mrow][8h + e] of
+// each MFMA, i.e. c[z][k - j] for output column j = jmap32(mrow) and the slot's (z, k).
+template <int ZLO, int ZHI, int PREC>
+struct Coef32 {
+    using F = Fold32<ZLO, ZHI>;
+    typedef typename Frag32<PREC>::type frag_t;
+    frag_t f[F::NMF];
+
+    __device__ __forceinline__ void build(const SweepArgs &a, int mrow, int h) {
+        const int j = jmap32(mrow);
+#pragma unroll
+        for (int mf = 0; mf < F::NMF; ++mf) {
+#pragma unroll
+            for (int e = 0; e < 8; ++e) {
+                int zi, k;
+                bool real = true;
+                if (mf < 2 * F::NZ) {
+                    zi = mf >> 1;
+                    k = 8 * (2 * (mf & 1) + h) + e;
+                } else {
+                    const int z = 2 * F::pair(mf, h) + (e >> 2);
+                    real = z < F::NZ;
+                    zi = real ? z : 0;
+                    k = 32 + (e & 3);
+                }
+                const int d = k - j;
+                float v = 0.0f;
+#pragma unroll
+                for (int dd = 0; dd < 5; ++dd)
+                    if (real && d == dd) v = a.cf[(ZLO + zi) * 5 + dd];
+                if constexpr (PREC == TCS_PREC_BF16)
+                    f[mf][e] = static_cast<__bf16>(v);
+                else
+                    f[mf][e] = static_cast<_Float16>(v);
+            }
+        }
+    }
+};
+
+// Operands of the block whose haloed window starts at LDS row 16*br, column
+// 16*bc, in chunks of CH MFMAs (chunk c: MFMAs c*CH .. c*CH+CH-1), so that one
+// chunk's reads are in flight while the previous chunk's MFMAs run.  Chunk
+// indices are compile-time constants once the block loops are unrolled.
+template <class G, int ZLO, int ZHI>
+struct Ops32 {
+    using F = Fold32<ZLO, ZHI>;
+    static constexpr int CH = 4;
+    static constexpr int NCK = (F::NMF + CH - 1) / CH;
+    u32x4 f[CH];
+
+    __device__ __forceinline__ void load(const uint16_t *lds, int c, int br, int bc, int i, int h) {
+        const uint16_t *base = lds + (16 * br + i + ZLO) * G::LP + 16 * bc;
+#pragma unroll
+        for (int t = 0; t < CH; ++t) {
+            const int mf = c * CH + t;
+            if (mf >= F::NMF) break;
+            if (mf < 2 * F::NZ) {
+                f[t] = *reinterpret_cast<const u32x4 *>(base + (mf >> 1) * G::LP + 8 * (2 * (mf & 1) + h));
+            } else {
+                const int p = F::pair(mf, h);
+                const u32x2 x = *reinterpret_cast<const u32x2 *>(base + F::half_shift(p, 0) * G::LP + 32);
+                const u32x2 y = *reinterpret_cast<const u32x2 *>(base + F::half_shift(p, 1) * G::LP + 32);
+                f[t] = u32x4{x[0], x[1], y[0], y[1]};
+            }
+        }
+    }
+};
+
+template <class G, int ZLO, int ZHI, int PREC>
+__device__ __forceinline__ f32x16 mma_chunk32(const Coef32<ZLO, ZHI, PREC> &K, const Ops32<G, ZLO, ZHI> &op, int c,
+                                              f32x16 acc) {
+    using F32 = Frag32<PREC>;
+    typedef typename F32::type frag_t;
+    constexpr int CH = Ops32<G, ZLO, ZHI>::CH;
+#pragma unroll
+    for (int t = 0; t < CH; ++t) {
+        const int mf = c * CH + t;
+        if (mf >= Fold32<ZLO, ZHI>::NMF) break;
+        acc = F32::mma(K.f[mf], __builtin_bit_cast(frag_t, op.f[t]), acc);
+    }
+    return acc;
+}
+
+// Epilogue of a block: lane (i, h) gets row i, columns 16s + 8h .. +7 (s = 0, 1:
+// left and right brick) in storage format as w[s].  gr: global row of the
+// lane's outputs, gcb: global column of the block's first column; with `edge`,
+// points outside the interior keep their input value (read from the LDS image).
+template <class G, int PREC>
+__device__ __forceinline__ void epilogue32(const SweepArgs &a, const uint16_t *lds, const f32x16 &acc, int br,
+                                           int bc, int i, int h, bool edge, int gr, int gcb, u32x4 (&w)[2]) {
+    uint16_t o[16];
+#pragma unroll
+    for (int v = 0; v < 16; ++v) o[v] = PREC == TCS_PREC_BF16 ? bf16_bits(acc[v]) : f16_bits(acc[v]);
+    if (edge) {
+        const uint16_t *row = lds + (16 * br + i + 2) * G::LP + 16 * bc + 2;
+#pragma unroll
+        for (int v = 0; v < 16; ++v) {
+            const int cb = 16 * (v >> 3) + 8 * h + (v & 7);
+            if (!point_is_interior(a, gr, gcb + cb)) o[v] = row[cb];
+        }
+    }
+#pragma unroll
+    for (int s = 0; s < 2; ++s)
+        w[s] = u32x4{pack2(o[8 * s], o[8 * s + 1]), pack2(o[8 * s + 2], o[8 * s + 3]),
+                     pack2(o[8 * s + 4], o[8 * s + 5]), pack2(o[8 * s + 6], o[8 * s + 7])};
+}
+
+__device__ __forceinline__ void store16_nt(uint16_t *p, u32x4 w) {
+    __builtin_nontemporal_store(w, reinterpret_cast<u32x4 *>(p));
+}
+
+// Runs the MFMAs of a wave's blocks -- block b = wave + 4q (q < NQ, b < NBLK) at
+// LDS brick (BASE + 2*(b / BX), BASE + 2*(b % BX)) -- and body(q, br, bc, acc)
+// on each result.  The operand reads run one chunk ahead of the MFMAs, across
+// block boundaries, through two chunk buffers.
+template <class G, int ZLO, int ZHI, int PREC, int NQ, int NBLK, int BX, int BASE, class Body>
+__device__ __forceinline__ void for_blocks32(const uint16_t *lds, const Coef32<ZLO, ZHI, PREC> &K, int wave,
+                                             int i, int h, Body body) {
+    static_assert(NBLK >= 4 && NQ * 4 >= NBLK && (NQ - 1) * 4 < NBLK, "blocks per wave");
+    constexpr int NCK = Ops32<G, ZLO, ZHI>::NCK;
+    Ops32<G, ZLO, ZHI> buf[2];
+    buf[0].load(lds, 0, BASE + 2 * (wave / BX), BASE + 2 * (wave % BX), i, h);
+    f32x16 acc = {};
+#pragma unroll
+    for (int n = 0; n < NQ * NCK; ++n) {
+        const int q = n / NCK, c = n % NCK;
+        const int b = wave + 4 * q;
+        if (NQ * 4 != NBLK && b >= NBLK) break;  // wave-uniform
+        const int n1 = n + 1, b1 = wave + 4 * (n1 / NCK);
+        if (n1 < NQ * NCK && (NQ * 4 == NBLK || b1 < NBLK))
+            buf[n1 & 1].load(lds, n1 % NCK, BASE + 2 * (b1 / BX), BASE + 2 * (b1 % BX), i, h);
+        __builtin_amdgcn_sched_barrier(0);
+        if (c == 0) acc = f32x16{};
+        acc = mma_chunk32<G, ZLO, ZHI, PREC>(K, buf[n & 1], c, acc);
+        __builtin_amdgcn_sched_barrier(0);
+        if (c == NCK - 1) body(q, BASE + 2 * (b / BX), BASE + 2 * (b % BX), acc);
+    }
+}
+
+template <int TH, int TW, int ZLO, int ZHI, int PREC>
+__device__ __forceinline__ void sweep_tile32(const SweepArgs &a, uint16_t *lds, int gr0, int gc0) {
+    using T = Tile<TH, TW, 1>;
+    constexpr int BX = TW / 2, NBLK = (TH / 2) * BX;
+    static_assert(TH % 2 == 0 && TW % 2 == 0 && NBLK % 4 == 0, "tile must split into 2x2-brick blocks over 4 waves");
+    const int tid = threadIdx.x;
+    {
+        TileRegs<T> R;
+        load_tile<T, false>(a, R, gr0, gc0, tid);
+        write_tile<T>(R, lds, tid);  // columns past 16*TW+4 are never read here
+    }
+    const int lane = tid & 63, wave = tid >> 6;
+    const int i = lane & 31, h = lane >> 5;
+    Coef32<ZLO, ZHI, PREC> C;
+    C.build(a, i, h);
+    __syncthreads();
+    const bool edge = !tile_is_interior(a, gr0, gc0, 16 * TH, 16 * TW);
+    for_blocks32<T, ZLO, ZHI, PREC, NBLK / 4, NBLK, BX, 0>(lds, C, wave, i, h, [&](int, int br, int bc, const f32x16 &acc) {
+        const int gr = gr0 + 16 * br + i, gcb = gc0 + 16 * bc;
+        u32x4 w[2];
+        epilogue32<T, PREC>(a, lds, acc, br, bc, i, h, edge, gr, gcb, w);
+        store16_nt(a.out + stor_index(gr, gcb + 8 * h, a.sbc), w[0]);
+        store16_nt(a.out + stor_index(gr, gcb + 16 + 8 * h, a.sbc), w[1]);
+    });
+}
+
+template <int TH, int TW, int ZLO, int ZHI, int PREC>
+__global__ __launch_bounds__(kWG) void tcs_mfma32_sweep(SweepArgs a) {
+    __shared__ __attribute__((aligned(16))) uint16_t lds[Tile<TH, TW, 1>::PLANE];
+    const Schedule sc = schedule(a);
+    if (sc.tile >= sc.end) return;
+    int gr0, gc0;
+    tile_origin<TH, TW>(a, sc.tile, gr0, gc0);
+    sweep_tile32<TH, TW, ZLO, ZHI, PREC>(a, lds, gr0, gc0);
+}
+
+// Temporal blocking with 32x32 fragments: the scheme of tcs_mfma_fused (one-
+// brick ring recomputed by every sweep but the last, per-sweep rounding, bitwise
+// equal to single sweeps) over 2x2-brick blocks.  The (TH+2) x (TW+2) image
+// splits into blocks from brick (0, 0); the core of the last sweep into blocks
+// from brick (1, 1).
+template <int TH, int TW, int ZLO, int ZHI, int PREC>
+__device__ __forceinline__ void fused_tile32(const SweepArgs &a, uint16_t *lds, int gr0, int gc0, int nsweeps) {
+    using G = FusedGeom<TH, TW, 1>;
+    constexpr int BXA = G::BW / 2, NBA = (G::BH / 2) * BXA, QA = (NBA + 3) / 4;
+    constexpr int BXC = TW / 2, NBC = (TH / 2) * BXC;
+    static_assert(TH % 2 == 0 && TW % 2 == 0 && NBC % 4 == 0, "tile must split into 2x2-brick blocks over 4 waves");
+    const int tid = threadIdx.x;
+    const int er0 = gr0 - 16, ec0 = gc0 - 16;  // global point of image brick (0, 0)
+    {
+        TileRegs<G> R;
+        load_tile<G, true>(a, R, er0, ec0, tid);
+        write_tile<G>(R, lds, tid);
+    }
+    const int lane = tid & 63, wave = tid >> 6;
+    const int i = lane & 31, h = lane >> 5;
+    Coef32<ZLO, ZHI, PREC> C;
+    C.build(a, i, h);
+    const bool edge = !tile_is_interior(a, er0, ec0, 16 * G::BH, 16 * G::BW);
+    __syncthreads();
+
+    uint32_t *l32 = reinterpret_cast<uint32_t *>(lds);
+    for (int sw = 0; sw + 1 < nsweeps; ++sw) {
+        u32x4 res[QA][2];
+        for_blocks32<G, ZLO, ZHI, PREC, QA, NBA, BXA, 0>(lds, C, wave, i, h, [&](int q, int br, int bc, const f32x16 &acc) {
+            epilogue32<G, PREC>(a, lds, acc, br, bc, i, h, edge, er0 + 16 * br + i, ec0 + 16 * bc, res[q]);
+        });
+        __syncthreads();  // every wave has read this sweep's input
+#pragma unroll
+        for (int qq = 0; qq < QA; ++qq) {
+            const int b = wave + 4 * qq;
+            if (QA * 4 != NBA && b >= NBA) break;
+            const int br = 2 * (b / BXA), bc = 2 * (b % BXA);
+            const int d = ((16 * br + i + 2) * G::LP + 16 * bc + 8 * h + 2) >> 1;
+#pragma unroll
+            for (int s = 0; s < 2; ++s)
+#pragma unroll
+                for (int t = 0; t < 4; ++t) l32[d + 8 * s + t] = res[qq][s][t];
+        }
+        __syncthreads();
+    }
+    for_blocks32<G, ZLO, ZHI, PREC, NBC / 4, NBC, BXC, 1>(lds, C, wave, i, h, [&](int, int br, int bc, const f32x16 &acc) {
+        const int gr = er0 + 16 * br + i, gcb = ec0 + 16 * bc;
+        u32x4 w[2];
+        epilogue32<G, PREC>(a, lds, acc, br, bc, i, h, edge, gr, gcb, w);
+        store16_nt(a.out + stor_index(gr, gcb + 8 * h, a.sbc), w[0]);
+        store16_nt(a.out + stor_index(gr, gcb + 16 + 8 * h, a.sbc), w[1]);
+    });
+}
+
+template <int TH, int TW, int ZLO, int ZHI, int PREC>
+__global__ __launch_bounds__(kWG) void tcs_mfma32_fused(SweepArgs a, int nsweeps) {
+    __shared__ __attribute__((aligned(16))) uint16_t lds[FusedGeom<TH, TW, 1>::PLANE];
+    const Schedule sc = schedule(a);
+    if (sc.tile >= sc.end) return;
+    int gr0, gc0;
+    tile_origin<TH, TW>(a, sc.tile, gr0, gc0);
+    fused_tile32<TH, TW, ZLO, ZHI, PREC>(a, lds, gr0, gc0, nsweeps);
+}
+
 // Batched uneven grids: one launch sweeps every tile of a list drawn from many
 // problems (each with its own grid, buffers, radius and coefficients).  The
 // tile list is problem-major, and tiles are handed to XCDs in contiguous runs,
@@ -587,39 +890,66 @@ __device__ __forceinline__ void batch_tile(const SweepArgs &a, uint16_t *lds, in
     }
 }
 
-template <int TH, int TW, int PREC>
+// One launch covers `ntiles` tiles of the list, FUSED: nsweeps sweeps per tile
+// in LDS (temporal blocking), else one sweep.  Workgroup w on XCD x = w % 8
+// walks the XCD's contiguous run of tiles with stride gridDim.x / 8 (a
+// persistent grid when the launch has fewer workgroups than tiles).  Every
+// tile reads buf[parity] and writes buf[parity ^ 1] of its problem.
+template <int TH, int TW, int PREC, int FRAG, bool FUSED>
 __global__ __launch_bounds__(kWG) void tcs_mfma_batch(const BatchProblem *__restrict__ probs,
                                                        const BatchTile *__restrict__ tiles, int ntiles,
-                                                       int tiles_per_xcd, int parity) {
+                                                       int tiles_per_xcd, int parity, int nsweeps) {
     constexpr int NPL = PREC == TCS_PREC_SPLIT ? 2 : 1;
-    __shared__ __attribute__((aligned(16))) uint16_t lds[NPL * Tile<TH, TW, NPL>::PLANE];
-    const int idx = (blockIdx.x % kXCD) * tiles_per_xcd + blockIdx.x / kXCD;
-    if (idx >= ntiles) return;
-    const BatchTile bt = tiles[idx];
-    const BatchProblem &P = probs[bt.prob];
-    SweepArgs a;
-    a.in = P.buf[parity];
-    a.out = P.buf[parity ^ 1];
-    a.plane = P.plane;
-    a.rows = P.rows;
-    a.cols = P.cols;
-    a.sbr = P.sbr;
-    a.sbc = P.sbc;
-    a.tiles_x = P.tiles_x;
-    a.rs = P.rs;
-    a.out_scale = P.out_scale;
+    constexpr int PLANE = FUSED ? FusedGeom<TH, TW, NPL>::PLANE : Tile<TH, TW, NPL>::PLANE;
+    __shared__ __attribute__((aligned(16))) uint16_t lds[NPL * PLANE];
+    const int x = blockIdx.x % kXCD, step = gridDim.x / kXCD;
+    const int begin = x * tiles_per_xcd;
+    const int end = begin + tiles_per_xcd < ntiles ? begin + tiles_per_xcd : ntiles;
+    for (int idx = begin + (int)blockIdx.x / kXCD; idx < end; idx += step) {
+        if (idx != begin + (int)blockIdx.x / kXCD) __syncthreads();  // the previous tile is done with LDS
+        const BatchTile bt = tiles[idx];
+        const BatchProblem &P = probs[bt.prob];
+        SweepArgs a;
+        a.in = P.buf[parity];
+        a.out = P.buf[parity ^ 1];
+        a.plane = P.plane;
+        a.rows = P.rows;
+        a.cols = P.cols;
+        a.sbr = P.sbr;
+        a.sbc = P.sbc;
+        a.tiles_x = P.tiles_x;
+        a.rs = P.rs;
+        a.out_scale = P.out_scale;
 #pragma unroll
-    for (int i = 0; i < 25; ++i) {
-        a.cf[i] = P.cf[i];
-        a.ch[i] = P.ch[i];
-        a.cl[i] = P.cl[i];
+        for (int i = 0; i < 25; ++i) {
+            a.cf[i] = P.cf[i];
+            a.ch[i] = P.ch[i];
+            a.cl[i] = P.cl[i];
+        }
+        int gr0, gc0;
+        tile_origin<TH, TW>(a, bt.tile, gr0, gc0);
+        if constexpr (FUSED && FRAG == 32) {
+            if (a.rs == 2)
+                fused_tile32<TH, TW, 0, 4, PREC>(a, lds, gr0, gc0, nsweeps);
+            else
+                fused_tile32<TH, TW, 1, 3, PREC>(a, lds, gr0, gc0, nsweeps);
+        } else if constexpr (FUSED) {
+            if (a.rs == 2)
+                fused_tile<TH, TW, 0, 4, PREC>(a, lds, gr0, gc0, nsweeps);
+            else
+                fused_tile<TH, TW, 1, 3, PREC>(a, lds, gr0, gc0, nsweeps);
+        } else if constexpr (FRAG == 32) {
+            if (a.rs == 2)
+                sweep_tile32<TH, TW, 0, 4, PREC>(a, lds, gr0, gc0);
+            else
+                sweep_tile32<TH, TW, 1, 3, PREC>(a, lds, gr0, gc0);
+        } else {
+            if (a.rs == 2)
+                batch_tile<TH, TW, 0, 4, PREC>(a, lds, gr0, gc0);
+            else
+                batch_tile<TH, TW, 1, 3, PREC>(a, lds, gr0, gc0);
+        }
     }
-    int gr0, gc0;
-    tile_origin<TH, TW>(a, bt.tile, gr0, gc0);
-    if (a.rs == 2)
-        batch_tile<TH, TW, 0, 4, PREC>(a, lds, gr0, gc0);
-    else
-        batch_tile<TH, TW, 1, 3, PREC>(a, lds, gr0, gc0);
 }
 
 // ----------------------------------------------------------- direct kernel
@@ -733,12 +1063,13 @@ typedef void (*sweep_fn)(SweepArgs);
 struct KernelEntry {
     int method, prec, radius, stencil, th, tw;  // stencil = -1: any stencil of that radius
     bool persist;
+    int frag;  // MFMA fragment edge (16 or 32); 0 for the direct kernel
     sweep_fn fn;
     const char *name;  // as rocprofv3 reports it (demangled, without the namespace)
 };
 
 #define TCS_MFMA(TH, TW, ZLO, ZHI, P, R, PS) \
-    {TCS_METHOD_MFMA, P, R, -1, TH, TW, PS, tcs_mfma_sweep<TH, TW, ZLO, ZHI, P, PS>, \
+    {TCS_METHOD_MFMA, P, R, -1, TH, TW, PS, 16, tcs_mfma_sweep<TH, TW, ZLO, ZHI, P, PS>, \
      "tcs_mfma_sweep<" #TH ", " #TW ", " #ZLO ", " #ZHI ", " #P ", " #PS ">"}
 #define TCS_MFMA_TILES(ZLO, ZHI, P, R, PS) \
     TCS_MFMA(2, 4, ZLO, ZHI, P, R, PS), TCS_MFMA(4, 4, ZLO, ZHI, P, R, PS), TCS_MFMA(2, 8, ZLO, ZHI, P, R, PS), \
@@ -749,90 +1080,121 @@ struct KernelEntry {
     TCS_MFMA_TILES(1, 3, 1, 1, PS), TCS_MFMA(8, 8, 1, 3, 1, 1, PS), TCS_MFMA_TILES(0, 4, 2, 2, PS), \
     TCS_MFMA_TILES(1, 3, 2, 1, PS)
 #define TCS_DIRECT(TH, TW, S, PS) \
-    {TCS_METHOD_DIRECT, TCS_PREC_FP16, -1, S, TH, TW, PS, tcs_direct_sweep<TH, TW, S, PS>, \
+    {TCS_METHOD_DIRECT, TCS_PREC_FP16, -1, S, TH, TW, PS, 0, tcs_direct_sweep<TH, TW, S, PS>, \
      "tcs_direct_sweep<" #TH ", " #TW ", " #S ", " #PS ">"}
 #define TCS_DIRECT_TILES(S, PS) \
     TCS_DIRECT(2, 4, S, PS), TCS_DIRECT(4, 4, S, PS), TCS_DIRECT(2, 8, S, PS), TCS_DIRECT(4, 8, S, PS), \
     TCS_DIRECT(8, 8, S, PS)
 
+#define TCS_MFMA32(TH, TW, ZLO, ZHI, P, R) \
+    {TCS_METHOD_MFMA, P, R, -1, TH, TW, false, 32, tcs_mfma32_sweep<TH, TW, ZLO, ZHI, P>, \
+     "tcs_mfma32_sweep<" #TH ", " #TW ", " #ZLO ", " #ZHI ", " #P ">"}
+#define TCS_MFMA32_TILES(ZLO, ZHI, P, R) \
+    TCS_MFMA32(4, 4, ZLO, ZHI, P, R), TCS_MFMA32(2, 8, ZLO, ZHI, P, R), TCS_MFMA32(4, 8, ZLO, ZHI, P, R), \
+    TCS_MFMA32(8, 8, ZLO, ZHI, P, R)
+
 // Tile shapes offered (bricks, TH x TW), each one-tile-per-workgroup (false)
 // and persistent (true).  SPLIT has no 8x8 tile: two planes of 132 x 136
-// elements would need 71.8 KB of LDS.
+// elements would need 71.8 KB of LDS.  32x32 fragments: FP16 and BF16, tiles
+// whose 2x2-brick blocks split evenly over the 4 waves, one tile per workgroup.
 static const KernelEntry kKernels[] = {
     TCS_MFMA_ALL(false), TCS_MFMA_ALL(true),
+    TCS_MFMA32_TILES(0, 4, 0, 2), TCS_MFMA32_TILES(1, 3, 0, 1), TCS_MFMA32_TILES(0, 4, 1, 2),
+    TCS_MFMA32_TILES(1, 3, 1, 1),
     TCS_DIRECT_TILES(0, false), TCS_DIRECT_TILES(1, false), TCS_DIRECT_TILES(2, false), TCS_DIRECT_TILES(3, false),
     TCS_DIRECT_TILES(0, true), TCS_DIRECT_TILES(1, true), TCS_DIRECT_TILES(2, true), TCS_DIRECT_TILES(3, true),
 };
 static_assert(TCS_PREC_FP16 == 0 && TCS_PREC_BF16 == 1 && TCS_PREC_SPLIT == 2, "precision ids");
 static_assert(TCS_STAR2D1R == 0 && TCS_STAR2D2R == 1 && TCS_BOX2D1R == 2 && TCS_BOX2D2R == 3, "stencil ids");
 
+static int frag_of(const tcs_problem *p) { return p->frag == 32 ? 32 : 16; }
+
 static const KernelEntry *select_kernel(const tcs_problem *p) {
     for (const KernelEntry &k : kKernels) {
         if (k.method != p->method || k.prec != p->precision || k.th != p->tile_h || k.tw != p->tile_w ||
             k.persist != (p->wg_per_cu > 0))
             continue;
+        if (k.method == TCS_METHOD_MFMA && k.frag != frag_of(p)) continue;
         if (k.method == TCS_METHOD_MFMA ? k.radius == p->radius : k.stencil == p->stencil) return &k;
     }
     return nullptr;
 }
 
-// Fused (temporally blocked) MFMA kernels: tiles 2x4, 4x4, 4x8, 8x8.
+// Fused (temporally blocked) MFMA kernels: tiles 2x4, 4x4, 4x8, 8x8 (16x16
+// fragments), 4x4, 4x8, 8x8 (32x32 fragments, FP16 and BF16).
 typedef void (*fused_fn)(SweepArgs, int);
 struct FusedEntry {
-    int prec, radius, th, tw;
+    int prec, radius, th, tw, frag;
     fused_fn fn;
     const char *name;
 };
 #define TCS_FUSED(TH, TW, ZLO, ZHI, P, R) \
-    {P, R, TH, TW, tcs_mfma_fused<TH, TW, ZLO, ZHI, P>, "tcs_mfma_fused<" #TH ", " #TW ", " #ZLO ", " #ZHI ", " #P ">"}
+    {P, R, TH, TW, 16, tcs_mfma_fused<TH, TW, ZLO, ZHI, P>, "tcs_mfma_fused<" #TH ", " #TW ", " #ZLO ", " #ZHI ", " #P ">"}
+#define TCS_FUSED32(TH, TW, ZLO, ZHI, P, R) \
+    {P, R, TH, TW, 32, tcs_mfma32_fused<TH, TW, ZLO, ZHI, P>, \
+     "tcs_mfma32_fused<" #TH ", " #TW ", " #ZLO ", " #ZHI ", " #P ">"}
+#define TCS_FUSED32_TILES(ZLO, ZHI, P, R) \
+    TCS_FUSED32(4, 4, ZLO, ZHI, P, R), TCS_FUSED32(4, 8, ZLO, ZHI, P, R), TCS_FUSED32(8, 8, ZLO, ZHI, P, R)
 #define TCS_FUSED_TILES(ZLO, ZHI, P, R)                                                           \
     TCS_FUSED(2, 4, ZLO, ZHI, P, R), TCS_FUSED(4, 4, ZLO, ZHI, P, R), TCS_FUSED(4, 8, ZLO, ZHI, P, R), \
         TCS_FUSED(8, 8, ZLO, ZHI, P, R)
 static const FusedEntry kFused[] = {
     TCS_FUSED_TILES(0, 4, 0, 2), TCS_FUSED_TILES(1, 3, 0, 1), TCS_FUSED_TILES(0, 4, 1, 2),
     TCS_FUSED_TILES(1, 3, 1, 1), TCS_FUSED_TILES(0, 4, 2, 2), TCS_FUSED_TILES(1, 3, 2, 1),
+    TCS_FUSED32_TILES(0, 4, 0, 2), TCS_FUSED32_TILES(1, 3, 0, 1), TCS_FUSED32_TILES(0, 4, 1, 2),
+    TCS_FUSED32_TILES(1, 3, 1, 1),
 };
 
 static const FusedEntry *select_fused(const tcs_problem *p) {
     if (p->method != TCS_METHOD_MFMA) return nullptr;
     for (const FusedEntry &k : kFused)
-        if (k.prec == p->precision && k.radius == p->radius && k.th == p->tile_h && k.tw == p->tile_w) return &k;
+        if (k.prec == p->precision && k.radius == p->radius && k.th == p->tile_h && k.tw == p->tile_w &&
+            k.frag == frag_of(p))
+            return &k;
     return nullptr;
 }
 
-// Batched kernels: FP16 and BF16, tiles 2x4, 4x4, 4x8.
-typedef void (*batch_fn)(const BatchProblem *, const BatchTile *, int, int, int);
+// Batched kernels: FP16 and BF16; tiles 2x4, 4x4, 4x8 (16x16 fragments) and
+// 4x4, 4x8 (32x32 fragments); one sweep or several fused per launch.
+typedef void (*batch_fn)(const BatchProblem *, const BatchTile *, int, int, int, int);
 struct BatchEntry {
-    int prec, th, tw;
+    int prec, th, tw, frag;
+    bool fused;
     batch_fn fn;
     const char *name;
 };
-#define TCS_BATCH(TH, TW, P) {P, TH, TW, tcs_mfma_batch<TH, TW, P>, "tcs_mfma_batch<" #TH ", " #TW ", " #P ">"}
+#define TCS_BATCH(TH, TW, P, F, FU)                                            \
+    {P, TH, TW, F, FU, tcs_mfma_batch<TH, TW, P, F, FU>,                     \
+     "tcs_mfma_batch<" #TH ", " #TW ", " #P ", " #F ", " #FU ">"}
+#define TCS_BATCH_SET(P, FU)                                                                          \
+    TCS_BATCH(2, 4, P, 16, FU), TCS_BATCH(4, 4, P, 16, FU), TCS_BATCH(4, 8, P, 16, FU),                 \
+        TCS_BATCH(4, 4, P, 32, FU), TCS_BATCH(4, 8, P, 32, FU)
 static const BatchEntry kBatch[] = {
-    TCS_BATCH(2, 4, 0), TCS_BATCH(4, 4, 0), TCS_BATCH(4, 8, 0),
-    TCS_BATCH(2, 4, 1), TCS_BATCH(4, 4, 1), TCS_BATCH(4, 8, 1),
+    TCS_BATCH_SET(0, false), TCS_BATCH_SET(1, false), TCS_BATCH_SET(0, true), TCS_BATCH_SET(1, true),
 };
 
-static const BatchEntry *select_batch(int prec, int th, int tw) {
+static const BatchEntry *select_batch(int prec, int th, int tw, int frag, bool fused) {
     for (const BatchEntry &k : kBatch)
-        if (k.prec == prec && k.th == th && k.tw == tw) return &k;
+        if (k.prec == prec && k.th == th && k.tw == tw && k.frag == frag && k.fused == fused) return &k;
     return nullptr;
 }
 
-const char *batch_kernel_name(int precision, int tile_h, int tile_w) {
-    const BatchEntry *k = select_batch(precision, tile_h, tile_w);
+const char *batch_kernel_name(int precision, int tile_h, int tile_w, int frag, int nsweeps) {
+    const BatchEntry *k = select_batch(precision, tile_h, tile_w, frag, nsweeps > 1);
     return k ? k->name : nullptr;
 }
 
-int launch_batch(int precision, int tile_h, int tile_w, const BatchProblem *d_probs, const BatchTile *d_tiles,
-                 int ntiles, int parity, hipStream_t stream) {
-    const BatchEntry *k = select_batch(precision, tile_h, tile_w);
-    if (!k) return set_error(TCS_ERR_UNSUPPORTED, "no batched kernel for precision %d tile %dx%d", precision,
-                             tile_h, tile_w);
+int launch_batch(int precision, int tile_h, int tile_w, int frag, const BatchProblem *d_probs,
+                 const BatchTile *d_tiles, int ntiles, int parity, int nsweeps, int grid_per_xcd,
+                 hipStream_t stream) {
+    const BatchEntry *k = select_batch(precision, tile_h, tile_w, frag, nsweeps > 1);
+    if (!k) return set_error(TCS_ERR_UNSUPPORTED, "no batched kernel for precision %d tile %dx%d fragment %d",
+                             precision, tile_h, tile_w, frag);
     if (ntiles <= 0) return TCS_OK;
     const int tpx = (ntiles + kXCD - 1) / kXCD;
-    hipLaunchKernelGGL(k->fn, dim3((unsigned)(tpx * kXCD)), dim3(kWG), 0, stream, d_probs, d_tiles, ntiles, tpx,
-                       parity);
+    const int per_xcd = grid_per_xcd > 0 && grid_per_xcd < tpx ? grid_per_xcd : tpx;
+    hipLaunchKernelGGL(k->fn, dim3((unsigned)(per_xcd * kXCD)), dim3(kWG), 0, stream, d_probs, d_tiles, ntiles, tpx,
+                       parity, nsweeps);
     TCS_HIP_CHECK(hipGetLastError());
     return TCS_OK;
 }
@@ -850,7 +1212,14 @@ const char *sweep_kernel_name(const tcs_problem *p) {
 int mfma_per_brick(const tcs_problem *p) {
     if (p->method != TCS_METHOD_MFMA) return 0;
     const int nz = 2 * p->radius + 1;
+    if (frag_of(p) == 32) return p->radius == 2 ? Fold32<0, 4>::NMF : Fold32<1, 3>::NMF;  // per 2x2-brick block
     return p->precision == TCS_PREC_SPLIT ? 4 * nz : nz;
+}
+
+int64_t mfma_flops_per_brick(const tcs_problem *p) {
+    if (p->method != TCS_METHOD_MFMA) return 0;
+    if (frag_of(p) == 32) return (int64_t)mfma_per_brick(p) * (32 * 32 * 16 * 2) / 4;
+    return (int64_t)mfma_per_brick(p) * (16 * 16 * 32 * 2);
 }
 
 static int make_args(const tcs_problem *p, const void *d_in, void *d_out, SweepArgs &a) {

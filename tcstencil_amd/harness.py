@@ -38,7 +38,7 @@ TILES = [(2, 4), (4, 4), (2, 8), (4, 8), (8, 8)]
 
 
 def run_workload(stencil, rows, cols, method="mfma", precision="fp16", tile=None, sweeps=20,
-                 warmup=5, copy_mb=0, seed=1, fuse=1, wg_per_cu=0):
+                 warmup=5, copy_mb=0, seed=1, fuse=1, wg_per_cu=0, frag=16):
     import numpy as np
     import torch
 
@@ -46,7 +46,7 @@ def run_workload(stencil, rows, cols, method="mfma", precision="fp16", tile=None
 
     B = T.gen_coeffs(stencil, seed)
     p = T.Problem(stencil, rows, cols, B, method=method, precision=precision, tile=tile, fuse=fuse,
-                  wg_per_cu=wg_per_cu)
+                  wg_per_cu=wg_per_cu, frag=frag)
     x = T.gen_uniform(rows, cols, seed=seed, dtype=torch.float32)
     e = T.split_scale_exp(1.0) if precision == "split" else 0
     b0, b1 = p.alloc(), p.alloc()
@@ -63,7 +63,7 @@ def run_workload(stencil, rows, cols, method="mfma", precision="fp16", tile=None
     ms = ev[0].elapsed_time(ev[1]) / sweeps
     pts = p.interior_points
     out = {"stencil": stencil, "rows": rows, "cols": cols, "method": method, "precision": precision,
-           "tile": list(p.tile), "fuse": fuse,
+           "tile": list(p.tile), "fuse": fuse, "frag": p.frag,
            "kernel": p.fused_kernel_name if fuse > 1 else p.kernel_name, "sweeps": sweeps, "ms_per_sweep": ms,
            "gstencil_s": pts / (ms * 1e-3) / 1e9,
            "hbm_gbs": BYTES_PER_UPDATE[precision] / max(fuse, 1) * pts / (ms * 1e-3) / 1e9}
@@ -122,6 +122,7 @@ def main(argv=None):
     r.add_argument("--copy-mb", type=float, default=0)
     r.add_argument("--fuse", type=int, default=1)
     r.add_argument("--wg-per-cu", type=int, default=0)
+    r.add_argument("--frag", type=int, default=16, choices=(16, 32))
     s = sub.add_parser("sweep")
     s.add_argument("--stencils", nargs="+", default=["star2d1r", "star2d2r", "box2d1r", "box2d2r"])
     s.add_argument("--methods", nargs="+", default=["mfma", "direct"])
@@ -136,7 +137,7 @@ def main(argv=None):
     if a.cmd == "run":
         res = run_workload(a.stencil, a.size, a.cols or a.size, a.method, a.precision,
                            tuple(a.tile) if a.tile else None, a.sweeps, a.warmup, a.copy_mb,
-                           fuse=a.fuse, wg_per_cu=a.wg_per_cu)
+                           fuse=a.fuse, wg_per_cu=a.wg_per_cu, frag=a.frag)
         print(json.dumps(res), flush=True)
         return 0
 

@@ -12,6 +12,10 @@ MFMA semantics assumed (v_mfma_f32_16x16x32_{f16,bf16}, wave64):
   A (16x32): lane l, element e holds A[l % 16][8*(l//16) + e]
   B (32x16): lane l, element e holds B[8*(l//16) + e][l % 16]
   D (16x16): lane l, element v holds D[4*(l//16) + v][l % 16]
+and (v_mfma_f32_32x32x16_{f16,bf16}, wave64):
+  A (32x16): lane l, element e holds A[l % 32][8*(l//32) + e]
+  B (16x32): lane l, element e holds B[8*(l//32) + e][l % 32]
+  D (32x32): lane l, element v holds D[8*(v//4) + 4*(l//32) + v%4][l % 32]
 The kernels depend only on the row/column placement and on A and B sharing
 one K order; the GPU test tests/test_gpu_parity.py checks the real hardware.
 
@@ -109,9 +113,11 @@ class TileGeom:
         self.PLANE = self.LR * self.LP
 
 
-def stage_tile(inp: np.ndarray, L: Layout, G: TileGeom, gr0: int, gc0: int, check=False) -> np.ndarray:
-    """load_tile + write_tile + zero_pad.  check: frame points outside the
-    stored bricks read as zero (the fused kernel's image includes the ghost ring)."""
+def stage_tile(inp: np.ndarray, L: Layout, G: TileGeom, gr0: int, gc0: int, check=False,
+               zero_pad=True) -> np.ndarray:
+    """load_tile + write_tile (+ zero_pad).  check: frame points outside the
+    stored bricks read as zero (the fused kernel's image includes the ghost ring).
+    The 32x32-fragment kernels skip zero_pad; LDS words nobody wrote stay NaN."""
     lds = np.full(G.PLANE, np.nan)
     tid = np.arange(WG, dtype=np.int64)
     TW, TH = G.TW, G.TH
@@ -155,7 +161,7 @@ def stage_tile(inp: np.ndarray, L: Layout, G: TileGeom, gr0: int, gc0: int, chec
         v0[inside], v1[inside] = inp[s[inside]], inp[s[inside] + 1]
         lds[2 * d] = v0
         lds[2 * d + 1] = v1
-    for zc in range(2 * G.LR):
+    for zc in range(2 * G.LR if zero_pad else 0):
         d = (((zc >> 1) * G.LP) >> 1) + 8 * TW + 2 + (zc & 1)
         assert 0 <= 2 * d < G.PLANE
         lds[2 * d] = 0.0
@@ -296,6 +302,164 @@ def mfma_fused(inp, L: Layout, TH, TW, coeff5, rs, nsweeps, round_coeff=f16):
                 gr, gcb = er0 + 16 * br + m, ec0 + 16 * bc + 4 * g
                 o = _mfma_brick(lds, G, afrag, ZLO, ZHI, br, bc, edge, gr, gcb, L, rs)
                 _store(out, written, L, gr, gcb, o)
+    return out, written
+
+
+# ------------------------------------------------- MFMA, 32x32 fragments
+def _fold32(rs):
+    ZLO, ZHI = (0, 4) if rs == 2 else (1, 3)
+    NZ = ZHI - ZLO + 1
+    NG = 4 * NZ + (NZ + 1) // 2
+    return ZLO, NZ, (NG + 1) // 2
+
+
+def _slot32(mf, h, e, NZ):
+    """K slot (group 2*mf + h, element e) -> (shift index, haloed column, real)."""
+    if mf < 2 * NZ:
+        return mf >> 1, 8 * (2 * (mf & 1) + h) + e, True
+    z = 2 * (2 * (mf - 2 * NZ) + h) + (e >> 2)
+    return (z if z < NZ else 0), 32 + (e & 3), z < NZ
+
+
+def jmap32(mu):
+    return 16 * (mu >> 4) + 8 * ((mu >> 2) & 1) + 4 * ((mu >> 3) & 1) + (mu & 3)
+
+
+def _afrag32(coeff5, rs, round_coeff=f16):
+    ZLO, NZ, NMF = _fold32(rs)
+    af = np.zeros((NMF, 64, 8))
+    for mf in range(NMF):
+        for l in range(64):
+            j, h = jmap32(l % 32), l // 32
+            for e in range(8):
+                zi, k, real = _slot32(mf, h, e, NZ)
+                d = k - j
+                if real and 0 <= d <= 4:
+                    af[mf, l, e] = round_coeff(coeff5[ZLO + zi, d])
+    return af
+
+
+def _mfma_block32(lds, G: TileGeom, af, rs, br, bc, edge, gr, gcb, L: Layout):
+    """Ops32::load + mma_chunk32 + epilogue32: o[lane][v] for the block whose
+    window starts at LDS brick (br, bc); lane (i, h) owns row i, columns
+    16*(v>>3) + 8h + (v&7)."""
+    ZLO, NZ, NMF = _fold32(rs)
+    lane = np.arange(64)
+    i, h = lane & 31, lane >> 5
+    base = (16 * br + i + ZLO) * G.LP + 16 * bc
+    D = np.zeros((32, 32))
+    for mf in range(NMF):
+        if mf < 2 * NZ:
+            off = base + (mf >> 1) * G.LP + 8 * (2 * (mf & 1) + h)
+            _check(off, G.PLANE - 7, 8, "B operand LDS read (16 B)")
+            bfrag = lds[off[:, None] + np.arange(8)[None, :]]
+        else:
+            p = 2 * (mf - 2 * NZ) + h
+            za, zb = np.minimum(2 * p, NZ - 1), np.minimum(2 * p + 1, NZ - 1)
+            oa, ob = base + za * G.LP + 32, base + zb * G.LP + 32
+            _check(oa, G.PLANE - 3, 4, "B operand LDS read (8 B)")
+            _check(ob, G.PLANE - 3, 4, "B operand LDS read (8 B)")
+            bfrag = np.concatenate([lds[oa[:, None] + np.arange(4)[None, :]],
+                                    lds[ob[:, None] + np.arange(4)[None, :]]], axis=1)
+        Am = np.zeros((32, 16))
+        Bm = np.zeros((16, 32))
+        for l in range(64):
+            Am[l % 32, 8 * (l // 32):8 * (l // 32) + 8] = af[mf, l]
+            Bm[8 * (l // 32):8 * (l // 32) + 8, l % 32] = bfrag[l]
+        D += Am @ Bm
+    o = np.zeros((64, 16))
+    for v in range(16):
+        mu = 8 * (v // 4) + 4 * h + v % 4
+        o[:, v] = f16(D[mu, i])
+    if edge:
+        row = (16 * br + i + 2) * G.LP + 16 * bc + 2
+        for v in range(16):
+            cb = 16 * (v >> 3) + 8 * h + (v & 7)
+            _check(row + cb, G.PLANE, 1, "centre read")
+            keep = ~_interior(gr, gcb + cb, L, rs)
+            o[keep, v] = lds[row[keep] + cb[keep]]
+    return o
+
+
+def _store32(out, written, L, gr, gcb, h, o):
+    for s in range(2):
+        st = stor_index(gr, gcb + 16 * s + 8 * h, L.sbc)
+        _check(st, L.plane - 7, 8, "output store (16 B)")
+        for t in range(8):
+            assert not written[st + t].any(), "two lanes store the same element"
+            out[st + t] = o[:, 8 * s + t]
+            written[st + t] = True
+
+
+def _blocks32(wave, nq, nblk, bx, base):
+    for q in range(nq):
+        b = wave + 4 * q
+        if b >= nblk:
+            break
+        yield base + 2 * (b // bx), base + 2 * (b % bx)
+
+
+def mfma32_sweep(inp, L: Layout, TH, TW, coeff5, rs, round_coeff=f16):
+    """tcs_mfma32_sweep for FP16."""
+    assert L.nbr % TH == 0 and L.nbc % TW == 0 and TH % 2 == 0 and TW % 2 == 0
+    BX, NBLK = TW // 2, (TH // 2) * (TW // 2)
+    assert NBLK % 4 == 0
+    G = TileGeom(TH, TW)
+    af = _afrag32(coeff5, rs, round_coeff)
+    out = np.zeros(L.plane)
+    written = np.zeros(L.plane, dtype=bool)
+    i, h = np.arange(64) & 31, np.arange(64) >> 5
+    for gr0, gc0 in _tiles(L, TH, TW, 0):
+        lds = stage_tile(inp, L, G, gr0, gc0, zero_pad=False)
+        edge = not (gr0 >= rs and gr0 + 16 * TH <= L.rows - rs and gc0 >= rs and gc0 + 16 * TW <= L.cols - rs)
+        for wave in range(4):
+            for br, bc in _blocks32(wave, NBLK // 4, NBLK, BX, 0):
+                gr, gcb = gr0 + 16 * br + i, gc0 + 16 * bc
+                o = _mfma_block32(lds, G, af, rs, br, bc, edge, gr, gcb, L)
+                assert not np.isnan(o).any(), "block read unwritten LDS"
+                _store32(out, written, L, gr, gcb, h, o)
+    return out, written
+
+
+def mfma32_fused(inp, L: Layout, TH, TW, coeff5, rs, nsweeps, round_coeff=f16):
+    """tcs_mfma32_fused: the fused scheme over 2x2-brick blocks."""
+    assert L.nbr % TH == 0 and L.nbc % TW == 0 and 1 <= nsweeps and rs * (nsweeps - 1) <= 16
+    BH, BW = TH + 2, TW + 2
+    BXA, NBA = BW // 2, (BH // 2) * (BW // 2)
+    QA = -(-NBA // 4)
+    BXC, NBC = TW // 2, (TH // 2) * (TW // 2)
+    G = TileGeom(BH, BW)
+    af = _afrag32(coeff5, rs, round_coeff)
+    out = np.zeros(L.plane)
+    written = np.zeros(L.plane, dtype=bool)
+    i, h = np.arange(64) & 31, np.arange(64) >> 5
+    for gr0, gc0 in _tiles(L, TH, TW, 0):
+        er0, ec0 = gr0 - 16, gc0 - 16
+        lds = stage_tile(inp, L, G, er0, ec0, check=True, zero_pad=False)
+        edge = not (er0 >= rs and er0 + 16 * BH <= L.rows - rs and ec0 >= rs and ec0 + 16 * BW <= L.cols - rs)
+        for sw in range(nsweeps - 1):
+            res = []
+            for wave in range(4):
+                for br, bc in _blocks32(wave, QA, NBA, BXA, 0):
+                    o = _mfma_block32(lds, G, af, rs, br, bc, edge, er0 + 16 * br + i, ec0 + 16 * bc, L)
+                    res.append((br, bc, o))
+            covered = np.zeros(G.PLANE, dtype=bool)
+            for br, bc, o in res:
+                d = ((16 * br + i + 2) * G.LP + 16 * bc + 8 * h + 2) >> 1
+                for s_ in range(2):
+                    for t in range(4):
+                        e0 = 2 * (d + 8 * s_ + t)
+                        _check(e0, G.PLANE - 1, 2, "fused write-back")
+                        assert not covered[e0].any()
+                        covered[e0] = covered[e0 + 1] = True
+                        lds[e0] = o[:, 8 * s_ + 2 * t]
+                        lds[e0 + 1] = o[:, 8 * s_ + 2 * t + 1]
+        for wave in range(4):
+            for br, bc in _blocks32(wave, NBC // 4, NBC, BXC, 1):
+                gr, gcb = er0 + 16 * br + i, ec0 + 16 * bc
+                o = _mfma_block32(lds, G, af, rs, br, bc, edge, gr, gcb, L)
+                assert not np.isnan(o).any(), "block read unwritten LDS"
+                _store32(out, written, L, gr, gcb, h, o)
     return out, written
 
 

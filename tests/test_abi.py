@@ -148,12 +148,38 @@ def test_batch_validation_needs_no_gpu():
     ok = T.Problem("box2d1r", 40, 40, B, tile=(2, 4))
     bad_tile = T.Problem("box2d1r", 40, 40, B, tile=(2, 8))
     direct = T.Problem("box2d1r", 40, 40, B, method="direct", tile=(2, 4))
+    bad32 = T.Problem("box2d1r", 40, 40, B, tile=(8, 8), frag=32)
+    too_fused = T.Problem("box2d2r", 40, 40, np.ones((5, 5), np.float32), tile=(2, 4), fuse=10)
     fake = (ctypes.c_void_p * 1)(0x1000)
     fake2 = (ctypes.c_void_p * 1)(0x2000)
     h = ctypes.c_void_p()
-    for p, want in ((bad_tile, T._lib.ERR_UNSUPPORTED), (direct, T._lib.ERR_INVALID)):
+    for p, want in ((bad_tile, T._lib.ERR_UNSUPPORTED), (direct, T._lib.ERR_INVALID),
+                    (bad32, T._lib.ERR_UNSUPPORTED), (too_fused, T._lib.ERR_INVALID)):
         arr = (T._lib.ProblemStruct * 1)(p.s)
         assert L.tcs_batch_create(ctypes.byref(h), 1, arr, fake, fake2) == want
     arr = (T._lib.ProblemStruct * 1)(ok.s)
     assert L.tcs_batch_create(ctypes.byref(h), 1, arr, fake, fake) == T._lib.ERR_INVALID  # same buffer twice
     assert L.tcs_batch_run(None, 1, None) == T._lib.ERR_INVALID
+
+
+def test_frag32_setup_and_validation():
+    import tcstencil_amd as T
+    B = np.ones((5, 5), np.float32) / 25
+    p = T.Problem("box2d2r", 16384, 16384, B, frag=32)
+    assert p.frag == 32 and p.tile == (8, 8)
+    assert p.kernel_name == "tcs_mfma32_sweep<8, 8, 0, 4, 0>"
+    assert p.mfma_per_brick == 12 and p.mfma_flops_per_brick == 12 * 32 * 32 * 16 * 2 // 4
+    p.fuse = 4
+    p.set_tile(4, 8)
+    assert p.fused_kernel_name == "tcs_mfma32_fused<4, 8, 0, 4, 0>"
+    p.set_tile(2, 4)  # no 2x2-brick split over 4 waves: no kernel
+    assert p.kernel_name == "" and p.fused_kernel_name == ""
+    q = T.Problem("box2d1r", 100, 100, np.ones((3, 3), np.float32), frag=32, precision="bf16")
+    assert q.tile in ((4, 4), (2, 8), (4, 8), (8, 8)) and q.mfma_per_brick == 7
+    assert T.Problem("box2d1r", 64, 64, np.ones((3, 3), np.float32)).mfma_flops_per_brick == 3 * 16384
+    with pytest.raises(T.TcsError, match="32x32"):
+        T.Problem("box2d2r", 64, 64, B, precision="split", frag=32)
+    with pytest.raises(T.TcsError, match="32x32"):
+        T.Problem("box2d2r", 64, 64, B, method="direct", frag=32)
+    with pytest.raises(T.TcsError, match="fragment"):
+        T.Problem("box2d2r", 64, 64, B, frag=24)

@@ -128,3 +128,78 @@ def test_emulated_fused_equals_repeated_sweeps(oracle, stencil, case):
     for _ in range(k):
         ref = E.f16(oracle.sweep_f64(stencil, ref, B))
     assert np.array_equal(E.unpack(fused, L), ref)
+
+
+CASES32 = [(70, 150, 4, 4), (70, 150, 4, 8), (33, 17, 4, 4), (130, 140, 8, 8), (5, 3, 2, 8), (64, 128, 2, 8)]
+
+
+def _data_mask(L):
+    data = np.zeros(L.plane, dtype=bool)
+    for bi in range(L.nbr):
+        base = ((bi + 1) * L.sbc + 1) * 256
+        data[base:base + L.nbc * 256] = True
+    return data
+
+
+@pytest.mark.parametrize("stencil", STENCILS)
+@pytest.mark.parametrize("case", CASES32, ids=lambda c: "%dx%d_t%dx%d" % c)
+def test_emulated_frag32_exact(oracle, stencil, case):
+    """The 32x32-fragment kernel (z folded into K, permuted output columns)
+    reproduces fp16(FP64 oracle) exactly on dyadic data, and never reads an
+    LDS word that staging did not write (no zero pad in this kernel)."""
+    rows, cols, TH, TW = case
+    r = RADIUS[stencil]
+    A, B, c5 = dyadic_problem(stencil, rows, cols, seed=rows * 1000 + cols + 7)
+    L = E.Layout(rows, cols, max(TH, TW))
+    st = E.pack(A, L)
+    out, written = E.mfma32_sweep(st, L, TH, TW, c5, r)
+    assert np.array_equal(written, _data_mask(L))
+    assert np.array_equal(E.unpack(out, L), E.f16(oracle.sweep_f64(stencil, A, B)))
+
+
+def test_frag32_slots_cover_each_shift_and_column_once():
+    """Every (shift, haloed column) pair of the 36-column window appears in
+    exactly one real K slot; the output column map is a permutation."""
+    for rs in (1, 2):
+        ZLO, NZ, NMF = E._fold32(rs)
+        seen = {}
+        for mf in range(NMF):
+            for h in range(2):
+                for e in range(8):
+                    zi, k, real = E._slot32(mf, h, e, NZ)
+                    if real:
+                        assert (zi, k) not in seen
+                        seen[(zi, k)] = (mf, h, e)
+        assert set(seen) == {(z, k) for z in range(NZ) for k in range(36)}
+        assert NMF == (12 if rs == 2 else 7)
+    assert sorted(E.jmap32(m) for m in range(32)) == list(range(32))
+
+
+@pytest.mark.parametrize("stencil", ["box2d2r", "star2d1r"])
+@pytest.mark.parametrize("case", [(70, 150, 4, 4, 3), (33, 140, 4, 8, 4), (40, 40, 8, 8, 2)],
+                         ids=lambda c: "%dx%d_t%dx%d_k%d" % c)
+def test_emulated_frag32_fused_equals_repeated_sweeps(oracle, stencil, case):
+    rows, cols, TH, TW, k = case
+    r = RADIUS[stencil]
+    A, _, _ = dyadic_problem(stencil, rows, cols, seed=rows + cols + 1)
+    n = 2 * r + 1
+    B = np.zeros((n, n))
+    B[r, r] = 0.5
+    B[r, 0] = 0.25
+    B[0, r] = 0.25
+    c5 = np.zeros((5, 5))
+    c5[2 - r:3 + r, 2 - r:3 + r] = B
+    L = E.Layout(rows, cols, max(TH, TW))
+    st = E.pack(A, L)
+    fused, written = E.mfma32_fused(st, L, TH, TW, c5, r, k)
+    data = _data_mask(L)
+    assert np.array_equal(written, data)
+    x = st
+    for _ in range(k):
+        x, _ = E.mfma32_sweep(x, L, TH, TW, c5, r)
+        x = np.where(data, x, 0.0)
+    assert np.array_equal(fused, x)
+    ref = A
+    for _ in range(k):
+        ref = E.f16(oracle.sweep_f64(stencil, ref, B))
+    assert np.array_equal(E.unpack(fused, L), ref)

@@ -53,9 +53,9 @@ def dyadic(stencil, rows, cols, seed):
 
 
 def gpu_sweeps(T, torch, stencil, A, B, method="mfma", precision="fp16", tile=None, sweeps=1,
-               scale_exp=0, graph=False):
+               scale_exp=0, graph=False, frag=16, fuse=1):
     rows, cols = A.shape
-    p = T.Problem(stencil, rows, cols, B, method=method, precision=precision, tile=tile)
+    p = T.Problem(stencil, rows, cols, B, method=method, precision=precision, tile=tile, frag=frag, fuse=fuse)
     src = torch.from_numpy(np.ascontiguousarray(A)).cuda()
     b0, b1 = p.alloc(), p.alloc()
     p.clear(b1)
@@ -483,6 +483,168 @@ def test_batch_bitwise_equals_per_grid(T, torch):
         assert torch.equal(p.unpack(b1), ref)
     g.destroy()
     batch.destroy()
+
+
+# ------------------------------------------------------- 32x32 fragments
+FRAG32_TILES = [(4, 4), (2, 8), (4, 8), (8, 8)]
+FRAG32_FUSED_TILES = [(4, 4), (4, 8), (8, 8)]
+
+
+@pytest.mark.parametrize("stencil", STENCILS)
+@pytest.mark.parametrize("precision", ["fp16", "bf16"])
+def test_frag32_dyadic_exact(T, torch, oracle, stencil, precision):
+    """v_mfma_f32_32x32x16 kernels: bitwise fp16/bf16(oracle) on exact inputs."""
+    for tile in FRAG32_TILES:
+        for rows, cols in ((70, 150), (33, 17), (130, 260)):
+            say("case frag32", stencil, precision, tile, rows, cols)
+            A, B = dyadic(stencil, rows, cols, seed=rows + cols + 5)
+            got = gpu_sweeps(T, torch, stencil, A, B, precision=precision, tile=tile, frag=32)
+            ref = oracle.sweep_f64(stencil, A, B)
+            want = round_bf16(torch, ref) if precision == "bf16" else ref.astype(np.float16).astype(np.float64)
+            assert np.array_equal(got, want), f"max diff {np.abs(got - want).max()}"
+
+
+@pytest.mark.parametrize("stencil", STENCILS)
+def test_frag32_random_within_bound(T, torch, oracle, stencil):
+    """Random FP16 data: same rounding bound as the 16x16 form, and tile-independent bits."""
+    A, B = random_problem(T, torch, stencil, 200, 333, seed=41)
+    ref = oracle.sweep_f64(stencil, A, B)
+    outs = []
+    for tile in FRAG32_TILES:
+        say("case frag32 random", stencil, tile)
+        outs.append(gpu_sweeps(T, torch, stencil, A, B.astype(np.float32), tile=tile, frag=32))
+    for o in outs[1:]:
+        assert np.array_equal(o, outs[0])
+    r = RADIUS[stencil]
+    err = np.abs(outs[0] - ref)[r:-r, r:-r]
+    max_ref = np.abs(ref).max()
+    # coefficient rounding (|B| <= 1, sum |B| = 1, |x| <= 1) + output rounding
+    assert err.max() <= 2.0 ** -11 * (1.0 + max_ref) * 1.01 + 1e-7
+    assert oracle.error_norms(outs[0], ref, r)["rel_l2"] <= 2e-3
+
+
+@pytest.mark.parametrize("stencil", STENCILS)
+@pytest.mark.parametrize("precision", ["fp16", "bf16"])
+def test_frag32_fused(T, torch, oracle, stencil, precision):
+    """Fused 32x32-fragment sweeps: bitwise equal to the oracle on inputs that
+    stay exact through k sweeps, and within rounding of single sweeps on random
+    inputs (the ring is computed in blocks offset by one brick, so FP32 sums of
+    a point may be grouped differently than in a single-sweep launch)."""
+    r = RADIUS[stencil]
+    n = 2 * r + 1
+    A, _ = dyadic(stencil, 96, 200, seed=9)
+    Bx = np.zeros((n, n), np.float32)
+    Bx[r, r], Bx[r, 0], Bx[0, r] = 0.5, 0.25, 0.25
+    for tile in FRAG32_FUSED_TILES:
+        for fuse in (2, 1 + 16 // r):
+            say("case frag32 fused exact", stencil, precision, tile, fuse)
+            got = gpu_sweeps(T, torch, stencil, A, Bx, precision=precision, tile=tile, frag=32, fuse=fuse,
+                             sweeps=fuse + 1)
+            want = A
+            for _ in range(fuse + 1):
+                w = oracle.sweep_f64(stencil, want, Bx.astype(np.float64))
+                want = round_bf16(torch, w) if precision == "bf16" else w.astype(np.float16).astype(np.float64)
+            assert np.array_equal(got, want)
+    Ar, Br = random_problem(T, torch, stencil, 160, 300, seed=43)
+    single = gpu_sweeps(T, torch, stencil, Ar, Br.astype(np.float32), precision=precision, frag=32, sweeps=6)
+    for tile in FRAG32_FUSED_TILES:
+        say("case frag32 fused random", stencil, precision, tile)
+        got = gpu_sweeps(T, torch, stencil, Ar, Br.astype(np.float32), precision=precision, tile=tile, frag=32,
+                         fuse=3, sweeps=6)
+        assert oracle.error_norms(got, single, r)["rel_l2"] <= (2e-3 if precision == "bf16" else 2e-4)
+
+
+def test_frag32_batch_bitwise_equals_per_grid(T, torch):
+    rng = np.random.default_rng(51)
+    probs, b0s, b1s, refs = [], [], [], []
+    for i in range(8):
+        stencil = ("box2d1r", "star2d2r")[i % 2]
+        rows, cols = (int(v) for v in rng.integers(5, 300, 2))
+        frag, tile = [(32, (4, 4)), (32, (4, 8)), (16, (2, 4)), (16, (4, 8))][i % 4]
+        B = T.gen_coeffs(stencil, 200 + i)
+        x = T.gen_uniform(rows, cols, seed=200 + i, dtype=torch.float32)
+        p = T.Problem(stencil, rows, cols, B, tile=tile, frag=frag)
+        b0, b1 = p.alloc(), p.alloc()
+        p.clear(b1)
+        p.pack(x, b0)
+        probs.append(p), b0s.append(b0), b1s.append(b1)
+        q0, q1 = p.alloc(), p.alloc()
+        q1.zero_()
+        p.pack(x, q0)
+        refs.append(p.unpack(p.run(q0, q1, 3)))
+    say("case frag32 batch", [(p.rows, p.cols, p.tile, p.frag) for p in probs])
+    batch = T.Batch(probs, b0s, b1s)
+    assert batch.launches_per_sweep == 4
+    batch.run(3)
+    torch.cuda.synchronize()
+    for p, b1, ref in zip(probs, b1s, refs):
+        assert torch.equal(p.unpack(b1), ref)
+    batch.destroy()
+
+
+def test_batch_fused_persistent_and_forked_graph(T, torch):
+    """Fused batched launches (per-problem fuse), persistent grids and the
+    forked-graph capture give the same bits as each grid run on its own with
+    the same settings."""
+    rng = np.random.default_rng(61)
+    cfgs = [(16, (2, 4), 3, 0), (16, (4, 8), 4, 1), (32, (4, 4), 3, 0), (32, (4, 8), 2, 2), (16, (4, 4), 1, 1)]
+    probs, b0s, b1s, refs, xs = [], [], [], [], []
+    for i in range(10):
+        stencil = ("box2d1r", "star2d2r")[i % 2]
+        rows, cols = (int(v) for v in rng.integers(20, 400, 2))
+        frag, tile, fuse, wg = cfgs[i % len(cfgs)]
+        B = T.gen_coeffs(stencil, 300 + i)
+        x = T.gen_uniform(rows, cols, seed=300 + i, dtype=torch.float32)
+        p = T.Problem(stencil, rows, cols, B, tile=tile, frag=frag, fuse=fuse, wg_per_cu=wg)
+        b0, b1 = p.alloc(), p.alloc()
+        p.clear(b1)
+        p.pack(x, b0)
+        probs.append(p), b0s.append(b0), b1s.append(b1), xs.append(x)
+        # per-grid reference: same tile, fragment and fuse, one tile per workgroup
+        pr = T.Problem(stencil, rows, cols, B, tile=tile, frag=frag, fuse=fuse)
+        q0, q1 = pr.alloc(), pr.alloc()
+        q1.zero_()
+        pr.pack(x, q0)
+        refs.append(pr.unpack(pr.run(q0, q1, 7)))
+    say("case batch fused", [(p.rows, p.cols, p.tile, p.frag, p.fuse, p.wg_per_cu) for p in probs])
+    batch = T.Batch(probs, b0s, b1s)
+    assert batch.launches_per_sweep == len(cfgs)
+    batch.run(7)
+    torch.cuda.synchronize()
+    for p, b1, ref in zip(probs, b1s, refs):
+        assert torch.equal(p.unpack(b1), ref)
+    g = batch.graph(7)
+    for rep in range(3):
+        for p, b0, b1, x in zip(probs, b0s, b1s, xs):
+            p.pack(x, b0)
+            p.clear(b1)
+        torch.cuda.synchronize()
+        g.launch()
+        torch.cuda.synchronize()
+        for p, b1, ref in zip(probs, b1s, refs):
+            assert torch.equal(p.unpack(b1), ref), f"graph replay {rep}"
+    g.destroy()
+    batch.destroy()
+
+
+def test_frag32_full_size_box2d2r_16384(T, torch):
+    say("case frag32 full size")
+    N = 16384
+    B = T.gen_coeffs("box2d2r", 1)
+    x64 = T.gen_sinusoid(N, N, seed=1, noise=1e-3)
+    T.round_to(x64, "fp16")
+    ref = torch.empty_like(x64).copy_(x64)
+    T.oracle_sweep_f64("box2d2r", x64, B.astype(np.float64), ref)
+    p = T.Problem("box2d2r", N, N, B, frag=32)
+    b0, b1 = p.alloc(), p.alloc()
+    p.clear(b1)
+    p.pack(x64, b0)
+    del x64
+    p.sweep(b0, b1)
+    res = T.error_norms(p.unpack(b1), ref, 2)
+    say("  ", res)
+    assert res["rel_l2"] <= 2e-3
+    assert res["max_abs"] <= (2.0 ** -11) * 2.0 * 1.001 + 1e-6
 
 
 def test_c1_star2d1r_1024_vs_gold(T, torch, oracle):
