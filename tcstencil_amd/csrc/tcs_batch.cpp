@@ -89,11 +89,7 @@ extern "C" int tcs_batch_create(tcs_batch **out, int nprob, const tcs_problem *p
         q.rs = p.radius;
         q.tiles_x = L.nbc / p.tile_w;
         q.out_scale = ldexpf(1.0f, -p.coeff_exp);
-        for (int k = 0; k < 25; ++k) {
-            q.cf[k] = p.coeff[k];
-            q.ch[k] = p.coeff_hi[k];
-            q.cl[k] = p.coeff_lo[k];
-        }
+        build_coef_windows(&p, &q.cw);
         const Shape shape{p.tile_h, p.tile_w, frag, fuse, p.wg_per_cu};
         int gi = -1;
         for (size_t s = 0; s < shapes.size(); ++s)

@@ -47,6 +47,35 @@ uint16_t f32_to_f16_bits(float f) {
     return sign | (uint16_t)q;
 }
 
+// float -> bfloat16 bits, round to nearest even (NaN kept quiet).
+uint16_t f32_to_bf16_bits(float f) {
+    uint32_t x;
+    memcpy(&x, &f, 4);
+    if ((x & 0x7fffffffu) > 0x7f800000u) return (uint16_t)((x >> 16) | 0x40u);
+    return (uint16_t)((x + 0x7fffu + ((x >> 16) & 1u)) >> 16);
+}
+
+void build_coef_windows(const tcs_problem *p, CoefWindows *cw) {
+    memset(cw, 0, sizeof *cw);
+    for (int pl = 0; pl < 2; ++pl)
+        for (int z = 0; z < 5; ++z)
+            for (int j = 0; j < 12; ++j)
+                for (int e = 0; e < 8; ++e) {
+                    const int h = j - 7 + e;  // tap of window element e
+                    if (h < 0 || h > 4) continue;
+                    const int k = z * 5 + h;
+                    uint16_t bits;
+                    if (p->precision == TCS_PREC_SPLIT)
+                        bits = pl == 0 ? p->coeff_hi[k] : p->coeff_lo[k];
+                    else if (pl == 0)
+                        bits = p->precision == TCS_PREC_BF16 ? f32_to_bf16_bits(p->coeff[k])
+                                                             : f32_to_f16_bits(p->coeff[k]);
+                    else
+                        continue;
+                    cw->w[pl][z][j][e >> 1] |= (uint32_t)bits << (16 * (e & 1));
+                }
+}
+
 float f16_bits_to_f32(uint16_t h) {
     const uint32_t sign = (uint32_t)(h & 0x8000u) << 16;
     const uint32_t e = (h >> 10) & 0x1fu, m = h & 0x3ffu;
@@ -148,7 +177,7 @@ static void default_tile(const tcs_problem *p, int *th, int *tw) {
     const int last = frag32(p) ? 3 : 4;  // 32x32 fragments need 2x2-brick blocks, 4 per tile
     int first;
     if (p->method == TCS_METHOD_MFMA)
-        first = (p->precision != TCS_PREC_SPLIT && p->radius == 2) ? 0 : 1;
+        first = frag32(p) ? 0 : 1;  // 8x8 (32x32 fragments), else 4x8
     else
         first = p->stencil == TCS_BOX2D2R ? 1 : 4;
     const long br = (p->layout.rows + 15) / 16, bc = (p->layout.cols + 15) / 16;

@@ -27,6 +27,17 @@ __host__ __device__ inline int64_t stor_index(int gr, int gc, int sbc) {
     return ((int64_t)bi * sbc + bj) * 256 + ((gr & 15) << 4) + (gc & 15);
 }
 
+// Coefficient operand windows of the 16x16x32 MFMA, built on the host
+// (build_coef_windows).  Lane (m, g) of the A operand holds c[z][8g + e - m],
+// e = 0..7 (zero outside the taps 0..4): the window at d0 = 8g - m of the
+// zero-padded tap row.  Only d0 in [-7, 4] gives a non-zero window, so
+// w[p][z][j] (j < 12) holds the window at d0 = j - 7 and w[p][z][12] zeros, in
+// storage bits (plane p: FP16 or BF16 taps; SPLIT: hi, then lo).  A lane loads
+// its fragments straight from here (16 bytes each), with no per-element selects.
+struct CoefWindows {
+    uint32_t w[2][5][13][4];
+};
+
 // Arguments of one sweep launch, passed by value.
 struct SweepArgs {
     const uint16_t *in;
@@ -35,12 +46,12 @@ struct SweepArgs {
     int32_t rows, cols;    // logical grid
     int32_t sbr, sbc;      // stored brick rows, stored bricks per brick row
     int32_t tiles_x;       // tiles per tile row
-    int32_t ntiles;        // tiles in the grid
-    int32_t tiles_per_xcd; // ceil(ntiles / 8)
+    int32_t tiles_xp;      // tiles_x rounded up to a multiple of 8 (virtual tile row)
+    int32_t nvirt;         // virtual tiles: tile rows * tiles_xp
     int32_t rs;            // stencil radius: points closer than rs to the edge are copied
     float out_scale;       // SPLIT: 2^-coeff_exp
-    float cf[25];          // 5x5 FP32 coefficient embedding
-    uint16_t ch[25], cl[25];
+    float cf[25];          // 5x5 FP32 coefficient embedding (direct kernel)
+    CoefWindows cw;        // MFMA operand windows (FP16/BF16 taps, or SPLIT hi and lo)
 };
 
 // One problem of a batch, device-resident (tcs_batch.cpp builds them).
@@ -51,8 +62,7 @@ struct BatchProblem {
     int32_t rs;            // radius
     int32_t tiles_x;       // tiles per tile row of this problem
     float out_scale;
-    float cf[25];
-    uint16_t ch[25], cl[25];
+    CoefWindows cw;        // MFMA operand windows of the problem's coefficients
 };
 
 // One tile of a batch launch: problem index and tile index within it.
@@ -80,7 +90,11 @@ int hip_error(hipError_t e, const char *what);
 
 // Host FP16/BF16 rounding helpers (round to nearest even).
 uint16_t f32_to_f16_bits(float f);
+uint16_t f32_to_bf16_bits(float f);
 float f16_bits_to_f32(uint16_t h);
+
+// MFMA operand windows of a problem's coefficients (tcs_host.cpp).
+void build_coef_windows(const tcs_problem *p, CoefWindows *cw);
 
 }  // namespace tcs
 

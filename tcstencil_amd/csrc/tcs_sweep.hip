@@ -93,7 +93,7 @@ __device__ __forceinline__ void halo_coords(int h, int &lr, int &ld) {
 // for lr < LR, lc < 16*BW + 4; columns [16*BW+4, LP) hold zeros.  (r0, c0) is
 // the first element of the first data brick.  With CHECK, frame points outside
 // the stored bricks (possible when the data bricks include the ghost ring) read
-// as zero.
+// as zero.  (Nontemporal data-brick loads measured slower.)
 template <class G, bool CHECK>
 __device__ __forceinline__ void load_tile(const SweepArgs &a, TileRegs<G> &R, int r0, int c0, int tid) {
 #pragma unroll
@@ -177,24 +177,34 @@ __device__ __forceinline__ bool point_is_interior(const SweepArgs &a, int gr, in
     return gr >= a.rs && gr < a.rows - a.rs && gc >= a.rs && gc < a.cols - a.rs;
 }
 
-// Persistent tile schedule.  Workgroup w runs on XCD w % 8; XCD x owns the
-// contiguous tile range [x*tpx, (x+1)*tpx), and its gridDim.x/8 workgroups step
-// through that range together, so the tiles in flight on one XCD are
-// neighbours and their halos meet in that XCD's L2.
+// Tile schedule.  Tile rows are padded to tiles_xp virtual columns (a multiple
+// of 8); virtual tiles with tx >= tiles_x are empty.  Workgroup w takes virtual
+// tiles v = w, w + gridDim.x, ... (gridDim.x a multiple of 8).  Workgroups are
+// dispatched to the 8 XCDs round robin, so every tile of tile column tx runs on
+// XCD tx % 8 and vertically adjacent tiles meet in that XCD's L2 for their halo
+// rows, while the tiles in flight at any moment form one contiguous band of
+// tile rows, which spreads the traffic over every HBM channel.  Measured on
+// box2d2r 16384^2 (8x8-brick tiles): this order beats eight XCD-private
+// contiguous tile ranges (the ranges keep horizontal neighbours on one XCD,
+// but run 13% slower as a plain tile copy) and runs of up to 16 tiles per XCD.
 struct Schedule {
     int tile, end, step;
 };
 
+// First virtual tile >= v (in steps of `step`) that is not empty.
+__device__ __forceinline__ int skip_empty(const SweepArgs &a, int v, int step) {
+    while (v < a.nvirt && v % a.tiles_xp >= a.tiles_x) v += step;
+    return v;
+}
+
 __device__ __forceinline__ Schedule schedule(const SweepArgs &a) {
-    const int x = blockIdx.x % kXCD, k = blockIdx.x / kXCD;
-    const int begin = x * a.tiles_per_xcd;
-    const int end = begin + a.tiles_per_xcd < a.ntiles ? begin + a.tiles_per_xcd : a.ntiles;
-    return Schedule{begin + k, end, (int)gridDim.x / kXCD};
+    const int step = (int)gridDim.x;
+    return Schedule{skip_empty(a, (int)blockIdx.x, step), a.nvirt, step};
 }
 
 template <int TH, int TW>
-__device__ __forceinline__ void tile_origin(const SweepArgs &a, int tile, int &gr0, int &gc0) {
-    const int ty = tile / a.tiles_x, tx = tile - ty * a.tiles_x;
+__device__ __forceinline__ void tile_origin(const SweepArgs &a, int v, int &gr0, int &gc0) {
+    const int ty = v / a.tiles_xp, tx = v - ty * a.tiles_xp;
     gr0 = 16 * TH * ty;
     gc0 = 16 * TW * tx;
 }
@@ -231,39 +241,25 @@ struct Frag<TCS_PREC_SPLIT> : Frag<TCS_PREC_FP16> {};
 
 // Coefficient fragments.  Lane l holds A[m][k] for m = l%16 and
 // k = 8*(l/16) + e, e = 0..7: the coefficient at row offset z and column
-// offset k - m when 0 <= k - m <= 4, else 0.
+// offset k - m when 0 <= k - m <= 4, else 0 -- the window at d0 = 8g - m of
+// the zero-padded tap row, loaded whole from the host-built window table
+// (CoefWindows; window 12 is all zeros).
 template <int ZLO, int ZHI, int PREC>
 struct Coef {
     typedef typename Frag<PREC>::type frag_t;
     static constexpr int NZ = ZHI - ZLO + 1;
     frag_t hi[NZ], lo[NZ];
 
-    __device__ __forceinline__ void build(const SweepArgs &a, int m, int g) {
+    __device__ __forceinline__ void load(const CoefWindows &cw, int m, int g) {
+        const int d0 = 8 * g - m;
+        const int j = d0 >= -7 && d0 <= 4 ? d0 + 7 : 12;
 #pragma unroll
         for (int zi = 0; zi < NZ; ++zi) {
-#pragma unroll
-            for (int e = 0; e < 8; ++e) {
-                const int d = 8 * g + e - m;
-                if constexpr (PREC == TCS_PREC_SPLIT) {
-                    uint16_t hb = 0, lb = 0;
-#pragma unroll
-                    for (int dd = 0; dd < 5; ++dd)
-                        if (d == dd) {
-                            hb = a.ch[(ZLO + zi) * 5 + dd];
-                            lb = a.cl[(ZLO + zi) * 5 + dd];
-                        }
-                    hi[zi][e] = __builtin_bit_cast(_Float16, hb);
-                    lo[zi][e] = __builtin_bit_cast(_Float16, lb);
-                } else {
-                    float v = 0.0f;
-#pragma unroll
-                    for (int dd = 0; dd < 5; ++dd)
-                        if (d == dd) v = a.cf[(ZLO + zi) * 5 + dd];
-                    if constexpr (PREC == TCS_PREC_BF16)
-                        hi[zi][e] = static_cast<__bf16>(v);
-                    else
-                        hi[zi][e] = static_cast<_Float16>(v);
-                }
+            const uint32_t *wh = cw.w[0][ZLO + zi][j];
+            hi[zi] = __builtin_bit_cast(frag_t, u32x4{wh[0], wh[1], wh[2], wh[3]});
+            if constexpr (PREC == TCS_PREC_SPLIT) {
+                const uint32_t *wl = cw.w[1][ZLO + zi][j];
+                lo[zi] = __builtin_bit_cast(frag_t, u32x4{wl[0], wl[1], wl[2], wl[3]});
             }
         }
     }
@@ -355,10 +351,21 @@ __device__ __forceinline__ void brick_of(int b, int &br, int &bc) {
 template <int PREC>
 constexpr bool kLookahead = PREC != TCS_PREC_SPLIT;
 
+// Single-sweep FP16/BF16 kernels on tiles below 8x8 bricks (LDS for up to 8
+// workgroups per CU) trade the look-ahead for 8 waves per SIMD (at most 64
+// VGPRs): other workgroups then hide the LDS latency, and the extra resident
+// tiles keep more loads in flight (box2d2r 16384^2, 4x8 tile: 193 -> 180 us).
+// 8x8 tiles are held to 4 workgroups per CU by LDS and keep the look-ahead;
+// persistent grids hold the next tile in registers and keep it too.
+template <int TH, int TW, int PREC, bool PERSIST>
+constexpr bool kHighOccupancy = PREC != TCS_PREC_SPLIT && TH * TW < 64 && !PERSIST;
+
 template <int TH, int TW, int ZLO, int ZHI, int PREC, bool PERSIST>
-__global__ __launch_bounds__(kWG) void tcs_mfma_sweep(SweepArgs a) {
+__global__ __launch_bounds__(kWG)
+__attribute__((amdgpu_waves_per_eu(kHighOccupancy<TH, TW, PREC, PERSIST> ? 8 : 1, 8)))
+void tcs_mfma_sweep(SweepArgs a) {
     constexpr int NPL = PREC == TCS_PREC_SPLIT ? 2 : 1;
-    constexpr bool LOOKAHEAD = kLookahead<PREC>;
+    constexpr bool LOOKAHEAD = kLookahead<PREC> && !kHighOccupancy<TH, TW, PREC, PERSIST>;
     using T = Tile<TH, TW, NPL>;
     __shared__ __attribute__((aligned(16))) uint16_t lds[NPL * T::PLANE];
 
@@ -374,14 +381,14 @@ __global__ __launch_bounds__(kWG) void tcs_mfma_sweep(SweepArgs a) {
     const int lane = tid & 63, wave = tid >> 6;
     const int m = lane & 15, g = lane >> 4;
     Coef<ZLO, ZHI, PREC> C;
-    C.build(a, m, g);
+    C.load(a.cw, m, g);
     constexpr int NB = TH * TW / 4;
 
     for (;;) {
         write_tile<T>(R, lds, tid);
         __syncthreads();
         const int cgr0 = gr0, cgc0 = gc0;  // the tile in LDS
-        const int next = sc.tile + sc.step;
+        const int next = skip_empty(a, sc.tile + sc.step, sc.step);
         if (PERSIST && next < sc.end) {
             tile_origin<TH, TW>(a, next, gr0, gc0);
             load_tile<T, false>(a, R, gr0, gc0, tid);  // in flight during the compute below
@@ -437,7 +444,8 @@ template <int TH, int TW, int NPL>
 using FusedGeom = Geom<TH + 2, TW + 2, NPL>;
 
 template <int TH, int TW, int ZLO, int ZHI, int PREC>
-__device__ __forceinline__ void fused_tile(const SweepArgs &a, uint16_t *lds, int gr0, int gc0, int nsweeps) {
+__device__ __forceinline__ void fused_tile(const SweepArgs &a, const CoefWindows &cw, uint16_t *lds, int gr0,
+                                           int gc0, int nsweeps) {
     constexpr int NPL = PREC == TCS_PREC_SPLIT ? 2 : 1;
     constexpr bool LOOKAHEAD = kLookahead<PREC>;
     using G = FusedGeom<TH, TW, NPL>;
@@ -455,7 +463,7 @@ __device__ __forceinline__ void fused_tile(const SweepArgs &a, uint16_t *lds, in
     const int lane = tid & 63, wave = tid >> 6;
     const int m = lane & 15, g = lane >> 4;
     Coef<ZLO, ZHI, PREC> C;
-    C.build(a, m, g);
+    C.load(cw, m, g);
     const bool edge = !tile_is_interior(a, er0, ec0, 16 * G::BH, 16 * G::BW);
     __syncthreads();
 
@@ -538,7 +546,7 @@ __global__ __launch_bounds__(kWG) void tcs_mfma_fused(SweepArgs a, int nsweeps) 
     if (sc.tile >= sc.end) return;
     int gr0, gc0;
     tile_origin<TH, TW>(a, sc.tile, gr0, gc0);
-    fused_tile<TH, TW, ZLO, ZHI, PREC>(a, lds, gr0, gc0, nsweeps);
+    fused_tile<TH, TW, ZLO, ZHI, PREC>(a, a.cw, lds, gr0, gc0, nsweeps);
 }
 
 // ----------------------------------------------- MFMA kernels, 32x32 fragments
@@ -598,39 +606,30 @@ __host__ __device__ constexpr int jmap32(int mu) {
 }
 
 // Coefficient fragments: lane (mrow, h) = (l%32, l/32) holds A[mrow][8h + e] of
-// each MFMA, i.e. c[z][k - j] for output column j = jmap32(mrow) and the slot's (z, k).
+// each MFMA, i.e. c[z][k - j] for output column j = jmap32(mrow) and the slot's
+// (z, k).  A full group (k = 8q .. 8q+7) is the window at d0 = 8q - j of row z;
+// a half-group pair takes the first four taps of the window at d0 = 32 - j of
+// two rows (zeros for a row past the last shift).  All come from CoefWindows.
 template <int ZLO, int ZHI, int PREC>
 struct Coef32 {
     using F = Fold32<ZLO, ZHI>;
     typedef typename Frag32<PREC>::type frag_t;
     frag_t f[F::NMF];
 
-    __device__ __forceinline__ void build(const SweepArgs &a, int mrow, int h) {
+    static __device__ __forceinline__ int window(int d0) { return d0 >= -7 && d0 <= 4 ? d0 + 7 : 12; }
+
+    __device__ __forceinline__ void load(const CoefWindows &cw, int mrow, int h) {
         const int j = jmap32(mrow);
 #pragma unroll
         for (int mf = 0; mf < F::NMF; ++mf) {
-#pragma unroll
-            for (int e = 0; e < 8; ++e) {
-                int zi, k;
-                bool real = true;
-                if (mf < 2 * F::NZ) {
-                    zi = mf >> 1;
-                    k = 8 * (2 * (mf & 1) + h) + e;
-                } else {
-                    const int z = 2 * F::pair(mf, h) + (e >> 2);
-                    real = z < F::NZ;
-                    zi = real ? z : 0;
-                    k = 32 + (e & 3);
-                }
-                const int d = k - j;
-                float v = 0.0f;
-#pragma unroll
-                for (int dd = 0; dd < 5; ++dd)
-                    if (real && d == dd) v = a.cf[(ZLO + zi) * 5 + dd];
-                if constexpr (PREC == TCS_PREC_BF16)
-                    f[mf][e] = static_cast<__bf16>(v);
-                else
-                    f[mf][e] = static_cast<_Float16>(v);
+            if (mf < 2 * F::NZ) {
+                const uint32_t *w = cw.w[0][ZLO + (mf >> 1)][window(8 * (2 * (mf & 1) + h) - j)];
+                f[mf] = __builtin_bit_cast(frag_t, u32x4{w[0], w[1], w[2], w[3]});
+            } else {
+                const int z0 = 2 * F::pair(mf, h), jw = window(32 - j);
+                const uint32_t *w0 = cw.w[0][ZLO + (z0 < F::NZ ? z0 : 0)][z0 < F::NZ ? jw : 12];
+                const uint32_t *w1 = cw.w[0][ZLO + (z0 + 1 < F::NZ ? z0 + 1 : 0)][z0 + 1 < F::NZ ? jw : 12];
+                f[mf] = __builtin_bit_cast(frag_t, u32x4{w0[0], w0[1], w1[0], w1[1]});
             }
         }
     }
@@ -737,7 +736,8 @@ __device__ __forceinline__ void for_blocks32(const uint16_t *lds, const Coef32<Z
 }
 
 template <int TH, int TW, int ZLO, int ZHI, int PREC>
-__device__ __forceinline__ void sweep_tile32(const SweepArgs &a, uint16_t *lds, int gr0, int gc0) {
+__device__ __forceinline__ void sweep_tile32(const SweepArgs &a, const CoefWindows &cw, uint16_t *lds, int gr0,
+                                             int gc0) {
     using T = Tile<TH, TW, 1>;
     constexpr int BX = TW / 2, NBLK = (TH / 2) * BX;
     static_assert(TH % 2 == 0 && TW % 2 == 0 && NBLK % 4 == 0, "tile must split into 2x2-brick blocks over 4 waves");
@@ -750,7 +750,7 @@ __device__ __forceinline__ void sweep_tile32(const SweepArgs &a, uint16_t *lds, 
     const int lane = tid & 63, wave = tid >> 6;
     const int i = lane & 31, h = lane >> 5;
     Coef32<ZLO, ZHI, PREC> C;
-    C.build(a, i, h);
+    C.load(cw, i, h);
     __syncthreads();
     const bool edge = !tile_is_interior(a, gr0, gc0, 16 * TH, 16 * TW);
     for_blocks32<T, ZLO, ZHI, PREC, NBLK / 4, NBLK, BX, 0>(lds, C, wave, i, h, [&](int, int br, int bc, const f32x16 &acc) {
@@ -769,7 +769,7 @@ __global__ __launch_bounds__(kWG) void tcs_mfma32_sweep(SweepArgs a) {
     if (sc.tile >= sc.end) return;
     int gr0, gc0;
     tile_origin<TH, TW>(a, sc.tile, gr0, gc0);
-    sweep_tile32<TH, TW, ZLO, ZHI, PREC>(a, lds, gr0, gc0);
+    sweep_tile32<TH, TW, ZLO, ZHI, PREC>(a, a.cw, lds, gr0, gc0);
 }
 
 // Temporal blocking with 32x32 fragments: the scheme of tcs_mfma_fused (one-
@@ -778,7 +778,8 @@ __global__ __launch_bounds__(kWG) void tcs_mfma32_sweep(SweepArgs a) {
 // splits into blocks from brick (0, 0); the core of the last sweep into blocks
 // from brick (1, 1).
 template <int TH, int TW, int ZLO, int ZHI, int PREC>
-__device__ __forceinline__ void fused_tile32(const SweepArgs &a, uint16_t *lds, int gr0, int gc0, int nsweeps) {
+__device__ __forceinline__ void fused_tile32(const SweepArgs &a, const CoefWindows &cw, uint16_t *lds, int gr0,
+                                             int gc0, int nsweeps) {
     using G = FusedGeom<TH, TW, 1>;
     constexpr int BXA = G::BW / 2, NBA = (G::BH / 2) * BXA, QA = (NBA + 3) / 4;
     constexpr int BXC = TW / 2, NBC = (TH / 2) * BXC;
@@ -793,7 +794,7 @@ __device__ __forceinline__ void fused_tile32(const SweepArgs &a, uint16_t *lds, 
     const int lane = tid & 63, wave = tid >> 6;
     const int i = lane & 31, h = lane >> 5;
     Coef32<ZLO, ZHI, PREC> C;
-    C.build(a, i, h);
+    C.load(cw, i, h);
     const bool edge = !tile_is_interior(a, er0, ec0, 16 * G::BH, 16 * G::BW);
     __syncthreads();
 
@@ -833,7 +834,7 @@ __global__ __launch_bounds__(kWG) void tcs_mfma32_fused(SweepArgs a, int nsweeps
     if (sc.tile >= sc.end) return;
     int gr0, gc0;
     tile_origin<TH, TW>(a, sc.tile, gr0, gc0);
-    fused_tile32<TH, TW, ZLO, ZHI, PREC>(a, lds, gr0, gc0, nsweeps);
+    fused_tile32<TH, TW, ZLO, ZHI, PREC>(a, a.cw, lds, gr0, gc0, nsweeps);
 }
 
 // Batched uneven grids: one launch sweeps every tile of a list drawn from many
@@ -842,7 +843,8 @@ __global__ __launch_bounds__(kWG) void tcs_mfma32_fused(SweepArgs a, int nsweeps
 // so neighbouring tiles of one problem share an L2.  The radius selects one of
 // two instantiations of the same brick code with a workgroup-uniform branch.
 template <int TH, int TW, int ZLO, int ZHI, int PREC>
-__device__ __forceinline__ void batch_tile(const SweepArgs &a, uint16_t *lds, int gr0, int gc0) {
+__device__ __forceinline__ void batch_tile(const SweepArgs &a, const CoefWindows &cw, uint16_t *lds, int gr0,
+                                           int gc0) {
     constexpr int NPL = PREC == TCS_PREC_SPLIT ? 2 : 1;
     constexpr bool LOOKAHEAD = kLookahead<PREC>;
     using T = Tile<TH, TW, NPL>;
@@ -856,7 +858,7 @@ __device__ __forceinline__ void batch_tile(const SweepArgs &a, uint16_t *lds, in
     const int lane = tid & 63, wave = tid >> 6;
     const int m = lane & 15, g = lane >> 4;
     Coef<ZLO, ZHI, PREC> C;
-    C.build(a, m, g);
+    C.load(cw, m, g);
     __syncthreads();
     const bool edge = !tile_is_interior(a, gr0, gc0, 16 * TH, 16 * TW);
     constexpr int NB = TH * TW / 4;
@@ -918,36 +920,32 @@ __global__ __launch_bounds__(kWG) void tcs_mfma_batch(const BatchProblem *__rest
         a.sbr = P.sbr;
         a.sbc = P.sbc;
         a.tiles_x = P.tiles_x;
+        a.tiles_xp = P.tiles_x;  // the tile list holds real tile indices
+        a.nvirt = 0;
         a.rs = P.rs;
         a.out_scale = P.out_scale;
-#pragma unroll
-        for (int i = 0; i < 25; ++i) {
-            a.cf[i] = P.cf[i];
-            a.ch[i] = P.ch[i];
-            a.cl[i] = P.cl[i];
-        }
         int gr0, gc0;
         tile_origin<TH, TW>(a, bt.tile, gr0, gc0);
         if constexpr (FUSED && FRAG == 32) {
             if (a.rs == 2)
-                fused_tile32<TH, TW, 0, 4, PREC>(a, lds, gr0, gc0, nsweeps);
+                fused_tile32<TH, TW, 0, 4, PREC>(a, P.cw, lds, gr0, gc0, nsweeps);
             else
-                fused_tile32<TH, TW, 1, 3, PREC>(a, lds, gr0, gc0, nsweeps);
+                fused_tile32<TH, TW, 1, 3, PREC>(a, P.cw, lds, gr0, gc0, nsweeps);
         } else if constexpr (FUSED) {
             if (a.rs == 2)
-                fused_tile<TH, TW, 0, 4, PREC>(a, lds, gr0, gc0, nsweeps);
+                fused_tile<TH, TW, 0, 4, PREC>(a, P.cw, lds, gr0, gc0, nsweeps);
             else
-                fused_tile<TH, TW, 1, 3, PREC>(a, lds, gr0, gc0, nsweeps);
+                fused_tile<TH, TW, 1, 3, PREC>(a, P.cw, lds, gr0, gc0, nsweeps);
         } else if constexpr (FRAG == 32) {
             if (a.rs == 2)
-                sweep_tile32<TH, TW, 0, 4, PREC>(a, lds, gr0, gc0);
+                sweep_tile32<TH, TW, 0, 4, PREC>(a, P.cw, lds, gr0, gc0);
             else
-                sweep_tile32<TH, TW, 1, 3, PREC>(a, lds, gr0, gc0);
+                sweep_tile32<TH, TW, 1, 3, PREC>(a, P.cw, lds, gr0, gc0);
         } else {
             if (a.rs == 2)
-                batch_tile<TH, TW, 0, 4, PREC>(a, lds, gr0, gc0);
+                batch_tile<TH, TW, 0, 4, PREC>(a, P.cw, lds, gr0, gc0);
             else
-                batch_tile<TH, TW, 1, 3, PREC>(a, lds, gr0, gc0);
+                batch_tile<TH, TW, 1, 3, PREC>(a, P.cw, lds, gr0, gc0);
         }
     }
 }
@@ -986,7 +984,7 @@ __global__ __launch_bounds__(kWG) void tcs_direct_sweep(SweepArgs a) {
     write_tile<T>(R, lds, tid);
     __syncthreads();
     const int cgr0 = gr0, cgc0 = gc0;  // the tile in LDS
-    const int next = sc.tile + sc.step;
+    const int next = skip_empty(a, sc.tile + sc.step, sc.step);
     if (PERSIST && next < sc.end) {
         tile_origin<TH, TW>(a, next, gr0, gc0);
         load_tile<T, false>(a, R, gr0, gc0, tid);  // in flight during the compute below
@@ -1235,15 +1233,12 @@ static int make_args(const tcs_problem *p, const void *d_in, void *d_out, SweepA
     a.sbr = L.sbr;
     a.sbc = L.sbc;
     a.tiles_x = L.nbc / p->tile_w;
-    a.ntiles = (L.nbr / p->tile_h) * a.tiles_x;
-    a.tiles_per_xcd = (a.ntiles + kXCD - 1) / kXCD;
+    a.tiles_xp = (a.tiles_x + kXCD - 1) / kXCD * kXCD;
+    a.nvirt = (L.nbr / p->tile_h) * a.tiles_xp;
     a.rs = p->radius;
     a.out_scale = ldexpf(1.0f, -p->coeff_exp);
-    for (int i = 0; i < 25; ++i) {
-        a.cf[i] = p->coeff[i];
-        a.ch[i] = p->coeff_hi[i];
-        a.cl[i] = p->coeff_lo[i];
-    }
+    for (int i = 0; i < 25; ++i) a.cf[i] = p->coeff[i];
+    build_coef_windows(p, &a.cw);
     return TCS_OK;
 }
 
@@ -1258,7 +1253,7 @@ int launch_fused(const tcs_problem *p, const void *d_in, void *d_out, int nsweep
     SweepArgs a;
     int st = make_args(p, d_in, d_out, a);
     if (st) return st;
-    hipLaunchKernelGGL(k->fn, dim3((unsigned)(a.tiles_per_xcd * kXCD)), dim3(kWG), 0, stream, a, nsweeps);
+    hipLaunchKernelGGL(k->fn, dim3((unsigned)a.nvirt), dim3(kWG), 0, stream, a, nsweeps);
     TCS_HIP_CHECK(hipGetLastError());
     return TCS_OK;
 }
@@ -1271,18 +1266,17 @@ int launch_sweep(const tcs_problem *p, const void *d_in, void *d_out, hipStream_
     SweepArgs a;
     int st = make_args(p, d_in, d_out, a);
     if (st) return st;
-    // One workgroup per tile, or (wg_per_cu > 0) a persistent grid of wg_per_cu
-    // workgroups per CU; either way a multiple of 8 so every XCD gets the same.
-    int per_xcd = a.tiles_per_xcd;
+    // One workgroup per virtual tile, or (wg_per_cu > 0) a persistent grid of
+    // wg_per_cu workgroups per CU; either way a multiple of 8.
+    unsigned grid = (unsigned)a.nvirt;
     if (k->persist) {
         int dev = 0, cus = 0;
         TCS_HIP_CHECK(hipGetDevice(&dev));
         TCS_HIP_CHECK(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
-        per_xcd = cus * p->wg_per_cu / kXCD;
+        int per_xcd = cus * p->wg_per_cu / kXCD;
         if (per_xcd < 1) per_xcd = 1;
-        if (per_xcd > a.tiles_per_xcd) per_xcd = a.tiles_per_xcd;
+        if (per_xcd * kXCD < a.nvirt) grid = (unsigned)(per_xcd * kXCD);
     }
-    const unsigned grid = (unsigned)(per_xcd * kXCD);
     hipLaunchKernelGGL(k->fn, dim3(grid), dim3(kWG), 0, stream, a);
     TCS_HIP_CHECK(hipGetLastError());
     return TCS_OK;

@@ -170,24 +170,28 @@ def stage_tile(inp: np.ndarray, L: Layout, G: TileGeom, gr0: int, gc0: int, chec
 
 
 def _tiles(L: Layout, TH: int, TW: int, per_xcd: int = 0):
-    """The persistent schedule of tcs_sweep.hip: grid = 8 * per_xcd workgroups;
-    workgroup w on XCD x = w % 8 walks tiles x*tpx + w//8, + per_xcd, ... below
-    min((x+1)*tpx, ntiles).  per_xcd = 0 means one workgroup per tile."""
+    """The tile schedule of tcs_sweep.hip: tile rows padded to tiles_xp (a
+    multiple of 8) virtual columns, virtual tiles with tx >= tiles_x empty;
+    grid = nvirt workgroups (per_xcd = 0: one per virtual tile) or 8 * per_xcd
+    (persistent); workgroup w walks v = w, w + grid, ... below nvirt, skipping
+    empty tiles.  Workgroup w runs on XCD w % 8, so tile column tx stays on XCD
+    tx % 8."""
     tiles_x = L.nbc // TW
-    ntiles = (L.nbr // TH) * tiles_x
-    tpx = -(-ntiles // XCD)
-    g = tpx if per_xcd <= 0 else min(per_xcd, tpx)
+    tiles_xp = -(-tiles_x // XCD) * XCD
+    nvirt = (L.nbr // TH) * tiles_xp
+    grid = nvirt if per_xcd <= 0 or per_xcd * XCD >= nvirt else per_xcd * XCD
+    assert grid % XCD == 0
     seen = []
-    for wg in range(g * XCD):
-        x, k = wg % XCD, wg // XCD
-        end = min(x * tpx + tpx, ntiles)
-        tile = x * tpx + k
-        while tile < end:
-            seen.append(tile)
-            ty, tx = tile // tiles_x, tile % tiles_x
-            yield 16 * TH * ty, 16 * TW * tx
-            tile += g
-    assert sorted(seen) == list(range(ntiles)), "schedule does not cover every tile exactly once"
+    for wg in range(grid):
+        v = wg
+        while v < nvirt:
+            ty, tx = divmod(v, tiles_xp)
+            if tx < tiles_x:
+                assert tx % XCD == wg % XCD, "a tile column left its XCD"
+                seen.append(ty * tiles_x + tx)
+                yield 16 * TH * ty, 16 * TW * tx
+            v += grid
+    assert sorted(seen) == list(range((L.nbr // TH) * tiles_x)), "schedule does not cover every tile exactly once"
 
 
 def _interior(gr, gc, L: Layout, rs: int):

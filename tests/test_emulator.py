@@ -84,7 +84,7 @@ def test_storage_index_is_the_reference_brick_map():
     assert np.array_equal(mine, ref_out)
 
 
-@pytest.mark.parametrize("ntile_shape", [(9, 4, 2, 4), (64, 64, 2, 4), (3, 200, 1, 8)])
+@pytest.mark.parametrize("ntile_shape", [(9, 4, 2, 4), (64, 64, 2, 4), (3, 200, 1, 8), (4, 1030, 2, 4)])
 def test_persistent_schedule_covers_every_tile(ntile_shape):
     rows_b, cols_b, TH, TW = ntile_shape
     L = E.Layout(16 * rows_b, 16 * cols_b, max(TH, TW))
