@@ -141,8 +141,9 @@ def c2():
         rec["errors_vs_fp64"] = _errors(p, x, refs)
         res[method] = rec["gstencil_s"]
         emit(config="c2", **rec)
-    p, rec = _measure(stencil, N, N, B, x, "mfma", "fp16", 100, fuse=8)
-    emit(config="c2", **rec)
+    for fuse in (8, 16):
+        p, rec = _measure(stencil, N, N, B, x, "mfma", "fp16", 100, fuse=fuse)
+        emit(config="c2", **rec)
     emit(config="c2", mfma_vs_direct=round(res["mfma"] / res["direct"], 3))
 
 

@@ -239,27 +239,58 @@ struct Frag<TCS_PREC_BF16> {
 template <>
 struct Frag<TCS_PREC_SPLIT> : Frag<TCS_PREC_FP16> {};
 
-// Coefficient fragments.  Lane l holds A[m][k] for m = l%16 and
-// k = 8*(l/16) + e, e = 0..7: the coefficient at row offset z and column
-// offset k - m when 0 <= k - m <= 4, else 0 -- the window at d0 = 8g - m of
-// the zero-padded tap row, loaded whole from the host-built window table
+// K slots of the 16x16x32 MFMAs of one brick.  K slot group g (k = 8g..8g+7)
+// of MFMA f multiplies haloed row m + ZLO + zi, columns 8kg..8kg+7 (operand
+// B, lane (m, g)) by the taps c[ZLO + zi][8kg + e - j] (operand A, lane
+// (j, g)).  Radius 1 (3 row shifts): one MFMA per shift, kg = g, and group 3
+// (columns 24..31) meets zero taps.  Radius 2 (5 row shifts, 20 haloed
+// columns): the 15 (shift, column group) pairs are packed densely, pair
+// q = 4f + g -> (zi, kg) = (q / 3, q % 3), into 4 MFMAs instead of 5 (20% fewer
+// MFMAs and LDS operand reads); the 16th group repeats pair 14's columns
+// against zero taps.  Unused groups read finite data of the brick's window.
+template <int ZLO, int ZHI, bool PACK = true>
+struct KMap {
+    static constexpr int NZ = ZHI - ZLO + 1;
+    static constexpr bool PACKED = PACK && NZ == 5;
+    static constexpr int NF = PACKED ? 4 : NZ;  // MFMAs (operand fragments) per brick and plane
+    static __device__ __forceinline__ void group(int f, int g, int &zi, int &kg, bool &real) {
+        if constexpr (PACKED) {
+            int q = 4 * f + g;
+            real = q < 15;
+            if (!real) q = 14;
+            zi = q / 3;
+            kg = q - 3 * zi;
+        } else {
+            zi = f;
+            kg = g < 2 ? g : 2;
+            real = g < 3;
+        }
+    }
+};
+
+// Coefficient fragments: lane (j, g) of MFMA f holds c[ZLO + zi][8kg + e - j],
+// e = 0..7 (zero outside the taps) -- the window at d0 = 8kg - j of the
+// zero-padded tap row, loaded whole from the host-built window table
 // (CoefWindows; window 12 is all zeros).
-template <int ZLO, int ZHI, int PREC>
+template <int ZLO, int ZHI, int PREC, bool PACK = true>
 struct Coef {
     typedef typename Frag<PREC>::type frag_t;
-    static constexpr int NZ = ZHI - ZLO + 1;
-    frag_t hi[NZ], lo[NZ];
+    using K = KMap<ZLO, ZHI, PACK>;
+    frag_t hi[K::NF], lo[K::NF];
 
     __device__ __forceinline__ void load(const CoefWindows &cw, int m, int g) {
-        const int d0 = 8 * g - m;
-        const int j = d0 >= -7 && d0 <= 4 ? d0 + 7 : 12;
 #pragma unroll
-        for (int zi = 0; zi < NZ; ++zi) {
+        for (int f = 0; f < K::NF; ++f) {
+            int zi, kg;
+            bool real;
+            K::group(f, g, zi, kg, real);
+            const int d0 = 8 * kg - m;
+            const int j = real && d0 >= -7 && d0 <= 4 ? d0 + 7 : 12;
             const uint32_t *wh = cw.w[0][ZLO + zi][j];
-            hi[zi] = __builtin_bit_cast(frag_t, u32x4{wh[0], wh[1], wh[2], wh[3]});
+            hi[f] = __builtin_bit_cast(frag_t, u32x4{wh[0], wh[1], wh[2], wh[3]});
             if constexpr (PREC == TCS_PREC_SPLIT) {
                 const uint32_t *wl = cw.w[1][ZLO + zi][j];
-                lo[zi] = __builtin_bit_cast(frag_t, u32x4{wl[0], wl[1], wl[2], wl[3]});
+                lo[f] = __builtin_bit_cast(frag_t, u32x4{wl[0], wl[1], wl[2], wl[3]});
             }
         }
     }
@@ -267,22 +298,37 @@ struct Coef {
 
 // Operands of one 16x16 output brick: brick (br, bc) of the LDS image of
 // geometry G, whose haloed 20x20 window starts at LDS row 16*br, column 16*bc.
-// Row shift z: lane (m, g) holds haloed row m + z, columns 8g..8g+7.  Slot
-// group 3 (k = 24..31) only meets zero coefficients; it re-reads group 2's
-// (finite) columns so that every lane reads and the reads issue back to back.
-template <class G, int ZLO, int ZHI, int PREC>
+// Fragment f: lane (m, g) holds haloed row m + ZLO + zi, columns 8kg..8kg+7
+// (KMap), one 16-byte LDS read; all reads of a brick issue back to back.
+template <class G, int ZLO, int ZHI, int PREC, bool PACK = true>
 struct Ops {
-    static constexpr int NZ = ZHI - ZLO + 1;
-    u32x4 h[NZ];
-    u32x4 l[PREC == TCS_PREC_SPLIT ? NZ : 1];
+    using K = KMap<ZLO, ZHI, PACK>;
+    u32x4 h[K::NF];
+    u32x4 l[PREC == TCS_PREC_SPLIT ? K::NF : 1];
 
-    __device__ __forceinline__ void load(const uint16_t *lds, int br, int bc, int m, int g) {
-        const int gs = g < 2 ? g : 2;
+    // Per-lane element offsets of the fragments within a brick's window.
+    struct Offsets {
+        int o[K::NF];
+        __device__ __forceinline__ Offsets(int m, int g) {
 #pragma unroll
-        for (int zi = 0; zi < NZ; ++zi) {
-            const int off = (16 * br + m + ZLO + zi) * G::LP + 16 * bc + 8 * gs;
-            h[zi] = *reinterpret_cast<const u32x4 *>(lds + off);
-            if constexpr (PREC == TCS_PREC_SPLIT) l[zi] = *reinterpret_cast<const u32x4 *>(lds + G::PLANE + off);
+            for (int f = 0; f < K::NF; ++f) {
+                int zi, kg;
+                bool real;
+                K::group(f, g, zi, kg, real);
+                o[f] = (m + ZLO + zi) * G::LP + 8 * kg;
+            }
+        }
+    };
+
+    // Unpacked fragments differ by whole LDS rows (compile-time offsets from
+    // fragment 0's address); packed ones need the per-lane offsets.
+    __device__ __forceinline__ void load(const uint16_t *lds, int br, int bc, const Offsets &off) {
+        const uint16_t *base = lds + 16 * br * G::LP + 16 * bc;
+#pragma unroll
+        for (int f = 0; f < K::NF; ++f) {
+            const int o = K::PACKED ? off.o[f] : off.o[0] + f * G::LP;
+            h[f] = *reinterpret_cast<const u32x4 *>(base + o);
+            if constexpr (PREC == TCS_PREC_SPLIT) l[f] = *reinterpret_cast<const u32x4 *>(base + G::PLANE + o);
         }
     }
 };
@@ -291,24 +337,25 @@ struct Ops {
 // 4g..4g+3 in storage format (w[p]: four 16-bit values of plane p).  gr, gcb:
 // global row and first column of those outputs; with `edge`, points outside
 // the interior keep their input value (read from the LDS image).
-template <class G, int ZLO, int ZHI, int PREC>
-__device__ __forceinline__ void mfma_brick(const SweepArgs &a, const uint16_t *lds, const Coef<ZLO, ZHI, PREC> &C,
-                                           const Ops<G, ZLO, ZHI, PREC> &op, int br, int bc, int m, int g,
+template <class G, int ZLO, int ZHI, int PREC, bool PACK>
+__device__ __forceinline__ void mfma_brick(const SweepArgs &a, const uint16_t *lds,
+                                           const Coef<ZLO, ZHI, PREC, PACK> &C,
+                                           const Ops<G, ZLO, ZHI, PREC, PACK> &op, int br, int bc, int m, int g,
                                            bool edge, int gr, int gcb, u32x2 (&w)[G::NPL]) {
     constexpr int NPL = G::NPL;
-    constexpr int NZ = ZHI - ZLO + 1;
+    constexpr int NF = KMap<ZLO, ZHI, PACK>::NF;
     using F = Frag<PREC>;
     typedef typename F::type frag_t;
     f32x4 acc = {0.0f, 0.0f, 0.0f, 0.0f};
 #pragma unroll
-    for (int zi = 0; zi < NZ; ++zi) {
-        const frag_t bh = __builtin_bit_cast(frag_t, op.h[zi]);
-        acc = F::mma(C.hi[zi], bh, acc);
+    for (int f = 0; f < NF; ++f) {
+        const frag_t bh = __builtin_bit_cast(frag_t, op.h[f]);
+        acc = F::mma(C.hi[f], bh, acc);
         if constexpr (PREC == TCS_PREC_SPLIT) {
-            const frag_t bl = __builtin_bit_cast(frag_t, op.l[zi]);
-            acc = F::mma(C.hi[zi], bl, acc);
-            acc = F::mma(C.lo[zi], bh, acc);
-            acc = F::mma(C.lo[zi], bl, acc);
+            const frag_t bl = __builtin_bit_cast(frag_t, op.l[f]);
+            acc = F::mma(C.hi[f], bl, acc);
+            acc = F::mma(C.lo[f], bh, acc);
+            acc = F::mma(C.lo[f], bl, acc);
         }
     }
     uint16_t o[NPL][4];
@@ -382,6 +429,7 @@ void tcs_mfma_sweep(SweepArgs a) {
     const int m = lane & 15, g = lane >> 4;
     Coef<ZLO, ZHI, PREC> C;
     C.load(a.cw, m, g);
+    const typename Ops<T, ZLO, ZHI, PREC>::Offsets off(m, g);
     constexpr int NB = TH * TW / 4;
 
     for (;;) {
@@ -399,7 +447,7 @@ void tcs_mfma_sweep(SweepArgs a) {
         if constexpr (LOOKAHEAD) {
             int br, bc;
             brick_of<TW, 0>(wave, br, bc);
-            cur.load(lds, br, bc, m, g);
+            cur.load(lds, br, bc, off);
         }
 #pragma unroll
         for (int q = 0; q < NB; ++q) {
@@ -409,10 +457,10 @@ void tcs_mfma_sweep(SweepArgs a) {
                 if (q + 1 < NB) {
                     int nbr, nbc;
                     brick_of<TW, 0>(wave + 4 * (q + 1), nbr, nbc);
-                    nxt.load(lds, nbr, nbc, m, g);
+                    nxt.load(lds, nbr, nbc, off);
                 }
             } else {
-                cur.load(lds, br, bc, m, g);
+                cur.load(lds, br, bc, off);
             }
             __builtin_amdgcn_sched_barrier(0);
             const int gr = cgr0 + 16 * br + m, gcb = cgc0 + 16 * bc + 4 * g;
@@ -447,8 +495,13 @@ template <int TH, int TW, int ZLO, int ZHI, int PREC>
 __device__ __forceinline__ void fused_tile(const SweepArgs &a, const CoefWindows &cw, uint16_t *lds, int gr0,
                                            int gc0, int nsweeps) {
     constexpr int NPL = PREC == TCS_PREC_SPLIT ? 2 : 1;
-    constexpr bool LOOKAHEAD = kLookahead<PREC>;
+    constexpr bool LOOKAHEAD = false;  // registers go to occupancy instead (see tcs_mfma_fused)
     using G = FusedGeom<TH, TW, NPL>;
+    // Packed K slots (radius 2) pay off for SPLIT's four MFMA passes, with the
+    // opaque per-sweep offset below holding register use down; FP16/BF16 run
+    // faster unpacked and with the compiler's hoisted addresses (box2d2r
+    // 16384^2, 4x8, 8 fused sweeps: SPLIT 394 -> 349 us, FP16 134 vs 148 us).
+    constexpr bool PACK = PREC == TCS_PREC_SPLIT;
     constexpr int QA = G::BH * G::BW / 4;  // bricks per wave, whole image
     constexpr int QC = TH * TW / 4;        // bricks per wave, core only
     static_assert(G::BH * G::BW % 4 == 0 && TH * TW % 4 == 0, "bricks must split over 4 waves");
@@ -462,19 +515,26 @@ __device__ __forceinline__ void fused_tile(const SweepArgs &a, const CoefWindows
     }
     const int lane = tid & 63, wave = tid >> 6;
     const int m = lane & 15, g = lane >> 4;
-    Coef<ZLO, ZHI, PREC> C;
+    Coef<ZLO, ZHI, PREC, PACK> C;
     C.load(cw, m, g);
+    const typename Ops<G, ZLO, ZHI, PREC, PACK>::Offsets off(m, g);
     const bool edge = !tile_is_interior(a, er0, ec0, 16 * G::BH, 16 * G::BW);
     __syncthreads();
 
-    uint32_t *l32 = reinterpret_cast<uint32_t *>(lds);
     for (int sw = 0; sw + 1 < nsweeps; ++sw) {
+        // Packed: an opaque zero offset per sweep keeps the compiler from
+        // hoisting every brick's operand and write-back addresses out of the
+        // sweep loop (about 4 registers per brick), which would cost occupancy.
+        int opaque = 0;
+        if constexpr (PACK) asm volatile("" : "+s"(opaque));
+        uint16_t *ls = lds + opaque;
+        uint32_t *l32 = reinterpret_cast<uint32_t *>(ls);
         u32x2 res[QA][NPL];
-        Ops<G, ZLO, ZHI, PREC> cur, nxt;
+        Ops<G, ZLO, ZHI, PREC, PACK> cur, nxt;
         if constexpr (LOOKAHEAD) {
             int br, bc;
             brick_of<G::BW, 0>(wave, br, bc);
-            cur.load(lds, br, bc, m, g);
+            cur.load(ls, br, bc, off);
         }
 #pragma unroll
         for (int q = 0; q < QA; ++q) {
@@ -484,13 +544,13 @@ __device__ __forceinline__ void fused_tile(const SweepArgs &a, const CoefWindows
                 if (q + 1 < QA) {
                     int nbr, nbc;
                     brick_of<G::BW, 0>(wave + 4 * (q + 1), nbr, nbc);
-                    nxt.load(lds, nbr, nbc, m, g);
+                    nxt.load(ls, nbr, nbc, off);
                 }
             } else {
-                cur.load(lds, br, bc, m, g);
+                cur.load(ls, br, bc, off);
             }
             __builtin_amdgcn_sched_barrier(0);
-            mfma_brick<G, ZLO, ZHI, PREC>(a, lds, C, cur, br, bc, m, g, edge, er0 + 16 * br + m,
+            mfma_brick<G, ZLO, ZHI, PREC>(a, ls, C, cur, br, bc, m, g, edge, er0 + 16 * br + m,
                                           ec0 + 16 * bc + 4 * g, res[q]);
             if constexpr (LOOKAHEAD) cur = nxt;
         }
@@ -508,11 +568,11 @@ __device__ __forceinline__ void fused_tile(const SweepArgs &a, const CoefWindows
         }
         __syncthreads();
     }
-    Ops<G, ZLO, ZHI, PREC> cur, nxt;
+    Ops<G, ZLO, ZHI, PREC, PACK> cur, nxt;
     if constexpr (LOOKAHEAD) {
         int br, bc;
         brick_of<TW, 1>(wave, br, bc);
-        cur.load(lds, br, bc, m, g);
+        cur.load(lds, br, bc, off);
     }
 #pragma unroll
     for (int q = 0; q < QC; ++q) {
@@ -522,10 +582,10 @@ __device__ __forceinline__ void fused_tile(const SweepArgs &a, const CoefWindows
             if (q + 1 < QC) {
                 int nbr, nbc;
                 brick_of<TW, 1>(wave + 4 * (q + 1), nbr, nbc);
-                nxt.load(lds, nbr, nbc, m, g);
+                nxt.load(lds, nbr, nbc, off);
             }
         } else {
-            cur.load(lds, br, bc, m, g);
+            cur.load(lds, br, bc, off);
         }
         __builtin_amdgcn_sched_barrier(0);
         const int gr = er0 + 16 * br + m, gcb = ec0 + 16 * bc + 4 * g;
@@ -538,8 +598,14 @@ __device__ __forceinline__ void fused_tile(const SweepArgs &a, const CoefWindows
     }
 }
 
+// Fused sweeps re-read every brick's operands from LDS each sweep (ring
+// included) and run near the LDS read bandwidth; without the operand
+// look-ahead FP16/BF16 kernels fit 4 waves per SIMD, which LDS allows for
+// tiles up to 4x8 (box2d2r 16384^2, 4x8 tile, 8 sweeps fused: 142 -> 134 us
+// per sweep).
 template <int TH, int TW, int ZLO, int ZHI, int PREC>
-__global__ __launch_bounds__(kWG) void tcs_mfma_fused(SweepArgs a, int nsweeps) {
+__global__ __launch_bounds__(kWG) __attribute__((amdgpu_waves_per_eu(PREC == TCS_PREC_SPLIT ? 1 : 4, 8)))
+void tcs_mfma_fused(SweepArgs a, int nsweeps) {
     constexpr int NPL = PREC == TCS_PREC_SPLIT ? 2 : 1;
     __shared__ __attribute__((aligned(16))) uint16_t lds[NPL * FusedGeom<TH, TW, NPL>::PLANE];
     const Schedule sc = schedule(a);
@@ -846,7 +912,7 @@ template <int TH, int TW, int ZLO, int ZHI, int PREC>
 __device__ __forceinline__ void batch_tile(const SweepArgs &a, const CoefWindows &cw, uint16_t *lds, int gr0,
                                            int gc0) {
     constexpr int NPL = PREC == TCS_PREC_SPLIT ? 2 : 1;
-    constexpr bool LOOKAHEAD = kLookahead<PREC>;
+    constexpr bool LOOKAHEAD = kLookahead<PREC> && !kHighOccupancy<TH, TW, PREC, false>;
     using T = Tile<TH, TW, NPL>;
     const int tid = threadIdx.x;
     {
@@ -859,6 +925,7 @@ __device__ __forceinline__ void batch_tile(const SweepArgs &a, const CoefWindows
     const int m = lane & 15, g = lane >> 4;
     Coef<ZLO, ZHI, PREC> C;
     C.load(cw, m, g);
+    const typename Ops<T, ZLO, ZHI, PREC>::Offsets off(m, g);
     __syncthreads();
     const bool edge = !tile_is_interior(a, gr0, gc0, 16 * TH, 16 * TW);
     constexpr int NB = TH * TW / 4;
@@ -866,7 +933,7 @@ __device__ __forceinline__ void batch_tile(const SweepArgs &a, const CoefWindows
     if constexpr (LOOKAHEAD) {
         int br, bc;
         brick_of<TW, 0>(wave, br, bc);
-        cur.load(lds, br, bc, m, g);
+        cur.load(lds, br, bc, off);
     }
 #pragma unroll
     for (int q = 0; q < NB; ++q) {
@@ -876,10 +943,10 @@ __device__ __forceinline__ void batch_tile(const SweepArgs &a, const CoefWindows
             if (q + 1 < NB) {
                 int nbr, nbc;
                 brick_of<TW, 0>(wave + 4 * (q + 1), nbr, nbc);
-                nxt.load(lds, nbr, nbc, m, g);
+                nxt.load(lds, nbr, nbc, off);
             }
         } else {
-            cur.load(lds, br, bc, m, g);
+            cur.load(lds, br, bc, off);
         }
         __builtin_amdgcn_sched_barrier(0);
         const int gr = gr0 + 16 * br + m, gcb = gc0 + 16 * bc + 4 * g;
@@ -1209,9 +1276,11 @@ const char *sweep_kernel_name(const tcs_problem *p) {
 
 int mfma_per_brick(const tcs_problem *p) {
     if (p->method != TCS_METHOD_MFMA) return 0;
-    const int nz = 2 * p->radius + 1;
     if (frag_of(p) == 32) return p->radius == 2 ? Fold32<0, 4>::NMF : Fold32<1, 3>::NMF;  // per 2x2-brick block
-    return p->precision == TCS_PREC_SPLIT ? 4 * nz : nz;
+    // Single sweeps pack radius-2 K slots (4 MFMAs per brick and plane pair);
+    // fused FP16/BF16 sweeps do not (5).
+    const int nf = p->radius == 2 ? KMap<0, 4>::NF : KMap<1, 3>::NF;
+    return p->precision == TCS_PREC_SPLIT ? 4 * nf : nf;
 }
 
 int64_t mfma_flops_per_brick(const tcs_problem *p) {

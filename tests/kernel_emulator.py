@@ -199,34 +199,55 @@ def _interior(gr, gc, L: Layout, rs: int):
 
 
 # ------------------------------------------------------------- MFMA sweep
+def _kmap(rs, f, g):
+    """KMap of tcs_sweep.hip: (row shift index, column group, real) of K slot
+    group g (array of lane groups) of MFMA f.  Radius 2 packs the 15 (shift,
+    column group) pairs into 4 MFMAs, q = 4f + g -> (q // 3, q % 3); pair 15
+    repeats pair 14 against zero taps.  Radius 1: one MFMA per shift, group 3
+    repeats group 2 against zero taps."""
+    g = np.asarray(g)
+    if rs == 2:
+        q = 4 * f + g
+        real = q < 15
+        q = np.where(real, q, 14)
+        zi = q // 3
+        return zi, q - 3 * zi, real
+    return np.full_like(g, f), np.minimum(g, 2), g < 3
+
+
+def _nfrag(rs):
+    return 4 if rs == 2 else 3
+
+
 def _afrag(coeff5, rs, round_coeff=f16):
     ZLO, ZHI = (0, 4) if rs == 2 else (1, 3)
     lane = np.arange(64)
     m, g = lane & 15, lane >> 4
-    afrag = np.zeros((ZHI - ZLO + 1, 64, 8))
-    for zi in range(ZHI - ZLO + 1):
+    afrag = np.zeros((_nfrag(rs), 64, 8))
+    for f in range(_nfrag(rs)):
+        zi, kg, real = _kmap(rs, f, g)
         for e in range(8):
-            d = 8 * g + e - m
-            ok = (d >= 0) & (d <= 4)
-            afrag[zi, ok, e] = round_coeff(coeff5[ZLO + zi, d[ok]])
+            d = 8 * kg + e - m
+            ok = real & (d >= 0) & (d <= 4)
+            afrag[f, ok, e] = round_coeff(coeff5[ZLO + zi[ok], d[ok]])
     return afrag, ZLO, ZHI
 
 
 def _mfma_brick(lds, G: TileGeom, afrag, ZLO, ZHI, br, bc, edge, gr, gcb, L: Layout, rs):
     """Ops::load + mfma_brick: returns o[lane][v] (FP16 values) for LDS brick
-    (br, bc).  Every lane reads; slot group 3 re-reads group 2's columns."""
+    (br, bc).  Every lane reads 16 bytes per fragment (KMap)."""
     lane = np.arange(64)
     m, g = lane & 15, lane >> 4
     D = np.zeros((16, 16))
-    for zi in range(ZHI - ZLO + 1):
-        gs = np.minimum(g, 2)
-        off = (16 * br + m + ZLO + zi) * G.LP + 16 * bc + 8 * gs
+    for f in range(_nfrag(rs)):
+        zi, kg, _ = _kmap(rs, f, g)
+        off = (16 * br + m + ZLO + zi) * G.LP + 16 * bc + 8 * kg
         _check(off, G.PLANE - 7, 8, "B operand LDS read (16 B)")
         bfrag = lds[off[:, None] + np.arange(8)[None, :]]  # [lane][e]
         Am = np.zeros((16, 32))
         Bm = np.zeros((32, 16))
         for l in range(64):
-            Am[l % 16, 8 * (l // 16):8 * (l // 16) + 8] = afrag[zi, l]
+            Am[l % 16, 8 * (l // 16):8 * (l // 16) + 8] = afrag[f, l]
             Bm[8 * (l // 16):8 * (l // 16) + 8, l % 16] = bfrag[l]
         D += Am @ Bm
     # lane l holds D[4g+v][n], n = l % 16: output row n, column 4g+v
