@@ -59,6 +59,87 @@ def best_config(stencil, rows, cols, method="mfma", precision="fp16", **kw):
     return tune(stencil, rows, cols, method, precision, **kw)[0][0]
 
 
+# ------------------------------------------------------------ batched grids
+# Size buckets of a batch, by the smaller brick dimension of a grid.
+BUCKETS = ((0, 16), (16, 64), (64, 1 << 30))
+
+
+def bucket_of(rows, cols):
+    m = min((rows + 15) // 16, (cols + 15) // 16)
+    return next(i for i, (lo, hi) in enumerate(BUCKETS) if lo <= m < hi)
+
+
+def max_fuse(stencil):
+    r = 1 if stencil in ("star2d1r", "box2d1r") else 2
+    return 1 + 16 // r
+
+
+def build_batch(specs, choice, precision="fp16"):
+    """Problems, buffers and tcs_batch for specs = [(stencil, rows, cols, B, x)],
+    choice(i) -> (tile, frag, fuse, wg_per_cu); x (device, row-major) is packed
+    into each grid's first buffer."""
+    from . import api as T
+    probs, b0s, b1s = [], [], []
+    for i, (stencil, rows, cols, B, x) in enumerate(specs):
+        tile, frag, fuse, wg = choice(i)
+        p = T.Problem(stencil, rows, cols, B, precision=precision, tile=tile, frag=frag,
+                      fuse=min(fuse, max_fuse(stencil)), wg_per_cu=wg)
+        b0, b1 = p.alloc(), p.alloc()
+        p.clear(b1)
+        p.pack(x, b0)
+        probs.append(p), b0s.append(b0), b1s.append(b1)
+    return probs, b0s, b1s, T.Batch(probs, b0s, b1s)
+
+
+def time_batch(batch, sweeps, reps=3):
+    """ms for `sweeps` sweeps of the batch, replayed as one hipGraph."""
+    import torch
+    g = batch.graph(sweeps)
+    g.launch()
+    torch.cuda.synchronize()
+    ev = [torch.cuda.Event(enable_timing=True) for _ in range(2)]
+    ev[0].record()
+    for _ in range(reps):
+        g.launch()
+    ev[1].record()
+    torch.cuda.synchronize()
+    g.destroy()
+    return ev[0].elapsed_time(ev[1]) / reps
+
+
+def tune_batch(specs, sweeps=20, tiles=((2, 4), (4, 4), (4, 8)), frags=(16, 32), wgs=(0, 2),
+               fuses=(1, 4, 8, 16)):
+    """Per size bucket (BUCKETS), the fastest (tile, fragment, wg_per_cu, sweeps per
+    launch) for a batch of grids: each bucket's grids alone are timed as a batch
+    for every tile/fragment/waves-per-CU candidate at one sweep per launch, then
+    for every sweeps-per-launch count and tile with the winning fragment and
+    waves per CU.  Returns
+    ({bucket: (tile, frag, fuse, wg)}, {bucket: [(config, ms)]})."""
+    best, log = {}, {}
+    for b in range(len(BUCKETS)):
+        sub = [s for s in specs if bucket_of(s[1], s[2]) == b]
+        if not sub:
+            continue
+        trials = []
+
+        def trial(tile, frag, fuse, wg):
+            if frag == 32 and tile == (2, 4):
+                return
+            objs = build_batch(sub, lambda i: (tile, frag, fuse, wg))
+            trials.append(((tile, frag, fuse, wg), time_batch(objs[3], sweeps)))
+            objs[3].destroy()
+
+        for tile, frag, wg in itertools.product(tiles, frags, wgs):
+            trial(tile, frag, 1, wg)
+        _, frag, _, wg = min(trials, key=lambda t: t[1])[0]
+        for fuse, tile in itertools.product(fuses, tiles):
+            if fuse > 1:
+                trial(tile, frag, fuse, wg)
+        best[b] = min(trials, key=lambda t: t[1])[0]
+        log[b] = trials
+    return best, log
+
+
 def main(argv=None):
     ap = argparse.ArgumentParser(prog="python -m tcstencil_amd.autotune")
     ap.add_argument("--stencils", nargs="+", default=["star2d1r", "star2d2r", "box2d1r", "box2d2r"])

@@ -15,7 +15,8 @@ against 8 TB/s.
   c2  box2d1r 10240^2, FP16 U[-1,1), 100 sweeps, MFMA vs direct, errors after 1/10/100
   c3  star2d2r and box2d2r 16384^2, 4 sinusoids + noise, 10 sweeps, FP16/BF16/SPLIT MFMA
   c4  512 batched uneven grids (box2d1r/star2d2r, 64..2048 per side), 20 sweeps in a
-      hipGraph, vs one launch per grid
+      hipGraph: tile, MFMA fragment, sweeps per launch and waves per CU autotuned per
+      size bucket, vs a fixed tile rule at one sweep per launch and vs one launch per grid
   c5  box2d2r 32768x16384, 256 sweeps, SPLIT, 2/4/8 sweeps fused vs unfused MFMA and
       direct; rel-L2 after 256 sweeps
 """
@@ -175,38 +176,34 @@ def c3():
 
 def c4(nprob=512, sweeps=20):
     from . import api as T
+    from . import autotune as A
     torch = _torch()
     rng = np.random.default_rng(SEED)
     kinds = rng.integers(0, 2, nprob)
     rows = np.exp(rng.uniform(math.log(64), math.log(2048), nprob)).astype(int)
     cols = np.exp(rng.uniform(math.log(64), math.log(2048), nprob)).astype(int)
-    probs, b0s, b1s, xs = [], [], [], []
+    specs = []
     for i in range(nprob):
         stencil = ("box2d1r", "star2d2r")[kinds[i]]
-        B = T.gen_coeffs(stencil, SEED + i)
-        br, bc = (rows[i] + 15) // 16, (cols[i] + 15) // 16
-        # tile per size bucket (measured): small grids keep many workgroups busy
-        tile = (2, 4) if min(br, bc) < 16 else ((4, 4) if min(br, bc) < 64 else (4, 8))
-        p = T.Problem(stencil, int(rows[i]), int(cols[i]), B, tile=tile)
         x = T.gen_uniform(int(rows[i]), int(cols[i]), seed=SEED + i, dtype=torch.float32)
-        b0, b1 = p.alloc(), p.alloc()
-        p.clear(b1)
-        p.pack(x, b0)
-        probs.append(p), b0s.append(b0), b1s.append(b1), xs.append(x)
-    batch = T.Batch(probs, b0s, b1s)
-    pts = batch.interior_points * sweeps
-    g = batch.graph(sweeps)
-    g.launch()
-    torch.cuda.synchronize()
-    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-    reps = 5
-    e0.record()
-    for _ in range(reps):
-        g.launch()
-    e1.record()
-    torch.cuda.synchronize()
-    ms_batch = e0.elapsed_time(e1) / reps
+        specs.append((stencil, int(rows[i]), int(cols[i]), T.gen_coeffs(stencil, SEED + i), x))
+    bucket = [A.bucket_of(r, c) for _, r, c, _, _ in specs]
+
+    def timed(choice):
+        probs, b0s, b1s, batch = A.build_batch(specs, choice)
+        ms = A.time_batch(batch, sweeps, reps=5)
+        rec = (batch.interior_points * sweeps / (ms * 1e-3) / 1e9, ms, batch.launches_per_sweep)
+        batch.destroy()
+        return rec, probs, b0s, b1s
+
+    # the autotuner's choice per size bucket (tile, fragment, sweeps per launch, waves per CU)
+    best, log = A.tune_batch(specs, sweeps)
+    (gst_t, ms_t, nl_t), _, _, _ = timed(lambda i: best[bucket[i]])
+    # fixed rule, one sweep per launch
+    fixed = {0: ((2, 4), 16, 1, 0), 1: ((4, 4), 16, 1, 0), 2: ((4, 8), 16, 1, 0)}
+    (gst_f, ms_f, nl_f), probs, b0s, b1s = timed(lambda i: fixed[bucket[i]])
     # one launch per grid per sweep, the same work
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
     for p, b0, b1 in zip(probs, b0s, b1s):
         p.run(b0, b1, 2)
     torch.cuda.synchronize()
@@ -216,15 +213,19 @@ def c4(nprob=512, sweeps=20):
     e1.record()
     torch.cuda.synchronize()
     ms_each = e0.elapsed_time(e1)
-    gst_b = pts / (ms_batch * 1e-3) / 1e9
+    pts = sum((r - 2 * (1 if s == "box2d1r" else 2)) * (c - 2 * (1 if s == "box2d1r" else 2))
+              for s, r, c, _, _ in specs) * sweeps
     gst_e = pts / (ms_each * 1e-3) / 1e9
-    emit(config="c4", problems=nprob, sweeps=sweeps, interior_points_per_sweep=batch.interior_points,
-         launches_per_sweep=batch.launches_per_sweep, batched_graph_ms=round(ms_batch, 3),
-         batched_gstencil_s=round(gst_b, 1), batched_hbm_frac=round(gst_b * 4 / HBM_PEAK_GBS, 4),
+    emit(config="c4", problems=nprob, sweeps=sweeps, interior_points_per_sweep=pts // sweeps,
+         buckets={str(A.BUCKETS[b]): {"grids": bucket.count(b), "tile": list(t), "frag": f, "fuse": k, "wg_per_cu": w}
+                  for b, (t, f, k, w) in sorted(best.items())},
+         tuned_graph_ms=round(ms_t, 3), tuned_gstencil_s=round(gst_t, 1), tuned_launches_per_sweep=nl_t,
+         fixed_graph_ms=round(ms_f, 3), fixed_gstencil_s=round(gst_f, 1), fixed_launches_per_sweep=nl_f,
+         fixed_hbm_frac=round(gst_f * 4 / HBM_PEAK_GBS, 4),
          one_launch_per_grid_ms=round(ms_each, 3), one_launch_per_grid_gstencil_s=round(gst_e, 1),
-         speedup=round(ms_each / ms_batch, 2))
-    g.destroy()
-    batch.destroy()
+         speedup_tuned=round(ms_each / ms_t, 2), speedup_fixed=round(ms_each / ms_f, 2),
+         trials={str(A.BUCKETS[b]): [["%dx%d" % c[0], c[1], c[2], c[3], round(ms, 3)] for c, ms in tr]
+                 for b, tr in log.items()})
 
 
 def c5(sweeps=256):

@@ -76,7 +76,7 @@ struct BatchTile {
 int launch_batch(int precision, int tile_h, int tile_w, int frag, const BatchProblem *d_probs,
                  const BatchTile *d_tiles, int ntiles, int parity, int nsweeps, int grid_per_xcd,
                  hipStream_t stream);
-const char *batch_kernel_name(int precision, int tile_h, int tile_w, int frag, int nsweeps);
+const char *batch_kernel_name(int precision, int tile_h, int tile_w, int frag, int nsweeps, bool persist);
 int launch_sweep(const tcs_problem *p, const void *d_in, void *d_out, hipStream_t stream);
 int launch_fused(const tcs_problem *p, const void *d_in, void *d_out, int nsweeps, hipStream_t stream);
 const char *sweep_kernel_name(const tcs_problem *p);

@@ -959,61 +959,81 @@ __device__ __forceinline__ void batch_tile(const SweepArgs &a, const CoefWindows
     }
 }
 
-// One launch covers `ntiles` tiles of the list, FUSED: nsweeps sweeps per tile
-// in LDS (temporal blocking), else one sweep.  Workgroup w on XCD x = w % 8
-// walks the XCD's contiguous run of tiles with stride gridDim.x / 8 (a
-// persistent grid when the launch has fewer workgroups than tiles).  Every
-// tile reads buf[parity] and writes buf[parity ^ 1] of its problem.
+// One tile of a batch launch: tile idx of the list, FUSED: nsweeps sweeps in
+// LDS (temporal blocking), else one sweep.  Reads buf[parity] and writes
+// buf[parity ^ 1] of its problem; the radius selects one of two inlined
+// instantiations with a workgroup-uniform branch.
 template <int TH, int TW, int PREC, int FRAG, bool FUSED>
-__global__ __launch_bounds__(kWG) void tcs_mfma_batch(const BatchProblem *__restrict__ probs,
-                                                       const BatchTile *__restrict__ tiles, int ntiles,
-                                                       int tiles_per_xcd, int parity, int nsweeps) {
+__device__ __forceinline__ void batch_body(const BatchProblem *__restrict__ probs,
+                                           const BatchTile *__restrict__ tiles, int idx, int parity, int nsweeps,
+                                           uint16_t *lds) {
+    const BatchTile bt = tiles[idx];
+    const BatchProblem &P = probs[bt.prob];
+    SweepArgs a;
+    a.in = P.buf[parity];
+    a.out = P.buf[parity ^ 1];
+    a.plane = P.plane;
+    a.rows = P.rows;
+    a.cols = P.cols;
+    a.sbr = P.sbr;
+    a.sbc = P.sbc;
+    a.tiles_x = P.tiles_x;
+    a.tiles_xp = P.tiles_x;  // the tile list holds real tile indices
+    a.nvirt = 0;
+    a.rs = P.rs;
+    a.out_scale = P.out_scale;
+    int gr0, gc0;
+    tile_origin<TH, TW>(a, bt.tile, gr0, gc0);
+    if constexpr (FUSED && FRAG == 32) {
+        if (a.rs == 2)
+            fused_tile32<TH, TW, 0, 4, PREC>(a, P.cw, lds, gr0, gc0, nsweeps);
+        else
+            fused_tile32<TH, TW, 1, 3, PREC>(a, P.cw, lds, gr0, gc0, nsweeps);
+    } else if constexpr (FUSED) {
+        if (a.rs == 2)
+            fused_tile<TH, TW, 0, 4, PREC>(a, P.cw, lds, gr0, gc0, nsweeps);
+        else
+            fused_tile<TH, TW, 1, 3, PREC>(a, P.cw, lds, gr0, gc0, nsweeps);
+    } else if constexpr (FRAG == 32) {
+        if (a.rs == 2)
+            sweep_tile32<TH, TW, 0, 4, PREC>(a, P.cw, lds, gr0, gc0);
+        else
+            sweep_tile32<TH, TW, 1, 3, PREC>(a, P.cw, lds, gr0, gc0);
+    } else {
+        if (a.rs == 2)
+            batch_tile<TH, TW, 0, 4, PREC>(a, P.cw, lds, gr0, gc0);
+        else
+            batch_tile<TH, TW, 1, 3, PREC>(a, P.cw, lds, gr0, gc0);
+    }
+}
+
+// One launch covers `ntiles` tiles of the list.  Tiles are handed to XCDs in
+// contiguous runs of tiles_per_xcd (the list is problem-major, so a problem's
+// neighbouring tiles share an L2).  PERSIST: workgroup w on XCD x = w % 8
+// walks the XCD's run with stride gridDim.x / 8; else one tile per workgroup,
+// which needs no loop state and lets small tiles run 8 waves per SIMD.
+template <int TH, int TW, int PREC, int FRAG, bool FUSED, bool PERSIST>
+__global__ __launch_bounds__(kWG)
+__attribute__((amdgpu_waves_per_eu(FRAG == 16 && !PERSIST && PREC != TCS_PREC_SPLIT
+                                       ? (FUSED ? 4 : (kHighOccupancy<TH, TW, PREC, false> ? 8 : 1))
+                                       : 1,
+                                   8)))
+void tcs_mfma_batch(const BatchProblem *__restrict__ probs, const BatchTile *__restrict__ tiles, int ntiles,
+                    int tiles_per_xcd, int parity, int nsweeps) {
     constexpr int NPL = PREC == TCS_PREC_SPLIT ? 2 : 1;
     constexpr int PLANE = FUSED ? FusedGeom<TH, TW, NPL>::PLANE : Tile<TH, TW, NPL>::PLANE;
     __shared__ __attribute__((aligned(16))) uint16_t lds[NPL * PLANE];
     const int x = blockIdx.x % kXCD, step = gridDim.x / kXCD;
     const int begin = x * tiles_per_xcd;
     const int end = begin + tiles_per_xcd < ntiles ? begin + tiles_per_xcd : ntiles;
-    for (int idx = begin + (int)blockIdx.x / kXCD; idx < end; idx += step) {
-        if (idx != begin + (int)blockIdx.x / kXCD) __syncthreads();  // the previous tile is done with LDS
-        const BatchTile bt = tiles[idx];
-        const BatchProblem &P = probs[bt.prob];
-        SweepArgs a;
-        a.in = P.buf[parity];
-        a.out = P.buf[parity ^ 1];
-        a.plane = P.plane;
-        a.rows = P.rows;
-        a.cols = P.cols;
-        a.sbr = P.sbr;
-        a.sbc = P.sbc;
-        a.tiles_x = P.tiles_x;
-        a.tiles_xp = P.tiles_x;  // the tile list holds real tile indices
-        a.nvirt = 0;
-        a.rs = P.rs;
-        a.out_scale = P.out_scale;
-        int gr0, gc0;
-        tile_origin<TH, TW>(a, bt.tile, gr0, gc0);
-        if constexpr (FUSED && FRAG == 32) {
-            if (a.rs == 2)
-                fused_tile32<TH, TW, 0, 4, PREC>(a, P.cw, lds, gr0, gc0, nsweeps);
-            else
-                fused_tile32<TH, TW, 1, 3, PREC>(a, P.cw, lds, gr0, gc0, nsweeps);
-        } else if constexpr (FUSED) {
-            if (a.rs == 2)
-                fused_tile<TH, TW, 0, 4, PREC>(a, P.cw, lds, gr0, gc0, nsweeps);
-            else
-                fused_tile<TH, TW, 1, 3, PREC>(a, P.cw, lds, gr0, gc0, nsweeps);
-        } else if constexpr (FRAG == 32) {
-            if (a.rs == 2)
-                sweep_tile32<TH, TW, 0, 4, PREC>(a, P.cw, lds, gr0, gc0);
-            else
-                sweep_tile32<TH, TW, 1, 3, PREC>(a, P.cw, lds, gr0, gc0);
-        } else {
-            if (a.rs == 2)
-                batch_tile<TH, TW, 0, 4, PREC>(a, P.cw, lds, gr0, gc0);
-            else
-                batch_tile<TH, TW, 1, 3, PREC>(a, P.cw, lds, gr0, gc0);
+    const int first = begin + (int)blockIdx.x / kXCD;
+    if constexpr (PERSIST) {
+        for (int idx = first; idx < end; idx += step) {
+            if (idx != first) __syncthreads();  // the previous tile is done with LDS
+            batch_body<TH, TW, PREC, FRAG, FUSED>(probs, tiles, idx, parity, nsweeps, lds);
         }
+    } else {
+        if (first < end) batch_body<TH, TW, PREC, FRAG, FUSED>(probs, tiles, first, parity, nsweeps, lds);
     }
 }
 
@@ -1224,35 +1244,39 @@ static const FusedEntry *select_fused(const tcs_problem *p) {
 typedef void (*batch_fn)(const BatchProblem *, const BatchTile *, int, int, int, int);
 struct BatchEntry {
     int prec, th, tw, frag;
-    bool fused;
+    bool fused, persist;
     batch_fn fn;
     const char *name;
 };
-#define TCS_BATCH(TH, TW, P, F, FU)                                            \
-    {P, TH, TW, F, FU, tcs_mfma_batch<TH, TW, P, F, FU>,                     \
-     "tcs_mfma_batch<" #TH ", " #TW ", " #P ", " #F ", " #FU ">"}
-#define TCS_BATCH_SET(P, FU)                                                                          \
-    TCS_BATCH(2, 4, P, 16, FU), TCS_BATCH(4, 4, P, 16, FU), TCS_BATCH(4, 8, P, 16, FU),                 \
-        TCS_BATCH(4, 4, P, 32, FU), TCS_BATCH(4, 8, P, 32, FU)
+#define TCS_BATCH(TH, TW, P, F, FU, PS)                                          \
+    {P, TH, TW, F, FU, PS, tcs_mfma_batch<TH, TW, P, F, FU, PS>,               \
+     "tcs_mfma_batch<" #TH ", " #TW ", " #P ", " #F ", " #FU ", " #PS ">"}
+#define TCS_BATCH_SET(P, FU, PS)                                                                              \
+    TCS_BATCH(2, 4, P, 16, FU, PS), TCS_BATCH(4, 4, P, 16, FU, PS), TCS_BATCH(4, 8, P, 16, FU, PS),             \
+        TCS_BATCH(4, 4, P, 32, FU, PS), TCS_BATCH(4, 8, P, 32, FU, PS)
 static const BatchEntry kBatch[] = {
-    TCS_BATCH_SET(0, false), TCS_BATCH_SET(1, false), TCS_BATCH_SET(0, true), TCS_BATCH_SET(1, true),
+    TCS_BATCH_SET(0, false, false), TCS_BATCH_SET(1, false, false), TCS_BATCH_SET(0, true, false),
+    TCS_BATCH_SET(1, true, false),  TCS_BATCH_SET(0, false, true),  TCS_BATCH_SET(1, false, true),
+    TCS_BATCH_SET(0, true, true),   TCS_BATCH_SET(1, true, true),
 };
 
-static const BatchEntry *select_batch(int prec, int th, int tw, int frag, bool fused) {
+static const BatchEntry *select_batch(int prec, int th, int tw, int frag, bool fused, bool persist) {
     for (const BatchEntry &k : kBatch)
-        if (k.prec == prec && k.th == th && k.tw == tw && k.frag == frag && k.fused == fused) return &k;
+        if (k.prec == prec && k.th == th && k.tw == tw && k.frag == frag && k.fused == fused &&
+            k.persist == persist)
+            return &k;
     return nullptr;
 }
 
-const char *batch_kernel_name(int precision, int tile_h, int tile_w, int frag, int nsweeps) {
-    const BatchEntry *k = select_batch(precision, tile_h, tile_w, frag, nsweeps > 1);
+const char *batch_kernel_name(int precision, int tile_h, int tile_w, int frag, int nsweeps, bool persist) {
+    const BatchEntry *k = select_batch(precision, tile_h, tile_w, frag, nsweeps > 1, persist);
     return k ? k->name : nullptr;
 }
 
 int launch_batch(int precision, int tile_h, int tile_w, int frag, const BatchProblem *d_probs,
                  const BatchTile *d_tiles, int ntiles, int parity, int nsweeps, int grid_per_xcd,
                  hipStream_t stream) {
-    const BatchEntry *k = select_batch(precision, tile_h, tile_w, frag, nsweeps > 1);
+    const BatchEntry *k = select_batch(precision, tile_h, tile_w, frag, nsweeps > 1, grid_per_xcd > 0);
     if (!k) return set_error(TCS_ERR_UNSUPPORTED, "no batched kernel for precision %d tile %dx%d fragment %d",
                              precision, tile_h, tile_w, frag);
     if (ntiles <= 0) return TCS_OK;
