@@ -7,6 +7,9 @@
 #include <stdlib.h>
 #include <string.h>
 
+#include <thread>
+#include <vector>
+
 #include "tcs_internal.hpp"
 
 namespace tcs {
@@ -384,6 +387,90 @@ struct DeviceScope {
     }
 };
 
+// Host <-> device copies of pageable host arrays through two pinned staging
+// chunks: the CPU copy of one chunk (split over several threads) overlaps the
+// DMA of the other.  Pageable copies measured 28-30 GB/s host to device and
+// 10-17 GB/s back on MI355X; pinned DMA runs at ~57 GB/s.
+struct PinnedStage {
+    static constexpr size_t kChunk = 32u << 20;
+    void *buf[2] = {nullptr, nullptr};
+    hipEvent_t ev[2] = {nullptr, nullptr};
+    ~PinnedStage() {
+        for (int i = 0; i < 2; ++i) {
+            if (buf[i]) (void)hipHostFree(buf[i]);
+            if (ev[i]) (void)hipEventDestroy(ev[i]);
+        }
+    }
+    int init(size_t bytes) {
+        const size_t len = bytes < kChunk ? bytes : kChunk;
+        for (int i = 0; i < 2; ++i) {
+            hipError_t e = hipHostMalloc(&buf[i], len, hipHostMallocDefault);
+            if (e != hipSuccess) return hip_error(e, "hipHostMalloc (staging)");
+            e = hipEventCreateWithFlags(&ev[i], hipEventDisableTiming);
+            if (e != hipSuccess) return hip_error(e, "hipEventCreate (staging)");
+        }
+        return TCS_OK;
+    }
+};
+
+// memcpy split over up to 8 threads (one CPU thread reaches ~10-20 GB/s).
+static void parallel_copy(void *dst, const void *src, size_t bytes) {
+    unsigned nt = std::thread::hardware_concurrency();
+    nt = nt < 1 ? 1 : (nt > 8 ? 8 : nt);
+    if (bytes < (4u << 20) || nt == 1) {
+        memcpy(dst, src, bytes);
+        return;
+    }
+    const size_t part = (bytes / nt + 63) & ~(size_t)63;
+    std::vector<std::thread> th;
+    for (unsigned t = 1; t < nt; ++t) {
+        const size_t off = part * t;
+        if (off >= bytes) break;
+        const size_t len = off + part > bytes ? bytes - off : part;
+        th.emplace_back([=] { memcpy(static_cast<char *>(dst) + off, static_cast<const char *>(src) + off, len); });
+    }
+    memcpy(dst, src, part < bytes ? part : bytes);
+    for (auto &t : th) t.join();
+}
+
+static int staged_h2d(void *d_dst, const void *h_src, size_t bytes, hipStream_t s, PinnedStage &st) {
+    for (size_t off = 0, i = 0; off < bytes; off += PinnedStage::kChunk, ++i) {
+        const size_t len = bytes - off < PinnedStage::kChunk ? bytes - off : PinnedStage::kChunk;
+        const int k = (int)(i & 1);
+        if (i >= 2) TCS_HIP_CHECK(hipEventSynchronize(st.ev[k]));  // chunk i-2's DMA has drained buf[k]
+        parallel_copy(st.buf[k], static_cast<const char *>(h_src) + off, len);
+        TCS_HIP_CHECK(hipMemcpyAsync(static_cast<char *>(d_dst) + off, st.buf[k], len, hipMemcpyHostToDevice, s));
+        TCS_HIP_CHECK(hipEventRecord(st.ev[k], s));
+    }
+    return TCS_OK;
+}
+
+static int staged_d2h(void *h_dst, const void *d_src, size_t bytes, hipStream_t s, PinnedStage &st) {
+    const size_t n = (bytes + PinnedStage::kChunk - 1) / PinnedStage::kChunk;
+    auto issue = [&](size_t i) -> int {
+        const size_t off = i * PinnedStage::kChunk;
+        const size_t len = bytes - off < PinnedStage::kChunk ? bytes - off : PinnedStage::kChunk;
+        TCS_HIP_CHECK(hipMemcpyAsync(st.buf[i & 1], static_cast<const char *>(d_src) + off, len,
+                                     hipMemcpyDeviceToHost, s));
+        TCS_HIP_CHECK(hipEventRecord(st.ev[i & 1], s));
+        return TCS_OK;
+    };
+    if (n == 0) return TCS_OK;
+    int st0 = issue(0);
+    if (st0) return st0;
+    for (size_t i = 0; i < n; ++i) {
+        if (i + 1 < n) {
+            const int st1 = issue(i + 1);  // buf[(i+1)&1] was emptied by the CPU copy of chunk i-1
+            if (st1) return st1;
+        }
+        const size_t off = i * PinnedStage::kChunk;
+        const size_t len = bytes - off < PinnedStage::kChunk ? bytes - off : PinnedStage::kChunk;
+        TCS_HIP_CHECK(hipEventSynchronize(st.ev[i & 1]));
+        parallel_copy(static_cast<char *>(h_dst) + off, st.buf[i & 1], len);
+    }
+    return TCS_OK;
+}
+
 }  // namespace
 
 #define TCS_TRY(expr)          \
@@ -419,7 +506,9 @@ static int host_sweep_impl(int stencil, int method, int precision, const float *
     TCS_TRY(ds.alloc(&buf1, buf_bytes));
     TCS_TRY(ds.alloc(&d_max, sizeof(double)));
     hipStream_t s = ds.stream;
-    TCS_HIP_CHECK(hipMemcpyAsync(d_src, h_A, grid_bytes, hipMemcpyHostToDevice, s));
+    PinnedStage stage;
+    TCS_TRY(stage.init(grid_bytes));
+    TCS_TRY(staged_h2d(d_src, h_A, grid_bytes, s, stage));
 
     int scale_exp = 0;
     if (precision == TCS_PREC_SPLIT) {
@@ -454,7 +543,7 @@ static int host_sweep_impl(int stencil, int method, int precision, const float *
 
     void *result = (sweeps & 1) ? buf1 : buf0;
     TCS_TRY(tcs_unpack_f32(&p, result, (float *)d_dst, cols, scale_exp, (const float *)d_src, s));
-    TCS_HIP_CHECK(hipMemcpyAsync(h_C, d_dst, grid_bytes, hipMemcpyDeviceToHost, s));
+    TCS_TRY(staged_d2h(h_C, d_dst, grid_bytes, s, stage));
     TCS_HIP_CHECK(hipStreamSynchronize(s));
     return TCS_OK;
 }
