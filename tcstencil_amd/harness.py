@@ -128,6 +128,8 @@ def main(argv=None):
     s.add_argument("--methods", nargs="+", default=["mfma", "direct"])
     s.add_argument("--precision", default="fp16")
     s.add_argument("--sizes", type=int, nargs="+", default=[1024, 4096, 16384])
+    s.add_argument("--reference-meshes", action="store_true",
+                   help="the reference's mesh sweep, 160..7200 step 160 (script/param.py:4)")
     s.add_argument("--tiles", nargs="+", default=["%dx%d" % t for t in TILES])
     s.add_argument("--sweeps", type=int, default=20)
     s.add_argument("--rocprof", action="store_true")
@@ -142,7 +144,8 @@ def main(argv=None):
         return 0
 
     rows = []
-    for stencil, method, size, tile in itertools.product(a.stencils, a.methods, a.sizes, a.tiles):
+    sizes = [160 * i for i in range(1, 46)] if a.reference_meshes else a.sizes
+    for stencil, method, size, tile in itertools.product(a.stencils, a.methods, sizes, a.tiles):
         th, tw = (int(v) for v in tile.split("x"))
         if a.rocprof:
             cmd = [sys.executable, "-m", "tcstencil_amd.harness", "run", "--stencil", stencil,
