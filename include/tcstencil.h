@@ -30,8 +30,9 @@ enum { TCS_METHOD_MFMA = 0, TCS_METHOD_DIRECT = 1 };
 /* FP16: FP16 storage, FP16 coefficients, MFMA f16 with FP32 accumulation.
  * BF16: the same in BF16.
  * SPLIT: storage is a hi+lo FP16 pair (two planes) with a per-grid power-of-two
- *        scale; coefficients are split hi+lo too; four MFMA passes
- *        (hi*hi, hi*lo, lo*hi, lo*lo) accumulate into one FP32 accumulator. */
+ *        scale; coefficients are split hi+lo too; three MFMA passes
+ *        (hi*hi, hi*lo, lo*hi; lo*lo is below 2^-22 of the result) accumulate
+ *        into one FP32 accumulator. */
 enum { TCS_PREC_FP16 = 0, TCS_PREC_BF16 = 1, TCS_PREC_SPLIT = 2 };
 
 enum {

@@ -42,22 +42,44 @@ __device__ __forceinline__ void store8(uint16_t *p, u32x2 w) {
 }
 
 // ------------------------------------------------------------ tile staging
-// LDS image of BH x BW data bricks plus a 2-point frame on every side.
-template <int BH_, int BW_, int NPL_>
+// LDS image of BH x BW data bricks plus a 2-point frame on every side: element
+// (r, c), r < LR, c < LC, where columns [16*BW+4, LC) hold zeros.  Two layouts:
+//   row-major (32x32 MFMA and direct kernels): pitch LC elements, so a row is
+//     16 (mod 32) bytes and rows step through the banks;
+//   STRIP (16x16 MFMA kernels): the image is cut into 8-column strips stored
+//     one after another, NR = 16*BH + 8 rows of 16 bytes each.  A 16x16 MFMA
+//     operand fragment (16 lanes = 16 consecutive rows of one 8-column group)
+//     then reads 256 contiguous bytes, and since NR = 8 (mod 16), strips two
+//     apart start at the same bank: lane groups (0,1) and (2,3) of a
+//     ds_read_b128 reading the same rows of strips s and s+2 (or the same
+//     address) complete in the minimum 4 LDS cycles.  Measured on gfx950:
+//     ds_read_b128 of the 16x16 operand pattern takes 4 cycles per wave in
+//     this layout against 6 (unpacked) to 8 (packed) in the row-major one.
+template <int BH_, int BW_, int NPL_, bool STRIP_ = false>
 struct Geom {
     static constexpr int BH = BH_, BW = BW_, NPL = NPL_;
-    static constexpr int LR = 16 * BH + 4;      // LDS rows (2 frame rows each side)
-    static constexpr int LP = 16 * BW + 8;      // LDS pitch in elements; 2*LP = 16 (mod 32) bytes
+    static constexpr bool STRIP = STRIP_;
+    static constexpr int LR = 16 * BH + 4;      // image rows (2 frame rows each side)
+    static constexpr int LC = 16 * BW + 8;      // image columns (row-major pitch)
+    static constexpr int NR = 16 * BH + 8;      // rows per strip (STRIP)
     static constexpr int NCH = BH * BW * 32;    // 16-byte chunks of the data bricks
     static constexpr int CPT = (NCH + kWG - 1) / kWG;
     static constexpr int HD = 8 * BW + 2;       // dwords in one top/bottom frame row
     static constexpr int NH = 4 * HD + 32 * BH; // frame dwords: 2 top + 2 bottom rows, left, right
     static constexpr int HPT = (NH + kWG - 1) / kWG;
-    static constexpr int PLANE = LR * LP;       // LDS elements per plane
+    static constexpr int PLANE = STRIP ? LC * NR : LR * LC;  // LDS elements per plane
+    // Element offset of image point (r, c).  Offsets add: at(r1 + r2, c1 + c2)
+    // = at(r1, c1) + at(r2, c2) whenever c1 is a multiple of 8.
+    static __device__ __forceinline__ constexpr int at(int r, int c) {
+        if constexpr (STRIP)
+            return (c >> 3) * (8 * NR) + 8 * r + (c & 7);
+        else
+            return r * LC + c;
+    }
 };
 
-template <int TH, int TW, int NPL>
-using Tile = Geom<TH, TW, NPL>;
+template <int TH, int TW, int NPL, bool STRIP = false>
+using Tile = Geom<TH, TW, NPL, STRIP>;
 
 // One tile's global loads, held in registers between issue and the LDS write
 // so that the next tile's loads can be in flight while the current one computes.
@@ -135,12 +157,11 @@ __device__ __forceinline__ void write_tile(const TileRegs<G> &R, uint16_t *lds, 
             const int b = c >> 5, w = c & 31;
             const int br = b / G::BW, bc = b - br * G::BW;
             const int lr = 16 * br + (w >> 1) + 2;
-            const int lc = 16 * bc + 8 * (w & 1) + 2;
-            const int d = (lr * G::LP + lc) >> 1;  // lc even: dword aligned, not 16-byte aligned
+            const int lc = 16 * bc + 8 * (w & 1) + 2;  // even: dword aligned, not 16-byte aligned
 #pragma unroll
             for (int p = 0; p < G::NPL; ++p) {
 #pragma unroll
-                for (int q = 0; q < 4; ++q) l32[p * (G::PLANE / 2) + d + q] = R.cv[p][i][q];
+                for (int q = 0; q < 4; ++q) l32[p * (G::PLANE / 2) + (G::at(lr, lc + 2 * q) >> 1)] = R.cv[p][i][q];
             }
         }
     }
@@ -150,20 +171,20 @@ __device__ __forceinline__ void write_tile(const TileRegs<G> &R, uint16_t *lds, 
         if (h < G::NH) {
             int lr, ld;
             halo_coords<G>(h, lr, ld);
-            const int d = ((lr * G::LP) >> 1) + ld;
+            const int d = G::at(lr, 2 * ld) >> 1;
 #pragma unroll
             for (int p = 0; p < G::NPL; ++p) l32[p * (G::PLANE / 2) + d] = R.hv[p][i];
         }
     }
 }
 
-// Columns [16*BW+4, 16*BW+8) of every LDS row: read by the MFMA operand of the
-// last brick column (against zero coefficients), never written by write_tile.
+// Columns [16*BW+4, 16*BW+8) of every image row: read by the MFMA operand of
+// the last brick column (against zero coefficients), never written by write_tile.
 template <class G>
 __device__ __forceinline__ void zero_pad(uint16_t *lds, int tid) {
     uint32_t *l32 = reinterpret_cast<uint32_t *>(lds);
     for (int zc = tid; zc < 2 * G::LR; zc += kWG) {
-        const int d = (((zc >> 1) * G::LP) >> 1) + 8 * G::BW + 2 + (zc & 1);
+        const int d = G::at(zc >> 1, 16 * G::BW + 4 + 2 * (zc & 1)) >> 1;
 #pragma unroll
         for (int p = 0; p < G::NPL; ++p) l32[p * (G::PLANE / 2) + d] = 0u;
     }
@@ -242,27 +263,41 @@ struct Frag<TCS_PREC_SPLIT> : Frag<TCS_PREC_FP16> {};
 // K slots of the 16x16x32 MFMAs of one brick.  K slot group g (k = 8g..8g+7)
 // of MFMA f multiplies haloed row m + ZLO + zi, columns 8kg..8kg+7 (operand
 // B, lane (m, g)) by the taps c[ZLO + zi][8kg + e - j] (operand A, lane
-// (j, g)).  Radius 1 (3 row shifts): one MFMA per shift, kg = g, and group 3
-// (columns 24..31) meets zero taps.  Radius 2 (5 row shifts, 20 haloed
-// columns): the 15 (shift, column group) pairs are packed densely, pair
-// q = 4f + g -> (zi, kg) = (q / 3, q % 3), into 4 MFMAs instead of 5 (20% fewer
-// MFMAs and LDS operand reads); the 16th group repeats pair 14's columns
-// against zero taps.  Unused groups read finite data of the brick's window.
-template <int ZLO, int ZHI, bool PACK = true>
+// (j, g)).  Radius 1 (3 row shifts): one MFMA per shift zi = f, groups
+// g = 0..3 -> kg = 0, 2, 1, 1, the last against zero taps.  Radius 2 (5 row
+// shifts, 20 haloed columns): the 15 (shift, column group) pairs are packed
+// into 4 MFMAs instead of 5 (20% fewer MFMAs and LDS operand reads):
+//     f = 0: (0,0) (0,2) (1,0) (1,2)     f = 2: (4,0) (4,2) (0,1) (0,1)*
+//     f = 1: (2,0) (2,2) (3,0) (3,2)     f = 3: (1,1) (2,1) (3,1) (4,1)
+// (* zero taps).  The order serves the STRIP LDS layout: groups 0,1 and
+// groups 2,3 read the same rows of strips two apart, or the same address,
+// which reads without bank conflicts; only the f = 3 pairs differ by a row.
+// Every 16x16 kernel (one sweep, fused, batched) uses this one map, so the
+// MFMAs add the same 8-column partial sums into the FP32 accumulator in the
+// same order everywhere, and k fused sweeps stay bitwise equal to k single
+// sweeps.
+template <int ZLO, int ZHI>
 struct KMap {
     static constexpr int NZ = ZHI - ZLO + 1;
-    static constexpr bool PACKED = PACK && NZ == 5;
+    static constexpr bool PACKED = NZ == 5;
     static constexpr int NF = PACKED ? 4 : NZ;  // MFMAs (operand fragments) per brick and plane
     static __device__ __forceinline__ void group(int f, int g, int &zi, int &kg, bool &real) {
+        real = true;
         if constexpr (PACKED) {
-            int q = 4 * f + g;
-            real = q < 15;
-            if (!real) q = 14;
-            zi = q / 3;
-            kg = q - 3 * zi;
+            if (f < 2 || (f == 2 && g < 2)) {
+                zi = 2 * f + (g >> 1);
+                kg = 2 * (g & 1);
+            } else if (f == 2) {
+                zi = 0;
+                kg = 1;
+                real = g == 2;
+            } else {
+                zi = g + 1;
+                kg = 1;
+            }
         } else {
             zi = f;
-            kg = g < 2 ? g : 2;
+            kg = g == 0 ? 0 : (g == 1 ? 2 : 1);
             real = g < 3;
         }
     }
@@ -272,10 +307,10 @@ struct KMap {
 // e = 0..7 (zero outside the taps) -- the window at d0 = 8kg - j of the
 // zero-padded tap row, loaded whole from the host-built window table
 // (CoefWindows; window 12 is all zeros).
-template <int ZLO, int ZHI, int PREC, bool PACK = true>
+template <int ZLO, int ZHI, int PREC>
 struct Coef {
     typedef typename Frag<PREC>::type frag_t;
-    using K = KMap<ZLO, ZHI, PACK>;
+    using K = KMap<ZLO, ZHI>;
     frag_t hi[K::NF], lo[K::NF];
 
     __device__ __forceinline__ void load(const CoefWindows &cw, int m, int g) {
@@ -300,9 +335,10 @@ struct Coef {
 // geometry G, whose haloed 20x20 window starts at LDS row 16*br, column 16*bc.
 // Fragment f: lane (m, g) holds haloed row m + ZLO + zi, columns 8kg..8kg+7
 // (KMap), one 16-byte LDS read; all reads of a brick issue back to back.
-template <class G, int ZLO, int ZHI, int PREC, bool PACK = true>
+template <class G, int ZLO, int ZHI, int PREC>
 struct Ops {
-    using K = KMap<ZLO, ZHI, PACK>;
+    static_assert(G::STRIP, "the K slot order is chosen for the STRIP LDS layout");
+    using K = KMap<ZLO, ZHI>;
     u32x4 h[K::NF];
     u32x4 l[PREC == TCS_PREC_SPLIT ? K::NF : 1];
 
@@ -315,7 +351,7 @@ struct Ops {
                 int zi, kg;
                 bool real;
                 K::group(f, g, zi, kg, real);
-                o[f] = (m + ZLO + zi) * G::LP + 8 * kg;
+                o[f] = G::at(m + ZLO + zi, 8 * kg);
             }
         }
     };
@@ -323,10 +359,10 @@ struct Ops {
     // Unpacked fragments differ by whole LDS rows (compile-time offsets from
     // fragment 0's address); packed ones need the per-lane offsets.
     __device__ __forceinline__ void load(const uint16_t *lds, int br, int bc, const Offsets &off) {
-        const uint16_t *base = lds + 16 * br * G::LP + 16 * bc;
+        const uint16_t *base = lds + G::at(16 * br, 16 * bc);
 #pragma unroll
         for (int f = 0; f < K::NF; ++f) {
-            const int o = K::PACKED ? off.o[f] : off.o[0] + f * G::LP;
+            const int o = K::PACKED ? off.o[f] : off.o[0] + G::at(f, 0);
             h[f] = *reinterpret_cast<const u32x4 *>(base + o);
             if constexpr (PREC == TCS_PREC_SPLIT) l[f] = *reinterpret_cast<const u32x4 *>(base + G::PLANE + o);
         }
@@ -337,13 +373,13 @@ struct Ops {
 // 4g..4g+3 in storage format (w[p]: four 16-bit values of plane p).  gr, gcb:
 // global row and first column of those outputs; with `edge`, points outside
 // the interior keep their input value (read from the LDS image).
-template <class G, int ZLO, int ZHI, int PREC, bool PACK>
+template <class G, int ZLO, int ZHI, int PREC>
 __device__ __forceinline__ void mfma_brick(const SweepArgs &a, const uint16_t *lds,
-                                           const Coef<ZLO, ZHI, PREC, PACK> &C,
-                                           const Ops<G, ZLO, ZHI, PREC, PACK> &op, int br, int bc, int m, int g,
+                                           const Coef<ZLO, ZHI, PREC> &C,
+                                           const Ops<G, ZLO, ZHI, PREC> &op, int br, int bc, int m, int g,
                                            bool edge, int gr, int gcb, u32x2 (&w)[G::NPL]) {
     constexpr int NPL = G::NPL;
-    constexpr int NF = KMap<ZLO, ZHI, PACK>::NF;
+    constexpr int NF = KMap<ZLO, ZHI>::NF;
     using F = Frag<PREC>;
     typedef typename F::type frag_t;
     f32x4 acc = {0.0f, 0.0f, 0.0f, 0.0f};
@@ -352,10 +388,14 @@ __device__ __forceinline__ void mfma_brick(const SweepArgs &a, const uint16_t *l
         const frag_t bh = __builtin_bit_cast(frag_t, op.h[f]);
         acc = F::mma(C.hi[f], bh, acc);
         if constexpr (PREC == TCS_PREC_SPLIT) {
+            // c*x = (ch + cl)(xh + xl) without the cl*xl term: |cl| <= 2^-11 |c|
+            // and |xl| <= 2^-11 |x| (round to nearest), so the dropped part of
+            // an output is at most 2^-22 * sum |c||x| -- at most 2.4e-7 of the
+            // max-abs of the (scaled) input for taps summing to 1, below the
+            // SPLIT target of 1e-6, for a quarter of the MFMAs.
             const frag_t bl = __builtin_bit_cast(frag_t, op.l[f]);
             acc = F::mma(C.hi[f], bl, acc);
             acc = F::mma(C.lo[f], bh, acc);
-            acc = F::mma(C.lo[f], bl, acc);
         }
     }
     uint16_t o[NPL][4];
@@ -373,27 +413,46 @@ __device__ __forceinline__ void mfma_brick(const SweepArgs &a, const uint16_t *l
         }
     }
     if (edge) {
-        const int cidx = (16 * br + m + 2) * G::LP + 16 * bc + 4 * g + 2;
+        // Columns 4g+2, 4g+3 and 4g+4, 4g+5 of the image row: two pairs, each
+        // within one 8-column strip.
+        const int cb = G::at(16 * br, 16 * bc);
+        const int c01 = cb + G::at(m + 2, 4 * g + 2);
+        const int c23 = cb + G::at(m + 2, 4 * g + 4);
 #pragma unroll
         for (int v = 0; v < 4; ++v)
             if (!point_is_interior(a, gr, gcb + v)) {
+                const int cidx = (v < 2 ? c01 : c23) + (v & 1);
 #pragma unroll
-                for (int p = 0; p < NPL; ++p) o[p][v] = lds[p * G::PLANE + cidx + v];
+                for (int p = 0; p < NPL; ++p) o[p][v] = lds[p * G::PLANE + cidx];
             }
     }
 #pragma unroll
     for (int p = 0; p < NPL; ++p) w[p] = u32x2{pack2(o[p][0], o[p][1]), pack2(o[p][2], o[p][3])};
 }
 
-// Brick q of a wave's list -> (br, bc) in the LDS image.
+// Brick q of a wave's list -> (br, bc) in the LDS image: bricks dealt to the
+// 4 waves round robin, b = wave + 4q (one-pass kernels).
 template <int BW, int BASE>
 __device__ __forceinline__ void brick_of(int b, int &br, int &bc) {
     br = BASE + b / BW;
     bc = BASE + (b - (b / BW) * BW);
 }
 
+// Brick q of a wave's share of a BH x BW brick region (both even) whose first
+// brick is (BASE, BASE), for the sweep loop of the fused kernels: the 4 waves
+// split the region 2 x 2, wave w taking the (BH/2) x (BW/2) block at block
+// position (w / 2, w % 2), q row-major within it.  A brick's offset from its
+// block's first brick is then a compile-time constant, so a per-brick LDS
+// address is one register plus an immediate.
+template <int BH, int BW, int BASE>
+__device__ __forceinline__ void block_brick(int wave, int q, int &br, int &bc) {
+    static_assert(BH % 2 == 0 && BW % 2 == 0, "brick region must split 2 x 2 over the waves");
+    br = BASE + (wave >> 1) * (BH / 2) + q / (BW / 2);
+    bc = BASE + (wave & 1) * (BW / 2) + q % (BW / 2);
+}
+
 // Operand look-ahead across bricks costs 4*NZ registers (8*NZ for SPLIT); it
-// pays for FP16/BF16 and measured slower for SPLIT, whose four MFMA passes per
+// pays for FP16/BF16 and measured slower for SPLIT, whose three MFMA passes per
 // shift already hide the LDS latency.
 template <int PREC>
 constexpr bool kLookahead = PREC != TCS_PREC_SPLIT;
@@ -413,7 +472,7 @@ __attribute__((amdgpu_waves_per_eu(kHighOccupancy<TH, TW, PREC, PERSIST> ? 8 : 1
 void tcs_mfma_sweep(SweepArgs a) {
     constexpr int NPL = PREC == TCS_PREC_SPLIT ? 2 : 1;
     constexpr bool LOOKAHEAD = kLookahead<PREC> && !kHighOccupancy<TH, TW, PREC, PERSIST>;
-    using T = Tile<TH, TW, NPL>;
+    using T = Tile<TH, TW, NPL, true>;
     __shared__ __attribute__((aligned(16))) uint16_t lds[NPL * T::PLANE];
 
     Schedule sc = schedule(a);
@@ -488,20 +547,15 @@ void tcs_mfma_sweep(SweepArgs a) {
 // equal to k single sweeps.  Values outside the valid region stay finite
 // (stencils of loaded data), which the MFMA form needs: a non-finite operand
 // anywhere in a brick row's window reaches all 16 outputs of that row.
-template <int TH, int TW, int NPL>
-using FusedGeom = Geom<TH + 2, TW + 2, NPL>;
+template <int TH, int TW, int NPL, bool STRIP = false>
+using FusedGeom = Geom<TH + 2, TW + 2, NPL, STRIP>;
 
 template <int TH, int TW, int ZLO, int ZHI, int PREC>
 __device__ __forceinline__ void fused_tile(const SweepArgs &a, const CoefWindows &cw, uint16_t *lds, int gr0,
                                            int gc0, int nsweeps) {
     constexpr int NPL = PREC == TCS_PREC_SPLIT ? 2 : 1;
     constexpr bool LOOKAHEAD = false;  // registers go to occupancy instead (see tcs_mfma_fused)
-    using G = FusedGeom<TH, TW, NPL>;
-    // Packed K slots (radius 2) pay off for SPLIT's four MFMA passes, with the
-    // opaque per-sweep offset below holding register use down; FP16/BF16 run
-    // faster unpacked and with the compiler's hoisted addresses (box2d2r
-    // 16384^2, 4x8, 8 fused sweeps: SPLIT 394 -> 349 us, FP16 134 vs 148 us).
-    constexpr bool PACK = PREC == TCS_PREC_SPLIT;
+    using G = FusedGeom<TH, TW, NPL, true>;
     constexpr int QA = G::BH * G::BW / 4;  // bricks per wave, whole image
     constexpr int QC = TH * TW / 4;        // bricks per wave, core only
     static_assert(G::BH * G::BW % 4 == 0 && TH * TW % 4 == 0, "bricks must split over 4 waves");
@@ -515,73 +569,77 @@ __device__ __forceinline__ void fused_tile(const SweepArgs &a, const CoefWindows
     }
     const int lane = tid & 63, wave = tid >> 6;
     const int m = lane & 15, g = lane >> 4;
-    Coef<ZLO, ZHI, PREC, PACK> C;
+    Coef<ZLO, ZHI, PREC> C;
     C.load(cw, m, g);
-    const typename Ops<G, ZLO, ZHI, PREC, PACK>::Offsets off(m, g);
+    const typename Ops<G, ZLO, ZHI, PREC>::Offsets off(m, g);
     const bool edge = !tile_is_interior(a, er0, ec0, 16 * G::BH, 16 * G::BW);
     __syncthreads();
 
     for (int sw = 0; sw + 1 < nsweeps; ++sw) {
-        // Packed: an opaque zero offset per sweep keeps the compiler from
-        // hoisting every brick's operand and write-back addresses out of the
-        // sweep loop (about 4 registers per brick), which would cost occupancy.
-        int opaque = 0;
-        if constexpr (PACK) asm volatile("" : "+s"(opaque));
-        uint16_t *ls = lds + opaque;
-        uint32_t *l32 = reinterpret_cast<uint32_t *>(ls);
+        // The lane row, made opaque every sweep: all lane-dependent address
+        // parts below derive from it, so the compiler cannot hoist per-brick
+        // addresses or edge tests out of the sweep loop (registers per brick
+        // would cost occupancy or spill).  With brick_of's compile-time brick
+        // offsets, each address is then one register plus an immediate.
+        int mo = m;
+        asm volatile("" : "+v"(mo));
+        const typename Ops<G, ZLO, ZHI, PREC>::Offsets offs(mo, g);
+        uint32_t *l32 = reinterpret_cast<uint32_t *>(lds);
         u32x2 res[QA][NPL];
-        Ops<G, ZLO, ZHI, PREC, PACK> cur, nxt;
+        Ops<G, ZLO, ZHI, PREC> cur, nxt;
         if constexpr (LOOKAHEAD) {
             int br, bc;
-            brick_of<G::BW, 0>(wave, br, bc);
-            cur.load(ls, br, bc, off);
+            block_brick<G::BH, G::BW, 0>(wave, 0, br, bc);
+            cur.load(lds, br, bc, offs);
         }
 #pragma unroll
         for (int q = 0; q < QA; ++q) {
             int br, bc;
-            brick_of<G::BW, 0>(wave + 4 * q, br, bc);
+            block_brick<G::BH, G::BW, 0>(wave, q, br, bc);
             if constexpr (LOOKAHEAD) {
                 if (q + 1 < QA) {
                     int nbr, nbc;
-                    brick_of<G::BW, 0>(wave + 4 * (q + 1), nbr, nbc);
-                    nxt.load(ls, nbr, nbc, off);
+                    block_brick<G::BH, G::BW, 0>(wave, q + 1, nbr, nbc);
+                    nxt.load(lds, nbr, nbc, offs);
                 }
             } else {
-                cur.load(ls, br, bc, off);
+                cur.load(lds, br, bc, offs);
             }
             __builtin_amdgcn_sched_barrier(0);
-            mfma_brick<G, ZLO, ZHI, PREC>(a, ls, C, cur, br, bc, m, g, edge, er0 + 16 * br + m,
+            mfma_brick<G, ZLO, ZHI, PREC>(a, lds, C, cur, br, bc, mo, g, edge, er0 + 16 * br + mo,
                                           ec0 + 16 * bc + 4 * g, res[q]);
             if constexpr (LOOKAHEAD) cur = nxt;
         }
         __syncthreads();  // every wave has read this sweep's input
 #pragma unroll
         for (int q = 0; q < QA; ++q) {
-            const int b = wave + 4 * q;
-            const int br = b / G::BW, bc = b - br * G::BW;
-            const int d = ((16 * br + m + 2) * G::LP + 16 * bc + 4 * g + 2) >> 1;
+            int br, bc;
+            block_brick<G::BH, G::BW, 0>(wave, q, br, bc);
+            const int db = G::at(16 * br, 16 * bc) >> 1;
+            const int d0 = db + (G::at(mo + 2, 4 * g + 2) >> 1);
+            const int d1 = db + (G::at(mo + 2, 4 * g + 4) >> 1);
 #pragma unroll
             for (int p = 0; p < NPL; ++p) {
-                l32[p * (G::PLANE / 2) + d] = res[q][p][0];
-                l32[p * (G::PLANE / 2) + d + 1] = res[q][p][1];
+                l32[p * (G::PLANE / 2) + d0] = res[q][p][0];
+                l32[p * (G::PLANE / 2) + d1] = res[q][p][1];
             }
         }
         __syncthreads();
     }
-    Ops<G, ZLO, ZHI, PREC, PACK> cur, nxt;
+    Ops<G, ZLO, ZHI, PREC> cur, nxt;
     if constexpr (LOOKAHEAD) {
         int br, bc;
-        brick_of<TW, 1>(wave, br, bc);
+        block_brick<TH, TW, 1>(wave, 0, br, bc);
         cur.load(lds, br, bc, off);
     }
 #pragma unroll
     for (int q = 0; q < QC; ++q) {
         int br, bc;
-        brick_of<TW, 1>(wave + 4 * q, br, bc);
+        block_brick<TH, TW, 1>(wave, q, br, bc);
         if constexpr (LOOKAHEAD) {
             if (q + 1 < QC) {
                 int nbr, nbc;
-                brick_of<TW, 1>(wave + 4 * (q + 1), nbr, nbc);
+                block_brick<TH, TW, 1>(wave, q + 1, nbr, nbc);
                 nxt.load(lds, nbr, nbc, off);
             }
         } else {
@@ -607,7 +665,7 @@ template <int TH, int TW, int ZLO, int ZHI, int PREC>
 __global__ __launch_bounds__(kWG) __attribute__((amdgpu_waves_per_eu(PREC == TCS_PREC_SPLIT ? 1 : 4, 8)))
 void tcs_mfma_fused(SweepArgs a, int nsweeps) {
     constexpr int NPL = PREC == TCS_PREC_SPLIT ? 2 : 1;
-    __shared__ __attribute__((aligned(16))) uint16_t lds[NPL * FusedGeom<TH, TW, NPL>::PLANE];
+    __shared__ __attribute__((aligned(16))) uint16_t lds[NPL * FusedGeom<TH, TW, NPL, true>::PLANE];
     const Schedule sc = schedule(a);
     if (sc.tile >= sc.end) return;
     int gr0, gc0;
@@ -713,17 +771,17 @@ struct Ops32 {
     u32x4 f[CH];
 
     __device__ __forceinline__ void load(const uint16_t *lds, int c, int br, int bc, int i, int h) {
-        const uint16_t *base = lds + (16 * br + i + ZLO) * G::LP + 16 * bc;
+        const uint16_t *base = lds + (16 * br + i + ZLO) * G::LC + 16 * bc;
 #pragma unroll
         for (int t = 0; t < CH; ++t) {
             const int mf = c * CH + t;
             if (mf >= F::NMF) break;
             if (mf < 2 * F::NZ) {
-                f[t] = *reinterpret_cast<const u32x4 *>(base + (mf >> 1) * G::LP + 8 * (2 * (mf & 1) + h));
+                f[t] = *reinterpret_cast<const u32x4 *>(base + (mf >> 1) * G::LC + 8 * (2 * (mf & 1) + h));
             } else {
                 const int p = F::pair(mf, h);
-                const u32x2 x = *reinterpret_cast<const u32x2 *>(base + F::half_shift(p, 0) * G::LP + 32);
-                const u32x2 y = *reinterpret_cast<const u32x2 *>(base + F::half_shift(p, 1) * G::LP + 32);
+                const u32x2 x = *reinterpret_cast<const u32x2 *>(base + F::half_shift(p, 0) * G::LC + 32);
+                const u32x2 y = *reinterpret_cast<const u32x2 *>(base + F::half_shift(p, 1) * G::LC + 32);
                 f[t] = u32x4{x[0], x[1], y[0], y[1]};
             }
         }
@@ -756,7 +814,7 @@ __device__ __forceinline__ void epilogue32(const SweepArgs &a, const uint16_t *l
 #pragma unroll
     for (int v = 0; v < 16; ++v) o[v] = PREC == TCS_PREC_BF16 ? bf16_bits(acc[v]) : f16_bits(acc[v]);
     if (edge) {
-        const uint16_t *row = lds + (16 * br + i + 2) * G::LP + 16 * bc + 2;
+        const uint16_t *row = lds + (16 * br + i + 2) * G::LC + 16 * bc + 2;
 #pragma unroll
         for (int v = 0; v < 16; ++v) {
             const int cb = 16 * (v >> 3) + 8 * h + (v & 7);
@@ -876,7 +934,7 @@ __device__ __forceinline__ void fused_tile32(const SweepArgs &a, const CoefWindo
             const int b = wave + 4 * qq;
             if (QA * 4 != NBA && b >= NBA) break;
             const int br = 2 * (b / BXA), bc = 2 * (b % BXA);
-            const int d = ((16 * br + i + 2) * G::LP + 16 * bc + 8 * h + 2) >> 1;
+            const int d = ((16 * br + i + 2) * G::LC + 16 * bc + 8 * h + 2) >> 1;
 #pragma unroll
             for (int s = 0; s < 2; ++s)
 #pragma unroll
@@ -913,7 +971,7 @@ __device__ __forceinline__ void batch_tile(const SweepArgs &a, const CoefWindows
                                            int gc0) {
     constexpr int NPL = PREC == TCS_PREC_SPLIT ? 2 : 1;
     constexpr bool LOOKAHEAD = kLookahead<PREC> && !kHighOccupancy<TH, TW, PREC, false>;
-    using T = Tile<TH, TW, NPL>;
+    using T = Tile<TH, TW, NPL, true>;
     const int tid = threadIdx.x;
     {
         TileRegs<T> R;
@@ -1021,7 +1079,7 @@ __attribute__((amdgpu_waves_per_eu(FRAG == 16 && !PERSIST && PREC != TCS_PREC_SP
 void tcs_mfma_batch(const BatchProblem *__restrict__ probs, const BatchTile *__restrict__ tiles, int ntiles,
                     int tiles_per_xcd, int parity, int nsweeps) {
     constexpr int NPL = PREC == TCS_PREC_SPLIT ? 2 : 1;
-    constexpr int PLANE = FUSED ? FusedGeom<TH, TW, NPL>::PLANE : Tile<TH, TW, NPL>::PLANE;
+    constexpr int PLANE = FUSED ? FusedGeom<TH, TW, NPL, FRAG == 16>::PLANE : Tile<TH, TW, NPL, FRAG == 16>::PLANE;
     __shared__ __attribute__((aligned(16))) uint16_t lds[NPL * PLANE];
     const int x = blockIdx.x % kXCD, step = gridDim.x / kXCD;
     const int begin = x * tiles_per_xcd;
@@ -1093,7 +1151,7 @@ __global__ __launch_bounds__(kWG) void tcs_direct_sweep(SweepArgs a) {
             for (int dj = 0; dj < 5; ++dj)
                 if (ir - o >= 0 && ir - o <= 4 && tap<STENCIL>(ir - o, dj)) used = true;
         if (!used) continue;
-        const uint16_t *row = lds + (r0 + ir) * T::LP + 4 * cg;
+        const uint16_t *row = lds + (r0 + ir) * T::LC + 4 * cg;
         const u32x2 lo = *reinterpret_cast<const u32x2 *>(row);
         const u32x2 hi = *reinterpret_cast<const u32x2 *>(row + 4);
         const uint32_t w4[4] = {lo[0], lo[1], hi[0], hi[1]};
@@ -1127,7 +1185,7 @@ __global__ __launch_bounds__(kWG) void tcs_direct_sweep(SweepArgs a) {
 #pragma unroll
         for (int v = 0; v < 4; ++v) h[v] = f16_bits(acc[o][v]);
         if (edge) {
-            const int cidx = (tr + 2) * T::LP + 4 * cg + 2;
+            const int cidx = (tr + 2) * T::LC + 4 * cg + 2;
 #pragma unroll
             for (int v = 0; v < 4; ++v)
                 if (!point_is_interior(a, gr, gcb + v)) h[v] = lds[cidx + v];
@@ -1301,10 +1359,10 @@ const char *sweep_kernel_name(const tcs_problem *p) {
 int mfma_per_brick(const tcs_problem *p) {
     if (p->method != TCS_METHOD_MFMA) return 0;
     if (frag_of(p) == 32) return p->radius == 2 ? Fold32<0, 4>::NMF : Fold32<1, 3>::NMF;  // per 2x2-brick block
-    // Single sweeps pack radius-2 K slots (4 MFMAs per brick and plane pair);
-    // fused FP16/BF16 sweeps do not (5).
+    // Radius 2 packs its K slots into 4 MFMAs per brick (3 for radius 1);
+    // SPLIT issues 3 per fragment (hi*hi, hi*lo, lo*hi).
     const int nf = p->radius == 2 ? KMap<0, 4>::NF : KMap<1, 3>::NF;
-    return p->precision == TCS_PREC_SPLIT ? 4 * nf : nf;
+    return p->precision == TCS_PREC_SPLIT ? 3 * nf : nf;
 }
 
 int64_t mfma_flops_per_brick(const tcs_problem *p) {

@@ -101,16 +101,34 @@ def unpack(st: np.ndarray, L: Layout, ring=None, rs=0) -> np.ndarray:
 
 # ------------------------------------------------------------ tile staging
 class TileGeom:
-    def __init__(self, TH, TW):
-        self.TH, self.TW = TH, TW
+    """LDS image of TH x TW bricks plus the 2-point frame: image point (r, c),
+    r < LR, c < LP.  Row-major (pitch LP) for the 32x32 and direct kernels;
+    strip=True (16x16 MFMA kernels): 8-column strips of NR = 16*TH + 8 rows,
+    stored one after another."""
+
+    def __init__(self, TH, TW, strip=False):
+        self.TH, self.TW, self.strip = TH, TW, strip
         self.LR = 16 * TH + 4
         self.LP = 16 * TW + 8
+        self.NR = 16 * TH + 8
         self.NCH = TH * TW * 32
         self.CPT = -(-self.NCH // WG)
         self.HD = 8 * TW + 2
         self.NH = 4 * self.HD + 32 * TH
         self.HPT = -(-self.NH // WG)
-        self.PLANE = self.LR * self.LP
+        self.PLANE = self.LP * (self.NR if strip else self.LR)
+
+    def image(self):
+        """LDS offsets of every image point (strip padding rows excluded)."""
+        r, c = np.meshgrid(np.arange(self.LR), np.arange(self.LP), indexing="ij")
+        return self.at(r, c).ravel()
+
+    def at(self, r, c):
+        r, c = np.asarray(r), np.asarray(c)
+        assert np.all((r >= 0) & (r < self.LR) & (c >= 0) & (c < self.LP)), "image point out of range"
+        if self.strip:
+            return (c // 8) * (8 * self.NR) + 8 * r + (c % 8)
+        return r * self.LP + c
 
 
 def stage_tile(inp: np.ndarray, L: Layout, G: TileGeom, gr0: int, gc0: int, check=False,
@@ -130,11 +148,11 @@ def stage_tile(inp: np.ndarray, L: Layout, G: TileGeom, gr0: int, gc0: int, chec
         _check(s, L.plane - 7, 8, "centre load (16 B)")
         lr = 16 * br + (w >> 1) + 2
         lc = 16 * bc + 8 * (w & 1) + 2
-        e0 = lr * G.LP + lc
-        assert np.all(e0 % 2 == 0), "centre LDS write not dword aligned"
-        _check(e0, G.PLANE - 7, 2, "centre LDS write")
-        for k in range(8):
-            lds[e0 + k] = inp[s + k]
+        for q in range(4):  # four dword writes
+            e = G.at(lr, lc + 2 * q)
+            _check(e, G.PLANE - 1, 2, "centre LDS write (dword)")
+            lds[e] = inp[s + 2 * q]
+            lds[e + 1] = inp[s + 2 * q + 1]
     for i in range(G.HPT):
         h = tid + i * WG
         lr = np.empty_like(h)
@@ -155,14 +173,14 @@ def stage_tile(inp: np.ndarray, L: Layout, G: TileGeom, gr0: int, gc0: int, chec
             inside = (((gr + 16) >> 4) >= 0) & (((gr + 16) >> 4) < L.sbr) & (((gc + 16) >> 4) >= 0) & (((gc + 16) >> 4) < L.sbc)
         s = stor_index(gr, gc, L.sbc)
         _check(s[inside], L.plane - 1, 2, "halo load (4 B)")
-        d = ((lr[act] * G.LP) >> 1) + ld[act]
+        d = G.at(lr[act], 2 * ld[act]) >> 1
         _check(2 * d, G.PLANE - 1, 2, "halo LDS write")
         v0, v1 = np.zeros(gr.shape), np.zeros(gr.shape)
         v0[inside], v1[inside] = inp[s[inside]], inp[s[inside] + 1]
         lds[2 * d] = v0
         lds[2 * d + 1] = v1
     for zc in range(2 * G.LR if zero_pad else 0):
-        d = (((zc >> 1) * G.LP) >> 1) + 8 * TW + 2 + (zc & 1)
+        d = int(G.at(zc >> 1, 16 * TW + 4 + 2 * (zc & 1))) >> 1
         assert 0 <= 2 * d < G.PLANE
         lds[2 * d] = 0.0
         lds[2 * d + 1] = 0.0
@@ -199,20 +217,29 @@ def _interior(gr, gc, L: Layout, rs: int):
 
 
 # ------------------------------------------------------------- MFMA sweep
+# (row shift index, column group) of the K slot groups g = 0..3 of MFMA f; the
+# radius-2 packing of the 15 pairs into 4 MFMAs, with None for the group that
+# meets zero taps (it reads the pair to its left).
+_PACKED_R2 = (((0, 0), (0, 2), (1, 0), (1, 2)),
+              ((2, 0), (2, 2), (3, 0), (3, 2)),
+              ((4, 0), (4, 2), (0, 1), None),
+              ((1, 1), (2, 1), (3, 1), (4, 1)))
+
+
 def _kmap(rs, f, g):
     """KMap of tcs_sweep.hip: (row shift index, column group, real) of K slot
-    group g (array of lane groups) of MFMA f.  Radius 2 packs the 15 (shift,
-    column group) pairs into 4 MFMAs, q = 4f + g -> (q // 3, q % 3); pair 15
-    repeats pair 14 against zero taps.  Radius 1: one MFMA per shift, group 3
-    repeats group 2 against zero taps."""
+    group g (array of lane groups) of MFMA f.  Radius 2: the packed table
+    above.  Radius 1: one MFMA per shift, column groups 0, 2, 1 and a repeat
+    of 1 against zero taps."""
     g = np.asarray(g)
     if rs == 2:
-        q = 4 * f + g
-        real = q < 15
-        q = np.where(real, q, 14)
-        zi = q // 3
-        return zi, q - 3 * zi, real
-    return np.full_like(g, f), np.minimum(g, 2), g < 3
+        row = _PACKED_R2[f]
+        ent = [row[k] if row[k] is not None else row[k - 1] for k in range(4)]
+        zi = np.array([ent[k][0] for k in range(4)])[g]
+        kg = np.array([ent[k][1] for k in range(4)])[g]
+        real = np.array([row[k] is not None for k in range(4)])[g]
+        return zi, kg, real
+    return np.full_like(g, f), np.array([0, 2, 1, 1])[g], g < 3
 
 
 def _nfrag(rs):
@@ -241,7 +268,7 @@ def _mfma_brick(lds, G: TileGeom, afrag, ZLO, ZHI, br, bc, edge, gr, gcb, L: Lay
     D = np.zeros((16, 16))
     for f in range(_nfrag(rs)):
         zi, kg, _ = _kmap(rs, f, g)
-        off = (16 * br + m + ZLO + zi) * G.LP + 16 * bc + 8 * kg
+        off = G.at(16 * br + m + ZLO + zi, 16 * bc + 8 * kg)
         _check(off, G.PLANE - 7, 8, "B operand LDS read (16 B)")
         bfrag = lds[off[:, None] + np.arange(8)[None, :]]  # [lane][e]
         Am = np.zeros((16, 32))
@@ -253,11 +280,11 @@ def _mfma_brick(lds, G: TileGeom, afrag, ZLO, ZHI, br, bc, edge, gr, gcb, L: Lay
     # lane l holds D[4g+v][n], n = l % 16: output row n, column 4g+v
     o = np.stack([f16(D[4 * g + v, m]) for v in range(4)], axis=1)  # [lane][v]
     if edge:
-        cidx = (16 * br + m + 2) * G.LP + 16 * bc + 4 * g + 2
-        _check(cidx, G.PLANE - 3, 1, "centre read")
         for v in range(4):
+            cidx = G.at(16 * br + m + 2, 16 * bc + 4 * g + 2 + v)
+            _check(cidx, G.PLANE - 1, 1, "centre read")
             keep = ~_interior(gr, gcb + v, L, rs)
-            o[keep, v] = lds[cidx[keep] + v]
+            o[keep, v] = lds[cidx[keep]]
     return o
 
 
@@ -273,7 +300,7 @@ def _store(out, written, L, gr, gcb, o):
 def mfma_sweep(inp, L: Layout, TH, TW, coeff5, rs, round_coeff=f16, per_xcd=0):
     """tcs_mfma_sweep for FP16 (one plane).  coeff5: 5x5 FP32 embedding."""
     assert L.nbr % TH == 0 and L.nbc % TW == 0
-    G = TileGeom(TH, TW)
+    G = TileGeom(TH, TW, strip=True)
     afrag, ZLO, ZHI = _afrag(coeff5, rs, round_coeff)
     out = np.zeros(L.plane)
     written = np.zeros(L.plane, dtype=bool)
@@ -291,11 +318,19 @@ def mfma_sweep(inp, L: Layout, TH, TW, coeff5, rs, round_coeff=f16, per_xcd=0):
     return out, written
 
 
+def _block_brick(BH, BW, base, wave, q):
+    """block_brick of tcs_sweep.hip: the 4 waves split a BH x BW brick region
+    2 x 2; wave w takes the (BH/2) x (BW/2) block at (w // 2, w % 2)."""
+    assert BH % 2 == 0 and BW % 2 == 0
+    hb, wb = BH // 2, BW // 2
+    return base + (wave >> 1) * hb + q // wb, base + (wave & 1) * wb + q % wb
+
+
 def mfma_fused(inp, L: Layout, TH, TW, coeff5, rs, nsweeps, round_coeff=f16):
     """tcs_mfma_fused: nsweeps sweeps per tile in an LDS image of the tile plus a
     one-brick ring and the 2-point frame, rewritten in place between sweeps."""
     assert L.nbr % TH == 0 and L.nbc % TW == 0 and 1 <= nsweeps and rs * (nsweeps - 1) <= 16
-    G = TileGeom(TH + 2, TW + 2)
+    G = TileGeom(TH + 2, TW + 2, strip=True)
     afrag, ZLO, ZHI = _afrag(coeff5, rs, round_coeff)
     out = np.zeros(L.plane)
     written = np.zeros(L.plane, dtype=bool)
@@ -304,26 +339,26 @@ def mfma_fused(inp, L: Layout, TH, TW, coeff5, rs, nsweeps, round_coeff=f16):
     for gr0, gc0 in _tiles(L, TH, TW, 0):
         er0, ec0 = gr0 - 16, gc0 - 16
         lds = stage_tile(inp, L, G, er0, ec0, check=True)
-        assert not np.isnan(lds).any(), "fused image has unwritten LDS words"
+        assert not np.isnan(lds[G.image()]).any(), "fused image has unwritten LDS words"
         edge = not (er0 >= rs and er0 + 16 * BH <= L.rows - rs and ec0 >= rs and ec0 + 16 * BW <= L.cols - rs)
         for sw in range(nsweeps - 1):
             res = {}
             for wave in range(4):
                 for q in range(BH * BW // 4):
-                    b = wave + 4 * q
-                    br, bc = b // BW, b % BW
+                    br, bc = _block_brick(BH, BW, 0, wave, q)
+                    assert (br, bc) not in res, "two waves compute one brick"
                     res[(br, bc)] = _mfma_brick(lds, G, afrag, ZLO, ZHI, br, bc, edge,
                                                 er0 + 16 * br + m, ec0 + 16 * bc + 4 * g, L, rs)
+            assert len(res) == BH * BW
             for (br, bc), o in res.items():
-                d = (16 * br + m + 2) * G.LP + 16 * bc + 4 * g + 2
-                assert np.all(d % 2 == 0)
-                _check(d, G.PLANE - 3, 2, "fused write-back")
-                for v in range(4):
-                    lds[d + v] = o[:, v]
+                for k in range(2):  # two dword writes per lane
+                    d = G.at(16 * br + m + 2, 16 * bc + 4 * g + 2 + 2 * k)
+                    _check(d, G.PLANE - 1, 2, "fused write-back (dword)")
+                    lds[d] = o[:, 2 * k]
+                    lds[d + 1] = o[:, 2 * k + 1]
         for wave in range(4):
             for q in range(TH * TW // 4):
-                b = wave + 4 * q
-                br, bc = 1 + b // TW, 1 + b % TW
+                br, bc = _block_brick(TH, TW, 1, wave, q)
                 gr, gcb = er0 + 16 * br + m, ec0 + 16 * bc + 4 * g
                 o = _mfma_brick(lds, G, afrag, ZLO, ZHI, br, bc, edge, gr, gcb, L, rs)
                 _store(out, written, L, gr, gcb, o)

@@ -70,6 +70,7 @@ def test_layout_and_problem_setup():
     assert (L.nbr, L.nbc, L.sbr, L.sbc, L.align) == (1024, 1024, 1026, 1026, 8)
     assert p.buffer_bytes == 1026 * 1026 * 256 * 2
     assert p.tile == (4, 8) and p.mfma_per_brick == 4
+    assert T.Problem("box2d2r", 64, 64, B, precision="split").mfma_per_brick == 12  # hi*hi, hi*lo, lo*hi
     assert list(p.s.coeff) == list(range(25))
     p = T.Problem("star2d1r", 100, 37, np.arange(9, dtype=np.float32).reshape(3, 3))
     c = np.array(p.s.coeff).reshape(5, 5)
