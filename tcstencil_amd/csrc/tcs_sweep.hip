@@ -554,7 +554,11 @@ template <int TH, int TW, int ZLO, int ZHI, int PREC>
 __device__ __forceinline__ void fused_tile(const SweepArgs &a, const CoefWindows &cw, uint16_t *lds, int gr0,
                                            int gc0, int nsweeps) {
     constexpr int NPL = PREC == TCS_PREC_SPLIT ? 2 : 1;
-    constexpr bool LOOKAHEAD = false;  // registers go to occupancy instead (see tcs_mfma_fused)
+    // Operand look-ahead fits the 4-waves-per-SIMD register budget of the
+    // FP16/BF16 kernels (box2d2r 16384^2, 4x8 tile, 8 fused sweeps: 118 ->
+    // 115 us per sweep; box2d1r 16 fused: 94 -> 90 us); it measured slightly
+    // slower on 4x4 tiles, and SPLIT has no registers left for it.
+    constexpr bool LOOKAHEAD = PREC != TCS_PREC_SPLIT && TH * TW >= 32;
     using G = FusedGeom<TH, TW, NPL, true>;
     constexpr int QA = G::BH * G::BW / 4;  // bricks per wave, whole image
     constexpr int QC = TH * TW / 4;        // bricks per wave, core only
@@ -657,10 +661,9 @@ __device__ __forceinline__ void fused_tile(const SweepArgs &a, const CoefWindows
 }
 
 // Fused sweeps re-read every brick's operands from LDS each sweep (ring
-// included) and run near the LDS read bandwidth; without the operand
-// look-ahead FP16/BF16 kernels fit 4 waves per SIMD, which LDS allows for
-// tiles up to 4x8 (box2d2r 16384^2, 4x8 tile, 8 sweeps fused: 142 -> 134 us
-// per sweep).
+// included); FP16/BF16 kernels are held to 128 VGPRs for 4 waves per SIMD,
+// which LDS allows for tiles up to 4x8 (box2d2r 16384^2, 4x8 tile, 8 sweeps
+// fused: 142 -> 134 us per sweep against 2 waves per SIMD).
 template <int TH, int TW, int ZLO, int ZHI, int PREC>
 __global__ __launch_bounds__(kWG) __attribute__((amdgpu_waves_per_eu(PREC == TCS_PREC_SPLIT ? 1 : 4, 8)))
 void tcs_mfma_fused(SweepArgs a, int nsweeps) {
