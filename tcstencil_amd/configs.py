@@ -95,17 +95,33 @@ def _errors(p, x, refs, scale_exp=0):
     return out
 
 
-def _measure(stencil, rows, cols, B, x, method, precision, sweeps, fuse=1, scale_exp=0, tile=None):
+def _timed_graph(p, b0, b1, sweeps):
+    """ms of one replay of the sweep loop captured as a hipGraph (after one
+    untimed replay)."""
+    torch = _torch()
+    g = p.graph(b0, b1, sweeps)
+    g.launch()
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    g.launch()
+    e1.record()
+    torch.cuda.synchronize()
+    g.destroy()
+    return e0.elapsed_time(e1)
+
+
+def _measure(stencil, rows, cols, B, x, method, precision, sweeps, fuse=1, scale_exp=0, tile=None, graph=False):
     from . import api as T
     p = T.Problem(stencil, rows, cols, B, method=method, precision=precision, fuse=fuse, tile=tile)
     b0, b1 = p.alloc(), p.alloc()
     p.clear(b1)
     p.pack(x, b0, scale_exp)
-    ms, _ = _timed_run(p, b0, b1, sweeps)
+    ms = _timed_graph(p, b0, b1, sweeps) if graph else _timed_run(p, b0, b1, sweeps)[0]
     pts = p.interior_points * sweeps
     gst = pts / (ms * 1e-3) / 1e9
     return p, {"stencil": stencil, "grid": [rows, cols], "method": method, "precision": precision,
-               "fuse": fuse, "tile": list(p.tile), "sweeps": sweeps, "ms": round(ms, 3),
+               "fuse": fuse, "tile": list(p.tile), "sweeps": sweeps, "graph": graph, "ms": round(ms, 3),
                "gstencil_s": round(gst, 1),
                "hbm_gbs": round(gst * BYTES[precision] / fuse, 1),
                "hbm_frac": round(gst * BYTES[precision] / fuse / HBM_PEAK_GBS, 4),
@@ -125,6 +141,11 @@ def c1():
         p, rec = _measure(stencil, N, N, B, x, method, "fp16", 200)
         rec["errors_vs_fp64"] = {1: _errors(p, x, refs)[1]}
         emit(config="c1", **rec)
+    # A 1024^2 sweep is a few microseconds of work: launch-bound on a stream.
+    # The same 200 sweeps as one hipGraph replay, and fused 9 to a launch.
+    for method in ("mfma", "direct"):
+        emit(config="c1", **_measure(stencil, N, N, B, x, method, "fp16", 200, graph=True)[1])
+    emit(config="c1", **_measure(stencil, N, N, B, x, "mfma", "fp16", 198, fuse=9, graph=True)[1])
 
 
 def c2():
