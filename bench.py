@@ -88,7 +88,19 @@ def cpu_baseline(np):
     return {"value": round((N - 4) ** 2 / dt / 1e9, 4), "unit": "GStencil/s", "cores": 1, "kind": kind,
             "sample": f"one box2d2r sweep of {N}x{N} FP32, single thread, {dt:.1f} s "
                       f"({'oracle/_ref/libgold_box2d2r_fast.so' if kind == 'reference' else 'oracle/_build/liboracle.so'}), "
-                      f"host {os.cpu_count()} logical CPUs"}
+                      f"host {os.cpu_count()} logical CPUs, {cpu_model()}"}
+
+
+def cpu_model():
+    """The host CPU's model name (Linux /proc/cpuinfo), or 'unknown CPU'."""
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown CPU"
 
 
 def main():
@@ -193,6 +205,7 @@ def main():
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic},
         "cpu_baseline": cpu,
+        "gpu_clocks": "default power management, not locked (the reference locks clocks, README.md:7)",
     }
     print(json.dumps(line), flush=True)
 
