@@ -184,6 +184,20 @@ int tcs_sweep(const tcs_problem *p, const void *d_in, void *d_out, void *stream)
  * k sweeps each (LDS-resident, bitwise equal to unfused sweeps), balanced so
  * that the launch count has the parity of `sweeps`. */
 int tcs_run(const tcs_problem *p, void *d_buf0, void *d_buf1, int sweeps, void *stream);
+/* Row-major FP32 grid in (d_src) -> `sweeps` sweeps -> row-major FP32 grid out
+ * (d_dst; both rows x cols with leading dimension ld, distinct): bitwise the
+ * result of tcs_pack_f32(d_src -> d_buf0, scale_exp), tcs_run(d_buf0, d_buf1,
+ * sweeps) and tcs_unpack_f32(result -> d_dst, scale_exp, d_ring = d_src).  For
+ * MFMA FP16/BF16 problems with 16x16 fragments, tiles 2x4/4x4/2x8/4x8 and one
+ * tile per workgroup, the pack is fused into the first sweep and the unpack
+ * into the last, so one sweep is one kernel reading and writing FP32 and
+ * touching no storage buffer (d_buf0/d_buf1 may then be NULL); otherwise the
+ * three steps run.  Storage buffers must have a zero ghost ring (tcs_clear).
+ * This is the GPU-resident edge of the reference's host driver: its host-side
+ * conversion, padding and brick layout (layout16/host_box2d2r.cu:131-141) and
+ * the reverse (:196-204), run on the device and fused into the sweeps. */
+int tcs_run_f32(const tcs_problem *p, const float *d_src, float *d_dst, int64_t ld, void *d_buf0, void *d_buf1,
+                int sweeps, int scale_exp, void *stream);
 /* Capture tcs_run into a hipGraphExec (returned as void*) and replay it. */
 int tcs_graph_create(const tcs_problem *p, void *d_buf0, void *d_buf1, int sweeps, void *stream,
                      void **graph_exec);

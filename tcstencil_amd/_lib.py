@@ -91,6 +91,7 @@ SIGNATURES = {
     "tcs_split_scale_exp": (_i, [ctypes.c_double]),
     "tcs_sweep": (_i, [_prob, _vp, _vp, _vp]),
     "tcs_run": (_i, [_prob, _vp, _vp, _i, _vp]),
+    "tcs_run_f32": (_i, [_prob, _vp, _vp, _i64, _vp, _vp, _i, _i, _vp]),
     "tcs_graph_create": (_i, [_prob, _vp, _vp, _i, _vp, _P(_vp)]),
     "tcs_graph_launch": (_i, [_vp, _vp]),
     "tcs_graph_destroy": (_i, [_vp]),

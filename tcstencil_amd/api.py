@@ -176,6 +176,24 @@ class Problem:
         check(lib().tcs_run(self.ref, _ptr(buf0), _ptr(buf1), sweeps, _stream(stream)))
         return buf1 if sweeps % 2 else buf0
 
+    def run_f32(self, src, sweeps: int, out=None, buf0=None, buf1=None, scale_exp: int = 0, stream=None):
+        """Row-major FP32 device grid in -> `sweeps` sweeps -> row-major FP32 out
+        (tcs_run_f32: pack and unpack fused into the first and last sweep where
+        the problem allows).  buf0/buf1: cleared storage buffers, needed unless
+        a fused single sweep runs."""
+        torch = _torch()
+        if src.dim() != 2 or src.shape[0] != self.rows or src.shape[1] != self.cols or src.dtype != torch.float32:
+            raise ValueError(f"run_f32: source must be a {self.rows}x{self.cols} float32 grid")
+        if out is None:
+            out = torch.empty_like(src)
+        if out.stride(0) != src.stride(0) or out.stride(1) != 1 or src.stride(1) != 1:
+            raise ValueError("run_f32: source and output need the same row stride and unit column stride")
+        b0 = _ptr(buf0) if buf0 is not None else None
+        b1 = _ptr(buf1) if buf1 is not None else None
+        check(lib().tcs_run_f32(self.ref, _ptr(src), _ptr(out), src.stride(0), b0, b1, sweeps, scale_exp,
+                                _stream(stream)))
+        return out
+
     def graph(self, buf0, buf1, sweeps: int, stream=None) -> "Graph":
         h = ctypes.c_void_p()
         check(lib().tcs_graph_create(self.ref, _ptr(buf0), _ptr(buf1), sweeps, _stream(stream),

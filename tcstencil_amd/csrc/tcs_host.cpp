@@ -322,6 +322,34 @@ extern "C" int tcs_run(const tcs_problem *p, void *d_buf0, void *d_buf1, int swe
     return TCS_OK;
 }
 
+// Row-major FP32 in and out.  Fused edges (pack into the first sweep, unpack
+// into the last) where an edge kernel exists; otherwise the three steps.
+extern "C" int tcs_run_f32(const tcs_problem *p, const float *d_src, float *d_dst, int64_t ld, void *d_buf0,
+                           void *d_buf1, int sweeps, int scale_exp, void *stream) {
+    if (!p) return set_error(TCS_ERR_INVALID, "run_f32: null problem");
+    if (!d_src || !d_dst) return set_error(TCS_ERR_INVALID, "run_f32: null source or destination");
+    if (d_src == d_dst) return set_error(TCS_ERR_INVALID, "run_f32: source and destination must differ");
+    if (sweeps < 0) return set_error(TCS_ERR_INVALID, "run_f32: negative sweep count");
+    if (ld < p->layout.cols)
+        return set_error(TCS_ERR_INVALID, "run_f32: ld %lld < cols %d", (long long)ld, p->layout.cols);
+    const hipStream_t s = (hipStream_t)stream;
+    if (sweeps >= 1 && edge_supported(p)) {
+        if (sweeps == 1) return launch_edge(p, 3, nullptr, nullptr, d_src, d_dst, ld, scale_exp, s);
+        if (!d_buf0 || !d_buf1 || d_buf0 == d_buf1)
+            return set_error(TCS_ERR_INVALID, "run_f32: %d sweeps need two distinct storage buffers", sweeps);
+        TCS_TRY(launch_edge(p, 1, nullptr, d_buf0, d_src, nullptr, ld, scale_exp, s));
+        // Sweeps 2 .. sweeps-1 ping-pong from d_buf0; the last reads the result.
+        TCS_TRY(tcs_run(p, d_buf0, d_buf1, sweeps - 2, stream));
+        void *last = (sweeps - 2) % 2 ? d_buf1 : d_buf0;
+        return launch_edge(p, 2, last, nullptr, d_src, d_dst, ld, scale_exp, s);
+    }
+    if (!d_buf0 || !d_buf1 || d_buf0 == d_buf1)
+        return set_error(TCS_ERR_INVALID, "run_f32: this problem needs two distinct storage buffers");
+    TCS_TRY(tcs_pack_f32(p, d_src, ld, d_buf0, scale_exp, stream));
+    TCS_TRY(tcs_run(p, d_buf0, d_buf1, sweeps, stream));
+    return tcs_unpack_f32(p, sweeps % 2 ? d_buf1 : d_buf0, d_dst, ld, scale_exp, d_src, stream);
+}
+
 extern "C" int tcs_graph_create(const tcs_problem *p, void *d_buf0, void *d_buf1, int sweeps, void *stream,
                                 void **graph_exec) {
     if (!graph_exec) return set_error(TCS_ERR_INVALID, "graph: null output");
@@ -474,12 +502,6 @@ static int staged_d2h(void *h_dst, const void *d_src, size_t bytes, hipStream_t 
 }
 
 }  // namespace
-
-#define TCS_TRY(expr)          \
-    do {                       \
-        int _st = (expr);      \
-        if (_st) return _st;   \
-    } while (0)
 
 static int host_sweep_impl(int stencil, int method, int precision, const float *h_A, const float *h_B, int n,
                            float *h_C, int rows, int cols, int sweeps, const tcs_params *params, float *ms_rep,

@@ -52,6 +52,14 @@ struct SweepArgs {
     float out_scale;       // SPLIT: 2^-coeff_exp
     float cf[25];          // 5x5 FP32 coefficient embedding (direct kernel)
     CoefWindows cw;        // MFMA operand windows (FP16/BF16 taps, or SPLIT hi and lo)
+    // Row-major FP32 edges (tcs_mfma_edge): the first sweep reads rm_src instead
+    // of bricks, the last writes rm_dst (ring points copied from rm_src).
+    const float *rm_src;
+    float *rm_dst;
+    int64_t rm_ld;         // leading dimension of rm_src and rm_dst, elements
+    float rm_scale_in;     // 2^scale_exp, applied before rounding to storage format
+    float rm_scale_out;    // 2^-scale_exp, applied to stored values on output
+    int32_t rm_vec;        // 1: rm_src, rm_dst and rm_ld allow 16-byte accesses
 };
 
 // One problem of a batch, device-resident (tcs_batch.cpp builds them).
@@ -78,6 +86,12 @@ int launch_batch(int precision, int tile_h, int tile_w, int frag, const BatchPro
                  hipStream_t stream);
 const char *batch_kernel_name(int precision, int tile_h, int tile_w, int frag, int nsweeps, bool persist);
 int launch_sweep(const tcs_problem *p, const void *d_in, void *d_out, hipStream_t stream);
+// Edge sweeps for tcs_run_f32: mode 1 reads row-major FP32 src and writes
+// storage d_out, mode 2 reads storage d_in and writes row-major FP32 dst (ring
+// from src), mode 3 both.  edge_supported: a kernel exists for the problem.
+bool edge_supported(const tcs_problem *p);
+int launch_edge(const tcs_problem *p, int mode, const void *d_in, void *d_out, const float *src, float *dst,
+                int64_t ld, int scale_exp, hipStream_t stream);
 int launch_fused(const tcs_problem *p, const void *d_in, void *d_out, int nsweeps, hipStream_t stream);
 const char *sweep_kernel_name(const tcs_problem *p);
 const char *fused_kernel_name(const tcs_problem *p);
@@ -97,6 +111,13 @@ float f16_bits_to_f32(uint16_t h);
 void build_coef_windows(const tcs_problem *p, CoefWindows *cw);
 
 }  // namespace tcs
+
+// Return a non-zero status from the enclosing function.
+#define TCS_TRY(expr)          \
+    do {                       \
+        int _st = (expr);      \
+        if (_st) return _st;   \
+    } while (0)
 
 #define TCS_HIP_CHECK(expr)                                          \
     do {                                                             \

@@ -41,6 +41,16 @@ __device__ __forceinline__ void store8(uint16_t *p, u32x2 w) {
         *reinterpret_cast<u32x2 *>(p) = w;
 }
 
+__device__ __forceinline__ uint16_t f16_bits(float v) {
+    return __builtin_bit_cast(uint16_t, static_cast<_Float16>(v));
+}
+__device__ __forceinline__ uint16_t bf16_bits(float v) {
+    return __builtin_bit_cast(uint16_t, static_cast<__bf16>(v));
+}
+__device__ __forceinline__ uint32_t pack2(uint16_t lo, uint16_t hi) {
+    return (uint32_t)lo | ((uint32_t)hi << 16);
+}
+
 // ------------------------------------------------------------ tile staging
 // LDS image of BH x BW data bricks plus a 2-point frame on every side: element
 // (r, c), r < LR, c < LC, where columns [16*BW+4, LC) hold zeros.  Two layouts:
@@ -147,6 +157,69 @@ __device__ __forceinline__ void load_tile(const SweepArgs &a, TileRegs<G> &R, in
     }
 }
 
+// Storage-format bits of a row-major FP32 value (times rm_scale_in), rounded
+// to nearest as the pack kernel rounds: FP16 or BF16.
+template <int PREC>
+__device__ __forceinline__ uint16_t rm_encode(const SweepArgs &a, float x) {
+    x *= a.rm_scale_in;
+    if constexpr (PREC == TCS_PREC_BF16)
+        return __builtin_bit_cast(uint16_t, static_cast<__bf16>(x));
+    else
+        return __builtin_bit_cast(uint16_t, static_cast<_Float16>(x));
+}
+
+// load_tile from the row-major FP32 grid a.rm_src instead of bricks (FP16/BF16,
+// one plane): the same registers, holding what the pack kernel would have
+// stored -- the rounded value inside the grid, zero for padding and for frame
+// points outside it.  Data chunks (8 elements of a row) use two 16-byte loads
+// when a.rm_vec allows and the chunk lies inside the grid.
+template <class G, int PREC>
+__device__ __forceinline__ void load_tile_rm(const SweepArgs &a, TileRegs<G> &R, int r0, int c0, int tid) {
+    static_assert(G::NPL == 1, "row-major edges: FP16/BF16 storage");
+#pragma unroll
+    for (int i = 0; i < G::CPT; ++i) {
+        const int c = tid + i * kWG;
+        if (G::NCH % kWG == 0 || c < G::NCH) {
+            const int b = c >> 5, w = c & 31;
+            const int br = b / G::BW, bc = b - br * G::BW;
+            const int gr = r0 + 16 * br + (w >> 1), gc = c0 + 16 * bc + 8 * (w & 1);  // both >= 0
+            float x[8];
+            if (gr < a.rows && a.rm_vec && gc + 8 <= a.cols) {
+                const float *src = a.rm_src + gr * a.rm_ld + gc;
+                const f32x4 u = *reinterpret_cast<const f32x4 *>(src);
+                const f32x4 v = *reinterpret_cast<const f32x4 *>(src + 4);
+#pragma unroll
+                for (int e = 0; e < 4; ++e) {
+                    x[e] = u[e];
+                    x[4 + e] = v[e];
+                }
+            } else {
+#pragma unroll
+                for (int e = 0; e < 8; ++e)
+                    x[e] = gr < a.rows && gc + e < a.cols ? a.rm_src[gr * a.rm_ld + gc + e] : 0.0f;
+            }
+#pragma unroll
+            for (int q = 0; q < 4; ++q)
+                R.cv[0][i][q] = pack2(rm_encode<PREC>(a, x[2 * q]), rm_encode<PREC>(a, x[2 * q + 1]));
+        }
+    }
+#pragma unroll
+    for (int i = 0; i < G::HPT; ++i) {
+        const int h = tid + i * kWG;
+        if (h < G::NH) {
+            int lr, ld;
+            halo_coords<G>(h, lr, ld);
+            const int gr = r0 - 2 + lr, gc = c0 - 2 + 2 * ld;  // gc even; either may be negative
+            const bool row_in = (unsigned)gr < (unsigned)a.rows;
+            float x[2];
+#pragma unroll
+            for (int e = 0; e < 2; ++e)
+                x[e] = row_in && (unsigned)(gc + e) < (unsigned)a.cols ? a.rm_src[gr * a.rm_ld + gc + e] : 0.0f;
+            R.hv[0][i] = pack2(rm_encode<PREC>(a, x[0]), rm_encode<PREC>(a, x[1]));
+        }
+    }
+}
+
 template <class G>
 __device__ __forceinline__ void write_tile(const TileRegs<G> &R, uint16_t *lds, int tid) {
     uint32_t *l32 = reinterpret_cast<uint32_t *>(lds);
@@ -230,15 +303,6 @@ __device__ __forceinline__ void tile_origin(const SweepArgs &a, int v, int &gr0,
     gc0 = 16 * TW * tx;
 }
 
-__device__ __forceinline__ uint16_t f16_bits(float v) {
-    return __builtin_bit_cast(uint16_t, static_cast<_Float16>(v));
-}
-__device__ __forceinline__ uint16_t bf16_bits(float v) {
-    return __builtin_bit_cast(uint16_t, static_cast<__bf16>(v));
-}
-__device__ __forceinline__ uint32_t pack2(uint16_t lo, uint16_t hi) {
-    return (uint32_t)lo | ((uint32_t)hi << 16);
-}
 
 // ------------------------------------------------------------- MFMA kernel
 template <int PREC>
@@ -466,12 +530,42 @@ constexpr bool kLookahead = PREC != TCS_PREC_SPLIT;
 template <int TH, int TW, int PREC, bool PERSIST>
 constexpr bool kHighOccupancy = PREC != TCS_PREC_SPLIT && TH * TW < 64 && !PERSIST;
 
-template <int TH, int TW, int ZLO, int ZHI, int PREC, bool PERSIST>
-__global__ __launch_bounds__(kWG)
-__attribute__((amdgpu_waves_per_eu(kHighOccupancy<TH, TW, PREC, PERSIST> ? 8 : 1, 8)))
-void tcs_mfma_sweep(SweepArgs a) {
+// Row-major FP32 output of one lane's four results (row gr, columns gcb..+3,
+// FP16/BF16 bits in w): what the unpack kernel writes -- the stored value
+// times rm_scale_out at interior points, the caller's rm_src value on the
+// ring; nothing outside the grid.
+template <int PREC>
+__device__ __forceinline__ void store_rm(const SweepArgs &a, int gr, int gcb, u32x2 w) {
+    if (gr >= a.rows) return;
+    float v[4];
+#pragma unroll
+    for (int e = 0; e < 4; ++e) {
+        const uint16_t bits = (uint16_t)(w[e >> 1] >> (16 * (e & 1)));
+        const float x = PREC == TCS_PREC_BF16 ? static_cast<float>(__builtin_bit_cast(__bf16, bits))
+                                              : static_cast<float>(__builtin_bit_cast(_Float16, bits));
+        v[e] = x * a.rm_scale_out;
+    }
+    const int64_t row = gr * a.rm_ld;
+#pragma unroll
+    for (int e = 0; e < 4; ++e)
+        if (gcb + e < a.cols && !point_is_interior(a, gr, gcb + e)) v[e] = a.rm_src[row + gcb + e];
+    if (a.rm_vec && gcb + 4 <= a.cols) {
+        *reinterpret_cast<f32x4 *>(a.rm_dst + row + gcb) = f32x4{v[0], v[1], v[2], v[3]};
+    } else {
+#pragma unroll
+        for (int e = 0; e < 4; ++e)
+            if (gcb + e < a.cols) a.rm_dst[row + gcb + e] = v[e];
+    }
+}
+
+// One sweep of TH x TW-brick tiles (tcs_mfma_sweep), or with MODE bit 1 the
+// input read from the row-major FP32 grid a.rm_src and with MODE bit 2 the
+// output written to the row-major FP32 grid a.rm_dst (tcs_mfma_edge).
+template <int TH, int TW, int ZLO, int ZHI, int PREC, bool PERSIST, int MODE>
+__device__ __forceinline__ void sweep_tile16(const SweepArgs &a) {
     constexpr int NPL = PREC == TCS_PREC_SPLIT ? 2 : 1;
     constexpr bool LOOKAHEAD = kLookahead<PREC> && !kHighOccupancy<TH, TW, PREC, PERSIST>;
+    static_assert(MODE == 0 || (!PERSIST && NPL == 1), "row-major edges: one tile per workgroup, FP16/BF16");
     using T = Tile<TH, TW, NPL, true>;
     __shared__ __attribute__((aligned(16))) uint16_t lds[NPL * T::PLANE];
 
@@ -481,7 +575,10 @@ void tcs_mfma_sweep(SweepArgs a) {
     int gr0, gc0;
     tile_origin<TH, TW>(a, sc.tile, gr0, gc0);
     TileRegs<T> R;
-    load_tile<T, false>(a, R, gr0, gc0, tid);
+    if constexpr (MODE & 1)
+        load_tile_rm<T, PREC>(a, R, gr0, gc0, tid);
+    else
+        load_tile<T, false>(a, R, gr0, gc0, tid);
     zero_pad<T>(lds, tid);
 
     const int lane = tid & 63, wave = tid >> 6;
@@ -525,9 +622,13 @@ void tcs_mfma_sweep(SweepArgs a) {
             const int gr = cgr0 + 16 * br + m, gcb = cgc0 + 16 * bc + 4 * g;
             u32x2 w[NPL];
             mfma_brick<T, ZLO, ZHI, PREC>(a, lds, C, cur, br, bc, m, g, edge, gr, gcb, w);
-            const int64_t s = stor_index(gr, gcb, a.sbc);
+            if constexpr (MODE & 2) {
+                store_rm<PREC>(a, gr, gcb, w[0]);
+            } else {
+                const int64_t s = stor_index(gr, gcb, a.sbc);
 #pragma unroll
-            for (int p = 0; p < NPL; ++p) store8<true>(a.out + p * a.plane + s, w[p]);
+                for (int p = 0; p < NPL; ++p) store8<true>(a.out + p * a.plane + s, w[p]);
+            }
             if constexpr (LOOKAHEAD) cur = nxt;
         }
         // One-tile-per-workgroup launches cover every tile with the grid itself.
@@ -535,6 +636,24 @@ void tcs_mfma_sweep(SweepArgs a) {
         sc.tile = next;
         __syncthreads();  // every wave is done reading LDS before it is overwritten
     }
+}
+
+template <int TH, int TW, int ZLO, int ZHI, int PREC, bool PERSIST>
+__global__ __launch_bounds__(kWG)
+__attribute__((amdgpu_waves_per_eu(kHighOccupancy<TH, TW, PREC, PERSIST> ? 8 : 1, 8)))
+void tcs_mfma_sweep(SweepArgs a) {
+    sweep_tile16<TH, TW, ZLO, ZHI, PREC, PERSIST, 0>(a);
+}
+
+// The first and last sweep of tcs_run_f32 with the layout transforms fused in
+// (MODE 1: row-major FP32 in, storage out; 2: storage in, row-major FP32 out;
+// 3: both, a whole one-sweep run in one kernel).  Results are bitwise those of
+// tcs_pack_f32 / tcs_mfma_sweep / tcs_unpack_f32.
+template <int TH, int TW, int ZLO, int ZHI, int PREC, int MODE>
+__global__ __launch_bounds__(kWG)
+__attribute__((amdgpu_waves_per_eu(kHighOccupancy<TH, TW, PREC, false> ? 8 : 1, 8)))
+void tcs_mfma_edge(SweepArgs a) {
+    sweep_tile16<TH, TW, ZLO, ZHI, PREC, false, MODE>(a);
 }
 
 // Temporal blocking: `nsweeps` sweeps (1 <= nsweeps, radius * (nsweeps - 1) <= 16)
@@ -1393,6 +1512,11 @@ static int make_args(const tcs_problem *p, const void *d_in, void *d_out, SweepA
     a.out_scale = ldexpf(1.0f, -p->coeff_exp);
     for (int i = 0; i < 25; ++i) a.cf[i] = p->coeff[i];
     build_coef_windows(p, &a.cw);
+    a.rm_src = nullptr;
+    a.rm_dst = nullptr;
+    a.rm_ld = 0;
+    a.rm_scale_in = a.rm_scale_out = 1.0f;
+    a.rm_vec = 0;
     return TCS_OK;
 }
 
@@ -1408,6 +1532,52 @@ int launch_fused(const tcs_problem *p, const void *d_in, void *d_out, int nsweep
     int st = make_args(p, d_in, d_out, a);
     if (st) return st;
     hipLaunchKernelGGL(k->fn, dim3((unsigned)a.nvirt), dim3(kWG), 0, stream, a, nsweeps);
+    TCS_HIP_CHECK(hipGetLastError());
+    return TCS_OK;
+}
+
+// Edge kernels (tcs_run_f32): FP16 and BF16, 16x16 fragments, the tiles the
+// default rule picks (2x4, 4x4, 2x8, 4x8), one tile per workgroup.
+struct EdgeEntry {
+    int prec, radius, th, tw, mode;
+    sweep_fn fn;
+};
+#define TCS_EDGE(TH, TW, ZLO, ZHI, P, R, M) {P, R, TH, TW, M, tcs_mfma_edge<TH, TW, ZLO, ZHI, P, M>}
+#define TCS_EDGE_MODES(TH, TW, ZLO, ZHI, P, R) \
+    TCS_EDGE(TH, TW, ZLO, ZHI, P, R, 1), TCS_EDGE(TH, TW, ZLO, ZHI, P, R, 2), TCS_EDGE(TH, TW, ZLO, ZHI, P, R, 3)
+#define TCS_EDGE_TILES(ZLO, ZHI, P, R)                                                                   \
+    TCS_EDGE_MODES(2, 4, ZLO, ZHI, P, R), TCS_EDGE_MODES(4, 4, ZLO, ZHI, P, R), TCS_EDGE_MODES(2, 8, ZLO, ZHI, P, R), \
+        TCS_EDGE_MODES(4, 8, ZLO, ZHI, P, R)
+static const EdgeEntry kEdge[] = {
+    TCS_EDGE_TILES(0, 4, 0, 2), TCS_EDGE_TILES(1, 3, 0, 1), TCS_EDGE_TILES(0, 4, 1, 2), TCS_EDGE_TILES(1, 3, 1, 1),
+};
+
+static const EdgeEntry *select_edge(const tcs_problem *p, int mode) {
+    if (p->method != TCS_METHOD_MFMA || frag_of(p) != 16 || p->wg_per_cu > 0) return nullptr;
+    for (const EdgeEntry &k : kEdge)
+        if (k.prec == p->precision && k.radius == p->radius && k.th == p->tile_h && k.tw == p->tile_w &&
+            k.mode == mode)
+            return &k;
+    return nullptr;
+}
+
+bool edge_supported(const tcs_problem *p) { return select_edge(p, 3) != nullptr; }
+
+int launch_edge(const tcs_problem *p, int mode, const void *d_in, void *d_out, const float *src, float *dst,
+                int64_t ld, int scale_exp, hipStream_t stream) {
+    const EdgeEntry *k = select_edge(p, mode);
+    if (!k) return set_error(TCS_ERR_UNSUPPORTED, "no row-major edge kernel for this problem");
+    SweepArgs a;
+    int st = make_args(p, d_in, d_out, a);
+    if (st) return st;
+    a.rm_src = src;
+    a.rm_dst = dst;
+    a.rm_ld = ld;
+    a.rm_scale_in = ldexpf(1.0f, scale_exp);
+    a.rm_scale_out = ldexpf(1.0f, -scale_exp);
+    a.rm_vec = ld % 4 == 0 && (reinterpret_cast<uintptr_t>(src) & 15) == 0 &&
+               (reinterpret_cast<uintptr_t>(dst) & 15) == 0;
+    hipLaunchKernelGGL(k->fn, dim3((unsigned)a.nvirt), dim3(kWG), 0, stream, a);
     TCS_HIP_CHECK(hipGetLastError());
     return TCS_OK;
 }

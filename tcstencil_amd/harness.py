@@ -12,6 +12,11 @@ FLAG_DICT script/param.py:1-6), run each under `ncu` and scrape "Duration"
       ceiling, and the calibration kernel for FETCH_SIZE / WRITE_SIZE).
       Prints one JSON line.
 
+  python -m tcstencil_amd.harness edges --stencil box2d2r --size 16384 --sweeps 1
+      row-major FP32 in -> sweeps -> row-major FP32 out: tcs_run_f32 (pack and
+      unpack fused into the first and last sweep) against the three steps
+      pack / run / unpack, HIP-event time per call.  Prints one JSON line.
+
   python -m tcstencil_amd.harness sweep --sizes 1024 4096 16384 --out results.csv [--rocprof]
       every (stencil, method, size, tile) in the cross product; with --rocprof
       each workload runs in its own `rocprofv3 --kernel-trace --stats` process
@@ -88,6 +93,48 @@ def run_workload(stencil, rows, cols, method="mfma", precision="fp16", tile=None
     return out
 
 
+def run_edges(stencil, rows, cols, precision="fp16", sweeps=1, reps=20, seed=1):
+    """ms per call of tcs_run_f32 (fused edges) and of pack + run + unpack on
+    the same row-major FP32 grid, and whether their outputs are bitwise equal."""
+    import torch
+
+    from . import api as T
+
+    B = T.gen_coeffs(stencil, seed)
+    p = T.Problem(stencil, rows, cols, B, precision=precision)
+    x = T.gen_uniform(rows, cols, seed=seed, dtype=torch.float32)
+    y0, y1 = torch.empty_like(x), torch.empty_like(x)
+    b0, b1 = p.alloc(), p.alloc()
+    p.clear(b0)
+    p.clear(b1)
+
+    def fused():
+        p.run_f32(x, sweeps, out=y0, buf0=b0, buf1=b1)
+
+    def three():
+        p.pack(x, b0)
+        p.unpack(p.run(b0, b1, sweeps), out=y1, ring=x)
+
+    ev = [torch.cuda.Event(enable_timing=True) for _ in range(2)]
+    res = {"stencil": stencil, "rows": rows, "cols": cols, "precision": precision, "sweeps": sweeps,
+           "tile": list(p.tile)}
+    for name, fn in (("fused_edges", fused), ("three_steps", three)):
+        for _ in range(3):
+            fn()
+        torch.cuda.synchronize()
+        ev[0].record()
+        for _ in range(reps):
+            fn()
+        ev[1].record()
+        torch.cuda.synchronize()
+        ms = ev[0].elapsed_time(ev[1]) / reps
+        res[name] = {"ms": ms, "gstencil_s": p.interior_points * sweeps / (ms * 1e-3) / 1e9,
+                     "fp32_io_gbs": 8 * rows * cols / (ms * 1e-3) / 1e9}
+    res["bitwise_equal"] = bool(torch.equal(y0, y1))
+    res["speedup"] = res["three_steps"]["ms"] / res["fused_edges"]["ms"]
+    return res
+
+
 def rocprof_kernel_avg_ns(cmd, kernel_prefix, workdir):
     """Run `cmd` under rocprofv3 --kernel-trace --stats; return (average ns, calls)
     of the kernels whose name contains kernel_prefix."""
@@ -123,6 +170,13 @@ def main(argv=None):
     r.add_argument("--fuse", type=int, default=1)
     r.add_argument("--wg-per-cu", type=int, default=0)
     r.add_argument("--frag", type=int, default=16, choices=(16, 32))
+    e = sub.add_parser("edges")
+    e.add_argument("--stencil", default="box2d2r")
+    e.add_argument("--size", type=int, default=16384)
+    e.add_argument("--cols", type=int, default=0)
+    e.add_argument("--precision", default="fp16")
+    e.add_argument("--sweeps", type=int, default=1)
+    e.add_argument("--reps", type=int, default=20)
     s = sub.add_parser("sweep")
     s.add_argument("--stencils", nargs="+", default=["star2d1r", "star2d2r", "box2d1r", "box2d2r"])
     s.add_argument("--methods", nargs="+", default=["mfma", "direct"])
@@ -141,6 +195,10 @@ def main(argv=None):
                            tuple(a.tile) if a.tile else None, a.sweeps, a.warmup, a.copy_mb,
                            fuse=a.fuse, wg_per_cu=a.wg_per_cu, frag=a.frag)
         print(json.dumps(res), flush=True)
+        return 0
+
+    if a.cmd == "edges":
+        print(json.dumps(run_edges(a.stencil, a.size, a.cols or a.size, a.precision, a.sweeps, a.reps)), flush=True)
         return 0
 
     rows = []

@@ -318,6 +318,113 @@ def mfma_sweep(inp, L: Layout, TH, TW, coeff5, rs, round_coeff=f16, per_xcd=0):
     return out, written
 
 
+# ------------------------------------------- row-major FP32 edges (tcs_mfma_edge)
+def stage_tile_rm(src, rows, cols, ld, G: TileGeom, gr0: int, gc0: int, vec: bool) -> np.ndarray:
+    """load_tile_rm + write_tile + zero_pad: the LDS image read from the
+    row-major grid src (flat, rows x cols with leading dimension ld).  Every
+    load is checked against [0, rows*ld) and its row; vector loads (two 16-byte
+    loads per 8-element chunk, when vec and the chunk lies inside the grid)
+    also for 16-byte alignment of the element index."""
+    n = rows * ld
+    lds = np.full(G.PLANE, np.nan)
+    tid = np.arange(WG, dtype=np.int64)
+    TW, TH = G.TW, G.TH
+    for i in range(G.CPT):
+        c = tid + i * WG
+        c = c[c < G.NCH]
+        b, w = c >> 5, c & 31
+        br, bc = b // TW, b % TW
+        gr, gc = gr0 + 16 * br + (w >> 1), gc0 + 16 * bc + 8 * (w & 1)
+        x = np.zeros((c.size, 8))
+        vecok = (gr < rows) & (gc + 8 <= cols) & vec
+        idx = gr[vecok] * ld + gc[vecok]
+        _check(idx, n - 7, 4, "row-major vector load (2 x 16 B)")
+        assert np.all(gc[vecok] + 7 < cols), "vector load leaves its row"
+        x[vecok] = src[idx[:, None] + np.arange(8)[None, :]]
+        for e in range(8):
+            ok = ~vecok & (gr < rows) & (gc + e < cols)
+            ie = gr[ok] * ld + gc[ok] + e
+            _check(ie, n, 1, "row-major load")
+            x[ok, e] = src[ie]
+        lr, lc = 16 * br + (w >> 1) + 2, 16 * bc + 8 * (w & 1) + 2
+        for q in range(4):
+            e0 = G.at(lr, lc + 2 * q)
+            lds[e0] = f16(x[:, 2 * q])
+            lds[e0 + 1] = f16(x[:, 2 * q + 1])
+    for i in range(G.HPT):
+        h = tid + i * WG
+        h = h[h < G.NH]
+        lr = np.empty_like(h)
+        ldw = np.empty_like(h)
+        m1 = h < 2 * G.HD
+        lr[m1], ldw[m1] = h[m1] // G.HD, h[m1] % G.HD
+        m2 = (~m1) & (h < 4 * G.HD)
+        q = h[m2] - 2 * G.HD
+        lr[m2], ldw[m2] = 16 * TH + 2 + q // G.HD, q % G.HD
+        m3 = (~m1) & (~m2) & (h < 4 * G.HD + 16 * TH)
+        lr[m3], ldw[m3] = 2 + (h[m3] - 4 * G.HD), 0
+        m4 = ~(m1 | m2 | m3)
+        lr[m4], ldw[m4] = 2 + (h[m4] - 4 * G.HD - 16 * TH), G.HD - 1
+        gr, gc = gr0 - 2 + lr, gc0 - 2 + 2 * ldw
+        d = G.at(lr, 2 * ldw)
+        for e in range(2):
+            ok = (gr >= 0) & (gr < rows) & (gc + e >= 0) & (gc + e < cols)
+            ie = gr[ok] * ld + gc[ok] + e
+            _check(ie, n, 1, "row-major halo load")
+            v = np.zeros(h.size)
+            v[ok] = src[ie]
+            lds[d + e] = f16(v)
+    for zc in range(2 * G.LR):
+        d = int(G.at(zc >> 1, 16 * TW + 4 + 2 * (zc & 1)))
+        lds[d] = lds[d + 1] = 0.0
+    return lds
+
+
+def mfma_edge(mode, src, rows, cols, ld, L: Layout, TH, TW, coeff5, rs, vec, inp=None):
+    """tcs_mfma_edge (FP16): mode 1 row-major src -> bricks, 2 bricks (inp) ->
+    row-major, 3 both.  Returns (bricks, written) or (row-major flat, written);
+    every row-major store is bounds- and alignment-checked."""
+    assert L.nbr % TH == 0 and L.nbc % TW == 0
+    G = TileGeom(TH, TW, strip=True)
+    afrag, ZLO, ZHI = _afrag(coeff5, rs)
+    n = rows * ld
+    out = np.zeros(n if mode & 2 else L.plane)
+    written = np.zeros(out.size, dtype=bool)
+    m, g = np.arange(64) & 15, np.arange(64) >> 4
+    for gr0, gc0 in _tiles(L, TH, TW, 0):
+        lds = stage_tile_rm(src, rows, cols, ld, G, gr0, gc0, vec) if mode & 1 else stage_tile(inp, L, G, gr0, gc0)
+        edge = not (gr0 >= rs and gr0 + 16 * TH <= L.rows - rs and gc0 >= rs and gc0 + 16 * TW <= L.cols - rs)
+        for wave in range(4):
+            for q in range(TH * TW // 4):
+                b = wave + 4 * q
+                br, bc = b // TW, b % TW
+                gr, gcb = gr0 + 16 * br + m, gc0 + 16 * bc + 4 * g
+                o = _mfma_brick(lds, G, afrag, ZLO, ZHI, br, bc, edge, gr, gcb, L, rs)
+                if not mode & 2:
+                    _store(out, written, L, gr, gcb, o)
+                    continue
+                for l in range(64):  # store_rm, lane by lane
+                    if gr[l] >= rows:
+                        continue
+                    v = o[l].copy()
+                    for e in range(4):
+                        c_ = gcb[l] + e
+                        if c_ < cols and not _interior(gr[l], c_, L, rs):
+                            v[e] = src[gr[l] * ld + c_]
+                    base = gr[l] * ld + gcb[l]
+                    if vec and gcb[l] + 4 <= cols:
+                        _check(np.array([base]), n - 3, 4, "row-major vector store (16 B)")
+                        es = range(4)
+                    else:
+                        es = [e for e in range(4) if gcb[l] + e < cols]
+                        _check(np.array([base + e for e in es], dtype=np.int64), n, 1, "row-major store")
+                    for e in es:
+                        assert not written[base + e], "two lanes store the same element"
+                        out[base + e] = v[e]
+                        written[base + e] = True
+    return out, written
+
+
 def _block_brick(BH, BW, base, wave, q):
     """block_brick of tcs_sweep.hip: the 4 waves split a BH x BW brick region
     2 x 2; wave w takes the (BH/2) x (BW/2) block at (w // 2, w % 2)."""

@@ -203,3 +203,36 @@ def test_emulated_frag32_fused_equals_repeated_sweeps(oracle, stencil, case):
     for _ in range(k):
         ref = E.f16(oracle.sweep_f64(stencil, ref, B))
     assert np.array_equal(E.unpack(fused, L), ref)
+
+
+EDGE_CASES = [(70, 150, 150, 2, 4), (70, 152, 152, 4, 8), (33, 17, 20, 2, 4), (64, 128, 132, 2, 8), (5, 3, 4, 4, 4),
+              (40, 61, 64, 4, 4)]
+
+
+@pytest.mark.parametrize("stencil", STENCILS)
+@pytest.mark.parametrize("case", EDGE_CASES, ids=lambda c: "%dx%d_ld%d_t%dx%d" % c)
+def test_emulated_rowmajor_edges(stencil, case):
+    """tcs_mfma_edge: the row-major FP32 loader and store reproduce pack ->
+    sweep -> unpack exactly; columns past `cols` in each row (the leading-
+    dimension gap) hold NaN, so any read there shows."""
+    rows, cols, ld, TH, TW = case
+    r = RADIUS[stencil]
+    A, _, c5 = dyadic_problem(stencil, rows, cols, seed=rows + 7 * cols)
+    src = np.full((rows, ld), np.nan)
+    src[:, :cols] = A
+    src = src.ravel()
+    vec = ld % 4 == 0
+    L = E.Layout(rows, cols, max(TH, TW))
+    st = E.pack(A, L)
+    bricks, bw = E.mfma_sweep(st, L, TH, TW, c5, r)
+    want = E.unpack(bricks, L, ring=A, rs=r)
+    inside = np.zeros((rows, ld), dtype=bool)
+    inside[:, :cols] = True
+    for mode in (1, 2, 3):
+        out, written = E.mfma_edge(mode, src, rows, cols, ld, L, TH, TW, c5, r, vec, inp=st)
+        if mode == 1:
+            assert np.array_equal(written, bw)
+            assert np.array_equal(out[bw], bricks[bw])
+        else:
+            assert np.array_equal(written.reshape(rows, ld), inside)
+            assert np.array_equal(out.reshape(rows, ld)[:, :cols], want)

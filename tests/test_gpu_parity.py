@@ -665,3 +665,62 @@ def test_c1_star2d1r_1024_vs_gold(T, torch, oracle):
         assert np.array_equal(C[0], gold[0]) and np.array_equal(C[:, -1], gold[:, -1])
         # coefficient rounding (MFMA) + output rounding, u = 2^-11, |x| <= 1
         assert err.max() <= (2.0 if method == "mfma" else 1.0) * 2.0 ** -11 + 1e-6
+
+
+def three_steps(T, torch, p, src, sweeps, scale_exp=0):
+    """tcs_pack_f32 -> tcs_run -> tcs_unpack_f32 (ring from the source)."""
+    b0, b1 = p.alloc(), p.alloc()
+    p.clear(b0)
+    p.clear(b1)
+    p.pack(src, b0, scale_exp)
+    return p.unpack(p.run(b0, b1, sweeps), scale_exp=scale_exp, ring=src)
+
+
+@pytest.mark.parametrize("precision", ["fp16", "bf16"])
+def test_run_f32_fused_edges_bitwise(T, torch, precision):
+    """tcs_run_f32 with the layout transforms fused into the first and last sweep
+    (tcs_mfma_edge) equals pack -> run -> unpack bitwise: random inputs, every
+    stencil and edge tile, 1-3 sweeps (and 4 with fused middle sweeps), widths
+    that do and do not allow 16-byte row accesses, and a source 4 bytes off a
+    16-byte boundary."""
+    for stencil in STENCILS:
+        B = T.gen_coeffs(stencil, 5)
+        for rows, cols in ((70, 150), (97, 517), (64, 128)):
+            base = torch.empty(rows * cols + 4, dtype=torch.float32, device="cuda")
+            for off in (0, 1):
+                src = base[off:off + rows * cols].view(rows, cols)
+                src.copy_(T.gen_uniform(rows, cols, seed=rows + cols, dtype=torch.float32))
+                for tile in ((2, 4), (4, 4), (2, 8), (4, 8)):
+                    p = T.Problem(stencil, rows, cols, B, precision=precision, tile=tile)
+                    b0, b1 = p.alloc(), p.alloc()
+                    p.clear(b0)
+                    p.clear(b1)
+                    for sweeps, fuse in ((1, 1), (2, 1), (3, 1), (4, 1 if tile == (2, 8) else 2)):  # no fused 2x8
+                        say("case run_f32", stencil, precision, rows, cols, off, tile, sweeps, fuse)
+                        p.fuse = fuse
+                        got = p.run_f32(src, sweeps, buf0=b0, buf1=b1)
+                        want = three_steps(T, torch, p, src, sweeps)
+                        assert torch.equal(got, want), (stencil, rows, cols, off, tile, sweeps, fuse)
+
+
+def test_run_f32_other_problems_run_three_steps(T, torch):
+    """Problems without an edge kernel (SPLIT, direct, 8x8 tiles, 32x32
+    fragments, persistent grids) take the three-step path through tcs_run_f32;
+    a fused single sweep needs no storage buffers."""
+    rows, cols = 70, 150
+    B = T.gen_coeffs("box2d2r", 3)
+    src = T.gen_uniform(rows, cols, seed=9, dtype=torch.float32)
+    e = T.split_scale_exp(1.0)
+    for kw, scale in (({"precision": "split"}, e), ({"method": "direct"}, 0), ({"tile": (8, 8)}, 0),
+                      ({"frag": 32}, 0), ({"wg_per_cu": 2}, 0)):
+        say("case run_f32 three steps", kw)
+        p = T.Problem("box2d2r", rows, cols, B, **kw)
+        b0, b1 = p.alloc(), p.alloc()
+        p.clear(b0)
+        p.clear(b1)
+        got = p.run_f32(src, 3, buf0=b0, buf1=b1, scale_exp=scale)
+        assert torch.equal(got, three_steps(T, torch, p, src, 3, scale))
+    p = T.Problem("box2d2r", rows, cols, B)
+    assert torch.equal(p.run_f32(src, 1), three_steps(T, torch, p, src, 1))
+    with pytest.raises(T.TcsError):
+        p.run_f32(src, 2)  # two sweeps need storage buffers
