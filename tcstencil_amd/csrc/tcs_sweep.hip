@@ -169,10 +169,12 @@ __device__ __forceinline__ uint16_t rm_encode(const SweepArgs &a, float x) {
 }
 
 // load_tile from the row-major FP32 grid a.rm_src instead of bricks (FP16/BF16,
-// one plane): the same registers, holding what the pack kernel would have
-// stored -- the rounded value inside the grid, zero for padding and for frame
-// points outside it.  Data chunks (8 elements of a row) use two 16-byte loads
-// when a.rm_vec allows and the chunk lies inside the grid.
+// one plane): what the pack kernel would have stored -- the rounded value
+// inside the grid, zero for padding and for frame points outside it.  Data
+// chunk c (8 elements of a row) is taken in row-major order over the tile,
+// chunk cc of tile row lrow, so consecutive threads read consecutive 32 bytes
+// of one row (two 16-byte loads when a.rm_vec allows and the chunk lies inside
+// the grid); write_tile<G, true> stores them with the same order.
 template <class G, int PREC>
 __device__ __forceinline__ void load_tile_rm(const SweepArgs &a, TileRegs<G> &R, int r0, int c0, int tid) {
     static_assert(G::NPL == 1, "row-major edges: FP16/BF16 storage");
@@ -180,9 +182,8 @@ __device__ __forceinline__ void load_tile_rm(const SweepArgs &a, TileRegs<G> &R,
     for (int i = 0; i < G::CPT; ++i) {
         const int c = tid + i * kWG;
         if (G::NCH % kWG == 0 || c < G::NCH) {
-            const int b = c >> 5, w = c & 31;
-            const int br = b / G::BW, bc = b - br * G::BW;
-            const int gr = r0 + 16 * br + (w >> 1), gc = c0 + 16 * bc + 8 * (w & 1);  // both >= 0
+            const int lrow = c / (2 * G::BW), cc = c - lrow * (2 * G::BW);
+            const int gr = r0 + lrow, gc = c0 + 8 * cc;  // both >= 0
             float x[8];
             if (gr < a.rows && a.rm_vec && gc + 8 <= a.cols) {
                 const float *src = a.rm_src + gr * a.rm_ld + gc;
@@ -220,17 +221,25 @@ __device__ __forceinline__ void load_tile_rm(const SweepArgs &a, TileRegs<G> &R,
     }
 }
 
-template <class G>
+// ROWMAJOR: the data chunks are in load_tile_rm's row-major order.
+template <class G, bool ROWMAJOR = false>
 __device__ __forceinline__ void write_tile(const TileRegs<G> &R, uint16_t *lds, int tid) {
     uint32_t *l32 = reinterpret_cast<uint32_t *>(lds);
 #pragma unroll
     for (int i = 0; i < G::CPT; ++i) {
         const int c = tid + i * kWG;
         if (G::NCH % kWG == 0 || c < G::NCH) {
-            const int b = c >> 5, w = c & 31;
-            const int br = b / G::BW, bc = b - br * G::BW;
-            const int lr = 16 * br + (w >> 1) + 2;
-            const int lc = 16 * bc + 8 * (w & 1) + 2;  // even: dword aligned, not 16-byte aligned
+            int lr, lc;  // lc even: dword aligned, not 16-byte aligned
+            if constexpr (ROWMAJOR) {
+                const int lrow = c / (2 * G::BW);
+                lr = lrow + 2;
+                lc = 8 * (c - lrow * (2 * G::BW)) + 2;
+            } else {
+                const int b = c >> 5, w = c & 31;
+                const int br = b / G::BW, bc = b - br * G::BW;
+                lr = 16 * br + (w >> 1) + 2;
+                lc = 16 * bc + 8 * (w & 1) + 2;
+            }
 #pragma unroll
             for (int p = 0; p < G::NPL; ++p) {
 #pragma unroll
@@ -530,31 +539,35 @@ constexpr bool kLookahead = PREC != TCS_PREC_SPLIT;
 template <int TH, int TW, int PREC, bool PERSIST>
 constexpr bool kHighOccupancy = PREC != TCS_PREC_SPLIT && TH * TW < 64 && !PERSIST;
 
-// Row-major FP32 output of one lane's four results (row gr, columns gcb..+3,
-// FP16/BF16 bits in w): what the unpack kernel writes -- the stored value
-// times rm_scale_out at interior points, the caller's rm_src value on the
-// ring; nothing outside the grid.
+// Row-major FP32 output of 8 results of row gr, columns gc..gc+7 (FP16/BF16
+// bits u): what the unpack kernel writes -- the stored value times
+// rm_scale_out at interior points, the caller's rm_src value on the ring
+// (looked up only in edge tiles); nothing outside the grid.  Two 16-byte
+// stores when a.rm_vec allows and the piece lies inside the grid.
 template <int PREC>
-__device__ __forceinline__ void store_rm(const SweepArgs &a, int gr, int gcb, u32x2 w) {
+__device__ __forceinline__ void store_rm8(const SweepArgs &a, int gr, int gc, u32x4 u, bool edge) {
     if (gr >= a.rows) return;
-    float v[4];
+    float v[8];
 #pragma unroll
-    for (int e = 0; e < 4; ++e) {
-        const uint16_t bits = (uint16_t)(w[e >> 1] >> (16 * (e & 1)));
+    for (int e = 0; e < 8; ++e) {
+        const uint16_t bits = (uint16_t)(u[e >> 1] >> (16 * (e & 1)));
         const float x = PREC == TCS_PREC_BF16 ? static_cast<float>(__builtin_bit_cast(__bf16, bits))
                                               : static_cast<float>(__builtin_bit_cast(_Float16, bits));
         v[e] = x * a.rm_scale_out;
     }
     const int64_t row = gr * a.rm_ld;
+    if (edge) {
 #pragma unroll
-    for (int e = 0; e < 4; ++e)
-        if (gcb + e < a.cols && !point_is_interior(a, gr, gcb + e)) v[e] = a.rm_src[row + gcb + e];
-    if (a.rm_vec && gcb + 4 <= a.cols) {
-        *reinterpret_cast<f32x4 *>(a.rm_dst + row + gcb) = f32x4{v[0], v[1], v[2], v[3]};
+        for (int e = 0; e < 8; ++e)
+            if (gc + e < a.cols && !point_is_interior(a, gr, gc + e)) v[e] = a.rm_src[row + gc + e];
+    }
+    if (a.rm_vec && gc + 8 <= a.cols) {
+        *reinterpret_cast<f32x4 *>(a.rm_dst + row + gc) = f32x4{v[0], v[1], v[2], v[3]};
+        *reinterpret_cast<f32x4 *>(a.rm_dst + row + gc + 4) = f32x4{v[4], v[5], v[6], v[7]};
     } else {
 #pragma unroll
-        for (int e = 0; e < 4; ++e)
-            if (gcb + e < a.cols) a.rm_dst[row + gcb + e] = v[e];
+        for (int e = 0; e < 8; ++e)
+            if (gc + e < a.cols) a.rm_dst[row + gc + e] = v[e];
     }
 }
 
@@ -587,9 +600,10 @@ __device__ __forceinline__ void sweep_tile16(const SweepArgs &a) {
     C.load(a.cw, m, g);
     const typename Ops<T, ZLO, ZHI, PREC>::Offsets off(m, g);
     constexpr int NB = TH * TW / 4;
+    u32x2 res[MODE & 2 ? NB : 1];  // results held for the row-major output
 
     for (;;) {
-        write_tile<T>(R, lds, tid);
+        write_tile<T, (MODE & 1) != 0>(R, lds, tid);
         __syncthreads();
         const int cgr0 = gr0, cgc0 = gc0;  // the tile in LDS
         const int next = skip_empty(a, sc.tile + sc.step, sc.step);
@@ -623,13 +637,39 @@ __device__ __forceinline__ void sweep_tile16(const SweepArgs &a) {
             u32x2 w[NPL];
             mfma_brick<T, ZLO, ZHI, PREC>(a, lds, C, cur, br, bc, m, g, edge, gr, gcb, w);
             if constexpr (MODE & 2) {
-                store_rm<PREC>(a, gr, gcb, w[0]);
+                res[q] = w[0];
             } else {
                 const int64_t s = stor_index(gr, gcb, a.sbc);
 #pragma unroll
                 for (int p = 0; p < NPL; ++p) store8<true>(a.out + p * a.plane + s, w[p]);
             }
             if constexpr (LOOKAHEAD) cur = nxt;
+        }
+        if constexpr (MODE & 2) {
+            // Row-major output through LDS: the tile's results go into a
+            // row-major FP16 image (pitch OP, over the input image, which every
+            // wave has finished reading), then every thread writes 8 results
+            // of one row -- 32 contiguous bytes, consecutive threads along the
+            // row -- instead of 16-byte pieces of 16 rows per brick.
+            constexpr int OP = 16 * TW + 8;
+            static_assert(16 * TH * OP <= T::PLANE, "output image must fit the input image");
+            __syncthreads();
+#pragma unroll
+            for (int q = 0; q < NB; ++q) {
+                int br, bc;
+                brick_of<TW, 0>(wave + 4 * q, br, bc);
+                *reinterpret_cast<u32x2 *>(lds + (16 * br + m) * OP + 16 * bc + 4 * g) = res[q];
+            }
+            __syncthreads();
+#pragma unroll
+            for (int i = 0; i < T::CPT; ++i) {
+                const int c = tid + i * kWG;
+                if (T::NCH % kWG == 0 || c < T::NCH) {
+                    const int lrow = c / (2 * TW), cc = c - lrow * (2 * TW);
+                    const u32x4 u = *reinterpret_cast<const u32x4 *>(lds + lrow * OP + 8 * cc);
+                    store_rm8<PREC>(a, cgr0 + lrow, cgc0 + 8 * cc, u, edge);
+                }
+            }
         }
         // One-tile-per-workgroup launches cover every tile with the grid itself.
         if (!PERSIST || next >= sc.end) break;
@@ -648,10 +688,13 @@ void tcs_mfma_sweep(SweepArgs a) {
 // The first and last sweep of tcs_run_f32 with the layout transforms fused in
 // (MODE 1: row-major FP32 in, storage out; 2: storage in, row-major FP32 out;
 // 3: both, a whole one-sweep run in one kernel).  Results are bitwise those of
-// tcs_pack_f32 / tcs_mfma_sweep / tcs_unpack_f32.
+// tcs_pack_f32 / tcs_mfma_sweep / tcs_unpack_f32.  The row-major output holds
+// the tile's results in registers until the LDS output image is written, and
+// is held to 4 waves per SIMD rather than spill at 8 (box2d2r 16384^2, one
+// sweep: 0.50 ms unstaged at 8 waves, 0.86 ms staged with spills, 0.45 ms).
 template <int TH, int TW, int ZLO, int ZHI, int PREC, int MODE>
 __global__ __launch_bounds__(kWG)
-__attribute__((amdgpu_waves_per_eu(kHighOccupancy<TH, TW, PREC, false> ? 8 : 1, 8)))
+__attribute__((amdgpu_waves_per_eu((MODE & 2) ? 4 : (kHighOccupancy<TH, TW, PREC, false> ? 8 : 1), 8)))
 void tcs_mfma_edge(SweepArgs a) {
     sweep_tile16<TH, TW, ZLO, ZHI, PREC, false, MODE>(a);
 }

@@ -332,9 +332,8 @@ def stage_tile_rm(src, rows, cols, ld, G: TileGeom, gr0: int, gc0: int, vec: boo
     for i in range(G.CPT):
         c = tid + i * WG
         c = c[c < G.NCH]
-        b, w = c >> 5, c & 31
-        br, bc = b // TW, b % TW
-        gr, gc = gr0 + 16 * br + (w >> 1), gc0 + 16 * bc + 8 * (w & 1)
+        lrow, cc = c // (2 * TW), c % (2 * TW)  # row-major chunk order over the tile
+        gr, gc = gr0 + lrow, gc0 + 8 * cc
         x = np.zeros((c.size, 8))
         vecok = (gr < rows) & (gc + 8 <= cols) & vec
         idx = gr[vecok] * ld + gc[vecok]
@@ -346,9 +345,10 @@ def stage_tile_rm(src, rows, cols, ld, G: TileGeom, gr0: int, gc0: int, vec: boo
             ie = gr[ok] * ld + gc[ok] + e
             _check(ie, n, 1, "row-major load")
             x[ok, e] = src[ie]
-        lr, lc = 16 * br + (w >> 1) + 2, 16 * bc + 8 * (w & 1) + 2
+        lr, lc = lrow + 2, 8 * cc + 2
         for q in range(4):
             e0 = G.at(lr, lc + 2 * q)
+            assert np.all(np.isnan(lds[e0])), "two chunks write one LDS word"
             lds[e0] = f16(x[:, 2 * q])
             lds[e0 + 1] = f16(x[:, 2 * q + 1])
     for i in range(G.HPT):
@@ -391,9 +391,12 @@ def mfma_edge(mode, src, rows, cols, ld, L: Layout, TH, TW, coeff5, rs, vec, inp
     out = np.zeros(n if mode & 2 else L.plane)
     written = np.zeros(out.size, dtype=bool)
     m, g = np.arange(64) & 15, np.arange(64) >> 4
+    OP = 16 * TW + 8  # pitch of the row-major FP16 output image (over the input image)
+    assert 16 * TH * OP <= G.PLANE
     for gr0, gc0 in _tiles(L, TH, TW, 0):
         lds = stage_tile_rm(src, rows, cols, ld, G, gr0, gc0, vec) if mode & 1 else stage_tile(inp, L, G, gr0, gc0)
         edge = not (gr0 >= rs and gr0 + 16 * TH <= L.rows - rs and gc0 >= rs and gc0 + 16 * TW <= L.cols - rs)
+        oimg = np.full(G.PLANE, np.nan)
         for wave in range(4):
             for q in range(TH * TW // 4):
                 b = wave + 4 * q
@@ -403,25 +406,42 @@ def mfma_edge(mode, src, rows, cols, ld, L: Layout, TH, TW, coeff5, rs, vec, inp
                 if not mode & 2:
                     _store(out, written, L, gr, gcb, o)
                     continue
-                for l in range(64):  # store_rm, lane by lane
-                    if gr[l] >= rows:
-                        continue
-                    v = o[l].copy()
-                    for e in range(4):
-                        c_ = gcb[l] + e
-                        if c_ < cols and not _interior(gr[l], c_, L, rs):
-                            v[e] = src[gr[l] * ld + c_]
-                    base = gr[l] * ld + gcb[l]
-                    if vec and gcb[l] + 4 <= cols:
-                        _check(np.array([base]), n - 3, 4, "row-major vector store (16 B)")
-                        es = range(4)
-                    else:
-                        es = [e for e in range(4) if gcb[l] + e < cols]
-                        _check(np.array([base + e for e in es], dtype=np.int64), n, 1, "row-major store")
-                    for e in es:
-                        assert not written[base + e], "two lanes store the same element"
-                        out[base + e] = v[e]
-                        written[base + e] = True
+                e0 = (16 * br + m) * OP + 16 * bc + 4 * g  # one 8-byte LDS write per lane
+                _check(e0, G.PLANE - 3, 4, "output image write (8 B)")
+                for v in range(4):
+                    assert np.all(np.isnan(oimg[e0 + v])), "two lanes write one output element"
+                    oimg[e0 + v] = o[:, v]
+        if not mode & 2:
+            continue
+        tid = np.arange(WG)
+        for i in range(G.CPT):  # store_rm8: 8 results of one row per thread
+            c = tid + i * WG
+            c = c[c < G.NCH]
+            lrow, cc = c // (2 * TW), c % (2 * TW)
+            e0 = lrow * OP + 8 * cc
+            _check(e0, G.PLANE - 7, 8, "output image read (16 B)")
+            vals = oimg[e0[:, None] + np.arange(8)[None, :]]
+            assert not np.isnan(vals).any(), "output image read an unwritten element"
+            for t in range(c.size):
+                gr, gc = gr0 + lrow[t], gc0 + 8 * cc[t]
+                if gr >= rows:
+                    continue
+                v = vals[t].copy()
+                for e in range(8):
+                    if gc + e < cols and not _interior(gr, gc + e, L, rs):
+                        assert edge, "ring point in a tile not marked edge"
+                        v[e] = src[gr * ld + gc + e]
+                base = gr * ld + gc
+                if vec and gc + 8 <= cols:
+                    _check(np.array([base]), n - 7, 4, "row-major vector store (2 x 16 B)")
+                    es = range(8)
+                else:
+                    es = [e for e in range(8) if gc + e < cols]
+                    _check(np.array([base + e for e in es], dtype=np.int64), n, 1, "row-major store")
+                for e in es:
+                    assert not written[base + e], "two threads store the same element"
+                    out[base + e] = v[e]
+                    written[base + e] = True
     return out, written
 
 
