@@ -34,6 +34,18 @@ def _ptr(t) -> ctypes.c_void_p:
     return ctypes.c_void_p(t.data_ptr())
 
 
+def _grid_ptr(t, rows: int, cols: int, what: str) -> ctypes.c_void_p:
+    """A row-major grid view: unit column stride, any row stride >= cols (the
+    C functions take it as the leading dimension)."""
+    if not t.is_cuda:
+        raise ValueError(f"{what}: expected a device tensor")
+    if t.dim() != 2 or t.shape[0] != rows or t.shape[1] != cols:
+        raise ValueError(f"{what}: shape {tuple(t.shape)} != grid {rows}x{cols}")
+    if t.stride(1) != 1 or t.stride(0) < cols:
+        raise ValueError(f"{what}: expected unit column stride and row stride >= {cols}")
+    return ctypes.c_void_p(t.data_ptr())
+
+
 def _fp(a: np.ndarray):
     return a.ctypes.data_as(ctypes.POINTER(ctypes.c_float))
 
@@ -148,23 +160,28 @@ class Problem:
     def pack(self, src, buf, scale_exp: int = 0, stream=None) -> None:
         """Row-major FP32/FP64 device grid -> brick storage."""
         torch = _torch()
-        if src.dim() != 2 or src.shape[0] != self.rows or src.shape[1] != self.cols:
-            raise ValueError(f"source shape {tuple(src.shape)} != grid {self.rows}x{self.cols}")
+        sp = _grid_ptr(src, self.rows, self.cols, "pack source")
         if buf.numel() < self.buffer_bytes:
             raise ValueError("storage buffer too small")
         fn = {torch.float32: lib().tcs_pack_f32, torch.float64: lib().tcs_pack_f64}.get(src.dtype)
         if fn is None:
             raise TypeError("pack: source must be float32 or float64")
-        check(fn(self.ref, _ptr(src), src.stride(0), _ptr(buf), scale_exp, _stream(stream)))
+        check(fn(self.ref, sp, src.stride(0), _ptr(buf), scale_exp, _stream(stream)))
 
     def unpack(self, buf, out=None, scale_exp: int = 0, ring=None, stream=None):
         """Brick storage -> row-major FP32 device grid (ring copied from `ring`)."""
         torch = _torch()
         if out is None:
             out = torch.empty((self.rows, self.cols), dtype=torch.float32, device=buf.device)
-        ringp = _ptr(ring) if ring is not None else None
-        check(lib().tcs_unpack_f32(self.ref, _ptr(buf), _ptr(out), out.stride(0), scale_exp,
-                                   ringp, _stream(stream)))
+        if out.dtype != torch.float32:
+            raise TypeError("unpack: output must be float32")
+        op = _grid_ptr(out, self.rows, self.cols, "unpack output")
+        ringp = None
+        if ring is not None:
+            ringp = _grid_ptr(ring, self.rows, self.cols, "unpack ring")
+            if ring.stride(0) != out.stride(0) or ring.dtype != torch.float32:
+                raise ValueError("unpack: ring must be float32 with the output's row stride")
+        check(lib().tcs_unpack_f32(self.ref, _ptr(buf), op, out.stride(0), scale_exp, ringp, _stream(stream)))
         return out
 
     # -- sweeps -------------------------------------------------------------
@@ -190,7 +207,9 @@ class Problem:
             raise ValueError("run_f32: source and output need the same row stride and unit column stride")
         b0 = _ptr(buf0) if buf0 is not None else None
         b1 = _ptr(buf1) if buf1 is not None else None
-        check(lib().tcs_run_f32(self.ref, _ptr(src), _ptr(out), src.stride(0), b0, b1, sweeps, scale_exp,
+        sp = _grid_ptr(src, self.rows, self.cols, "run_f32 source")
+        op = _grid_ptr(out, self.rows, self.cols, "run_f32 output")
+        check(lib().tcs_run_f32(self.ref, sp, op, src.stride(0), b0, b1, sweeps, scale_exp,
                                 _stream(stream)))
         return out
 

@@ -40,68 +40,106 @@ __device__ __forceinline__ float decode(uint16_t hi, uint16_t lo, int prec) {
     return h;
 }
 
-// -------------------------------------------------------------------- pack
-// One thread per 8-element chunk of a storage row (16*sbr storage rows, 2*sbc
-// chunks each); consecutive threads read consecutive source chunks.
+// ------------------------------------------------------- pack and unpack
+// Both transforms give one wave a span of 16 rows x 64 columns: four bricks
+// side by side in one brick row.  Lane l handles the 4-column piece
+// c4 = l % 16 (brick 4*span + c4/4 of the row, brick columns 4*(c4%4)..+3)
+// in the rows x = 4k + l/16, k = 0..3.  For each k the 64 lanes cover four
+// complete rows of 256 bytes (FP32) and, in each of the four bricks, four
+// complete brick rows (128 contiguous bytes of FP16), so every memory
+// instruction of the wave moves whole 128-byte lines on both sides.
+constexpr int kSpanBricks = 4;
+
+// Pack: spans over the stored bricks (ghost ring included; points outside the
+// grid are stored as zero).  FP32 pieces inside the grid are read with one
+// 16-byte load when vec (16-byte aligned source and leading dimension).
 template <typename T>
 __global__ void tcs_pack_kernel(const T *__restrict__ src, int64_t ld, uint16_t *__restrict__ dst,
-                                int64_t plane, int rows, int cols, int sbr, int sbc, int prec,
-                                T scale) {
-    const int64_t chunks_per_row = 2 * (int64_t)sbc;
-    const int64_t total = 16 * (int64_t)sbr * chunks_per_row;
+                                int64_t plane, int rows, int cols, int sbr, int sbc, int prec, T scale, int vec) {
+    const int spans_per_row = (sbc + kSpanBricks - 1) / kSpanBricks;
+    const int64_t total = 64 * (int64_t)sbr * spans_per_row;
     for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < total;
          t += (int64_t)gridDim.x * blockDim.x) {
-        const int sr = (int)(t / chunks_per_row);
-        const int cc = (int)(t - sr * chunks_per_row);
-        const int gr = sr - 16, gc0 = 8 * cc - 16;
-        uint16_t hi[8], lo[8];
+        const int64_t span = t >> 6;
+        const int l = (int)(t & 63), c4 = l & 15;
+        const int bi = (int)(span / spans_per_row);
+        const int bj = kSpanBricks * (int)(span - (int64_t)bi * spans_per_row) + (c4 >> 2);
+        if (bj >= sbc) continue;
+        const int cb = 4 * (c4 & 3), gc = 16 * bj - 16 + cb;
+        const bool col_vec = sizeof(T) == 4 && vec && gc >= 0 && gc + 4 <= cols;
+        const int64_t brick = ((int64_t)bi * sbc + bj) * 256 + cb;
 #pragma unroll
-        for (int e = 0; e < 8; ++e) {
-            const int gc = gc0 + e;
-            T x = 0;
-            if (gr >= 0 && gr < rows && gc >= 0 && gc < cols) x = src[gr * ld + gc] * scale;
-            const Enc2 en = encode<T>(x, prec);
-            hi[e] = en.hi;
-            lo[e] = en.lo;
-        }
-        const int64_t s = stor_index(gr, gc0, sbc);
-        u32x4 w;
+        for (int k = 0; k < 4; ++k) {
+            const int x = 4 * k + (l >> 4), gr = 16 * bi - 16 + x;
+            const bool row_in = (unsigned)gr < (unsigned)rows;
+            T v[4];
+            if (row_in && col_vec) {
+                const f32x4 w = *reinterpret_cast<const f32x4 *>(src + gr * ld + gc);
 #pragma unroll
-        for (int q = 0; q < 4; ++q) w[q] = (uint32_t)hi[2 * q] | ((uint32_t)hi[2 * q + 1] << 16);
-        *reinterpret_cast<u32x4 *>(dst + s) = w;
-        if (prec == TCS_PREC_SPLIT) {
+                for (int e = 0; e < 4; ++e) v[e] = (T)w[e];
+            } else {
 #pragma unroll
-            for (int q = 0; q < 4; ++q) w[q] = (uint32_t)lo[2 * q] | ((uint32_t)lo[2 * q + 1] << 16);
-            *reinterpret_cast<u32x4 *>(dst + plane + s) = w;
+                for (int e = 0; e < 4; ++e)
+                    v[e] = row_in && (unsigned)(gc + e) < (unsigned)cols ? src[gr * ld + gc + e] : (T)0;
+            }
+            Enc2 en[4];
+#pragma unroll
+            for (int e = 0; e < 4; ++e) en[e] = encode<T>(v[e] * scale, prec);
+            const int64_t d = brick + 16 * x;
+            *reinterpret_cast<u32x2 *>(dst + d) =
+                u32x2{(uint32_t)en[0].hi | ((uint32_t)en[1].hi << 16), (uint32_t)en[2].hi | ((uint32_t)en[3].hi << 16)};
+            if (prec == TCS_PREC_SPLIT)
+                *reinterpret_cast<u32x2 *>(dst + plane + d) = u32x2{(uint32_t)en[0].lo | ((uint32_t)en[1].lo << 16),
+                                                                    (uint32_t)en[2].lo | ((uint32_t)en[3].lo << 16)};
         }
     }
 }
 
-// ------------------------------------------------------------------ unpack
-// One thread per 4 consecutive output columns of one row.
+// Unpack: spans over the data bricks; writes the points inside the grid, the
+// ring points (outside [rs, rows-rs) x [rs, cols-rs)) from `ring` when given.
+// Pieces inside the grid are stored with one 16-byte store when vec.
 __global__ void tcs_unpack_kernel(const uint16_t *__restrict__ src, int64_t plane, float *__restrict__ dst,
-                                  int64_t ld, int rows, int cols, int sbc, int prec, float scale,
-                                  const float *__restrict__ ring, int rs) {
-    const int64_t q4 = (cols + 3) / 4;
-    const int64_t total = (int64_t)rows * q4;
+                                  int64_t ld, int rows, int cols, int nbr, int nbc, int sbc, int prec, float scale,
+                                  const float *__restrict__ ring, int rs, int vec) {
+    const int spans_per_row = (nbc + kSpanBricks - 1) / kSpanBricks;
+    const int64_t total = 64 * (int64_t)nbr * spans_per_row;
     for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < total;
          t += (int64_t)gridDim.x * blockDim.x) {
-        const int gr = (int)(t / q4);
-        const int gc0 = 4 * (int)(t - gr * q4);
-        const int64_t s = stor_index(gr, gc0, sbc);
-        const u32x2 h = *reinterpret_cast<const u32x2 *>(src + s);
-        u32x2 l = {0u, 0u};
-        if (prec == TCS_PREC_SPLIT) l = *reinterpret_cast<const u32x2 *>(src + plane + s);
+        const int64_t span = t >> 6;
+        const int l = (int)(t & 63), c4 = l & 15;
+        const int i = (int)(span / spans_per_row);
+        const int j = kSpanBricks * (int)(span - (int64_t)i * spans_per_row) + (c4 >> 2);
+        const int cb = 4 * (c4 & 3), gc = 16 * j + cb;
+        if (j >= nbc || gc >= cols) continue;
+        const int64_t brick = ((int64_t)(i + 1) * sbc + (j + 1)) * 256 + cb;
 #pragma unroll
-        for (int e = 0; e < 4; ++e) {
-            const int gc = gc0 + e;
-            if (gc >= cols) break;
-            const uint16_t hb = (uint16_t)(h[e >> 1] >> (16 * (e & 1)));
-            const uint16_t lb = (uint16_t)(l[e >> 1] >> (16 * (e & 1)));
-            float v = decode(hb, lb, prec) * scale;
-            const bool interior = gr >= rs && gr < rows - rs && gc >= rs && gc < cols - rs;
-            if (ring && !interior) v = ring[gr * ld + gc];
-            dst[gr * ld + gc] = v;
+        for (int k = 0; k < 4; ++k) {
+            const int x = 4 * k + (l >> 4), gr = 16 * i + x;
+            if (gr >= rows) continue;
+            const int64_t s = brick + 16 * x;
+            const u32x2 h = *reinterpret_cast<const u32x2 *>(src + s);
+            u32x2 lw = {0u, 0u};
+            if (prec == TCS_PREC_SPLIT) lw = *reinterpret_cast<const u32x2 *>(src + plane + s);
+            float v[4];
+#pragma unroll
+            for (int e = 0; e < 4; ++e)
+                v[e] = decode((uint16_t)(h[e >> 1] >> (16 * (e & 1))), (uint16_t)(lw[e >> 1] >> (16 * (e & 1))), prec) *
+                       scale;
+            const int64_t o = gr * ld + gc;
+            if (ring && !(gr >= rs && gr < rows - rs && gc >= rs && gc + 4 <= cols - rs)) {
+#pragma unroll
+                for (int e = 0; e < 4; ++e) {
+                    const int c = gc + e;
+                    if (c < cols && !(gr >= rs && gr < rows - rs && c >= rs && c < cols - rs)) v[e] = ring[o + e];
+                }
+            }
+            if (vec && gc + 4 <= cols) {
+                *reinterpret_cast<f32x4 *>(dst + o) = f32x4{v[0], v[1], v[2], v[3]};
+            } else {
+#pragma unroll
+                for (int e = 0; e < 4; ++e)
+                    if (gc + e < cols) dst[o + e] = v[e];
+            }
         }
     }
 }
@@ -245,10 +283,11 @@ static int pack_impl(const tcs_problem *p, const T *d_src, int64_t ld, void *d_d
     if (!d_src || !d_dst) return set_error(TCS_ERR_INVALID, "pack: null pointer");
     if (ld < p->layout.cols) return set_error(TCS_ERR_INVALID, "pack: ld %lld < cols %d", (long long)ld, p->layout.cols);
     const tcs_layout &L = p->layout;
-    const int64_t total = 16 * (int64_t)L.sbr * 2 * L.sbc;
+    const int64_t total = 64 * (int64_t)L.sbr * ((L.sbc + kSpanBricks - 1) / kSpanBricks);
     const T scale = (T)ldexp(1.0, scale_exp);
+    const int vec = sizeof(T) == 4 && ld % 4 == 0 && (reinterpret_cast<uintptr_t>(d_src) & 15) == 0;
     hipLaunchKernelGGL(tcs_pack_kernel<T>, dim3(grid_for(total)), dim3(256), 0, (hipStream_t)stream, d_src, ld,
-                       (uint16_t *)d_dst, L.plane_elems, L.rows, L.cols, L.sbr, L.sbc, p->precision, scale);
+                       (uint16_t *)d_dst, L.plane_elems, L.rows, L.cols, L.sbr, L.sbc, p->precision, scale, vec);
     TCS_HIP_CHECK(hipGetLastError());
     return TCS_OK;
 }
@@ -270,10 +309,11 @@ extern "C" int tcs_unpack_f32(const tcs_problem *p, const void *d_src, float *d_
     if (!d_src || !d_dst) return set_error(TCS_ERR_INVALID, "unpack: null pointer");
     const tcs_layout &L = p->layout;
     if (ld < L.cols) return set_error(TCS_ERR_INVALID, "unpack: ld %lld < cols %d", (long long)ld, L.cols);
-    const int64_t total = (int64_t)L.rows * ((L.cols + 3) / 4);
+    const int64_t total = 64 * (int64_t)L.nbr * ((L.nbc + kSpanBricks - 1) / kSpanBricks);
+    const int vec = ld % 4 == 0 && (reinterpret_cast<uintptr_t>(d_dst) & 15) == 0;
     hipLaunchKernelGGL(tcs_unpack_kernel, dim3(grid_for(total)), dim3(256), 0, (hipStream_t)stream,
-                       (const uint16_t *)d_src, L.plane_elems, d_dst, ld, L.rows, L.cols, L.sbc, p->precision,
-                       ldexpf(1.0f, -scale_exp), d_ring, p->radius);
+                       (const uint16_t *)d_src, L.plane_elems, d_dst, ld, L.rows, L.cols, L.nbr, L.nbc, L.sbc,
+                       p->precision, ldexpf(1.0f, -scale_exp), d_ring, p->radius, vec);
     TCS_HIP_CHECK(hipGetLastError());
     return TCS_OK;
 }

@@ -263,17 +263,26 @@ def test_generators_match_oracle(T, torch, oracle):
     assert T.maxabs(x) == float(x.abs().max())
 
 
-def test_pack_unpack_roundtrip_and_ring(T, torch):
-    x = T.gen_uniform(45, 77, seed=9, dtype=torch.float32)
-    T.round_to(x, "fp16")
+@pytest.mark.parametrize("rows,cols,ld", [(45, 77, 77), (45, 88, 88), (33, 77, 96), (64, 64, 64)])
+def test_pack_unpack_roundtrip_and_ring(T, torch, rows, cols, ld):
+    """Unaligned leading dimensions take the scalar path; aligned ones the
+    16-byte loads and stores, with scalar pieces at ragged right edges."""
+    base = T.gen_uniform(rows, ld, seed=9, dtype=torch.float32)
+    T.round_to(base, "fp16")
+    x = base[:, :cols]
     for precision in ("fp16", "split"):
-        p = T.Problem("box2d2r", 45, 77, np.ones((5, 5), np.float32), precision=precision)
+        p = T.Problem("box2d2r", rows, cols, np.ones((5, 5), np.float32), precision=precision)
         b = p.alloc()
         p.pack(x, b)
-        assert torch.equal(p.unpack(b), x)
+        full = torch.full((rows, ld), -3.0, device=x.device)
+        assert torch.equal(p.unpack(b, out=full[:, :cols]), x)
+        assert bool(full[:, cols:].eq(-3.0).all())
     ring = torch.full_like(x, 7.0)
     y = p.unpack(b, ring=ring)
-    assert torch.equal(y[2:-2, 2:-2], x[2:-2, 2:-2]) and bool((y[:2] == 7.0).all())
+    assert torch.equal(y[2:-2, 2:-2], x[2:-2, 2:-2])
+    edge = torch.ones_like(y, dtype=torch.bool)
+    edge[2:-2, 2:-2] = False
+    assert bool((y[edge] == 7.0).all())
 
 
 # ------------------------------------------------ graphs and determinism
@@ -701,6 +710,30 @@ def test_run_f32_fused_edges_bitwise(T, torch, precision):
                         got = p.run_f32(src, sweeps, buf0=b0, buf1=b1)
                         want = three_steps(T, torch, p, src, sweeps)
                         assert torch.equal(got, want), (stencil, rows, cols, off, tile, sweeps, fuse)
+
+
+def test_run_f32_row_strided_views(T, torch):
+    """A grid that is a column slice of a wider array (leading dimension 160
+    for 150 columns) gives the result of the contiguous grid, fused and in
+    three steps, and leaves the columns past the grid untouched."""
+    rows, cols, ld = 70, 150, 160
+    B = T.gen_coeffs("box2d2r", 5)
+    x = T.gen_uniform(rows, cols, seed=4, dtype=torch.float32)
+    wide = torch.zeros((rows, ld), dtype=torch.float32, device="cuda")
+    wide[:, :cols] = x
+    for kw in ({}, {"precision": "split"}):
+        p = T.Problem("box2d2r", rows, cols, B, **kw)
+        e = T.split_scale_exp(1.0) if kw else 0
+        b0, b1 = p.alloc(), p.alloc()
+        p.clear(b0)
+        p.clear(b1)
+        for sweeps in (1, 3):
+            say("case run_f32 strided", kw, sweeps)
+            want = p.run_f32(x, sweeps, buf0=b0, buf1=b1, scale_exp=e)
+            out = torch.full((rows, ld), -5.0, dtype=torch.float32, device="cuda")
+            got = p.run_f32(wide[:, :cols], sweeps, out=out[:, :cols], buf0=b0, buf1=b1, scale_exp=e)
+            assert torch.equal(got, want), (kw, sweeps)
+            assert bool(out[:, cols:].eq(-5.0).all())
 
 
 def test_run_f32_other_problems_run_three_steps(T, torch):
