@@ -47,7 +47,8 @@ struct SweepArgs {
     int32_t sbr, sbc;      // stored brick rows, stored bricks per brick row
     int32_t tiles_x;       // tiles per tile row
     int32_t tiles_xp;      // tiles_x rounded up to a multiple of 8 (virtual tile row)
-    int32_t nvirt;         // virtual tiles: tile rows * tiles_xp
+    int32_t nvirt;         // virtual tiles: tile rows * tiles_xp (row-band order: see tcs_sweep.hip)
+    int32_t tiles_y;       // tile rows
     int32_t rs;            // stencil radius: points closer than rs to the edge are copied
     float out_scale;       // SPLIT: 2^-coeff_exp
     float cf[25];          // 5x5 FP32 coefficient embedding (direct kernel)

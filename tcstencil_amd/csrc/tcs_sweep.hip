@@ -290,24 +290,51 @@ __device__ __forceinline__ bool point_is_interior(const SweepArgs &a, int gr, in
 // box2d2r 16384^2 (8x8-brick tiles): this order beats eight XCD-private
 // contiguous tile ranges (the ranges keep horizontal neighbours on one XCD,
 // but run 13% slower as a plain tile copy) and runs of up to 16 tiles per XCD.
+//
+// Row-band order (BAND, the kernels that read row-major FP32 input): tile rows
+// are grouped in bands of 8 (padded with empty tiles), and within a band
+// virtual tile v takes tile column (v % (8 tiles_x)) / 8 of tile row
+// 8 band + v % 8.  Tile row ty then runs on XCD ty % 8, so horizontally
+// adjacent tiles share an L2.  There the horizontal halo is the costly one: a
+// 2-column strip of an FP32 row costs a 128-byte line per row per side, on a
+// 512-byte tile row (4x8 bricks: 1.57x the algorithmic input fetched in tile
+// column order), against two extra rows of 64 for the vertical halo.
 struct Schedule {
     int tile, end, step;
 };
 
+template <bool BAND>
+__device__ __forceinline__ bool tile_is_empty(const SweepArgs &a, int v) {
+    if constexpr (BAND)
+        return 8 * (v / (8 * a.tiles_x)) + (v & 7) >= a.tiles_y;
+    else
+        return v % a.tiles_xp >= a.tiles_x;
+}
+
 // First virtual tile >= v (in steps of `step`) that is not empty.
+template <bool BAND = false>
 __device__ __forceinline__ int skip_empty(const SweepArgs &a, int v, int step) {
-    while (v < a.nvirt && v % a.tiles_xp >= a.tiles_x) v += step;
+    while (v < a.nvirt && tile_is_empty<BAND>(a, v)) v += step;
     return v;
 }
 
+template <bool BAND = false>
 __device__ __forceinline__ Schedule schedule(const SweepArgs &a) {
     const int step = (int)gridDim.x;
-    return Schedule{skip_empty(a, (int)blockIdx.x, step), a.nvirt, step};
+    return Schedule{skip_empty<BAND>(a, (int)blockIdx.x, step), a.nvirt, step};
 }
 
-template <int TH, int TW>
+template <int TH, int TW, bool BAND = false>
 __device__ __forceinline__ void tile_origin(const SweepArgs &a, int v, int &gr0, int &gc0) {
-    const int ty = v / a.tiles_xp, tx = v - ty * a.tiles_xp;
+    int ty, tx;
+    if constexpr (BAND) {
+        const int band = v / (8 * a.tiles_x), rem = v - band * 8 * a.tiles_x;
+        ty = 8 * band + (rem & 7);
+        tx = rem >> 3;
+    } else {
+        ty = v / a.tiles_xp;
+        tx = v - ty * a.tiles_xp;
+    }
     gr0 = 16 * TH * ty;
     gc0 = 16 * TW * tx;
 }
@@ -582,11 +609,12 @@ __device__ __forceinline__ void sweep_tile16(const SweepArgs &a) {
     using T = Tile<TH, TW, NPL, true>;
     __shared__ __attribute__((aligned(16))) uint16_t lds[NPL * T::PLANE];
 
-    Schedule sc = schedule(a);
+    constexpr bool BAND = (MODE & 1) != 0;
+    Schedule sc = schedule<BAND>(a);
     if (sc.tile >= sc.end) return;
     const int tid = threadIdx.x;
     int gr0, gc0;
-    tile_origin<TH, TW>(a, sc.tile, gr0, gc0);
+    tile_origin<TH, TW, BAND>(a, sc.tile, gr0, gc0);
     TileRegs<T> R;
     if constexpr (MODE & 1)
         load_tile_rm<T, PREC>(a, R, gr0, gc0, tid);
@@ -606,9 +634,9 @@ __device__ __forceinline__ void sweep_tile16(const SweepArgs &a) {
         write_tile<T, (MODE & 1) != 0>(R, lds, tid);
         __syncthreads();
         const int cgr0 = gr0, cgc0 = gc0;  // the tile in LDS
-        const int next = skip_empty(a, sc.tile + sc.step, sc.step);
+        const int next = skip_empty<BAND>(a, sc.tile + sc.step, sc.step);
         if (PERSIST && next < sc.end) {
-            tile_origin<TH, TW>(a, next, gr0, gc0);
+            tile_origin<TH, TW, BAND>(a, next, gr0, gc0);
             load_tile<T, false>(a, R, gr0, gc0, tid);  // in flight during the compute below
         }
         const bool edge = !tile_is_interior(a, cgr0, cgc0, 16 * TH, 16 * TW);
@@ -1203,6 +1231,7 @@ __device__ __forceinline__ void batch_body(const BatchProblem *__restrict__ prob
     a.tiles_x = P.tiles_x;
     a.tiles_xp = P.tiles_x;  // the tile list holds real tile indices
     a.nvirt = 0;
+    a.tiles_y = 0;
     a.rs = P.rs;
     a.out_scale = P.out_scale;
     int gr0, gc0;
@@ -1550,7 +1579,8 @@ static int make_args(const tcs_problem *p, const void *d_in, void *d_out, SweepA
     a.sbc = L.sbc;
     a.tiles_x = L.nbc / p->tile_w;
     a.tiles_xp = (a.tiles_x + kXCD - 1) / kXCD * kXCD;
-    a.nvirt = (L.nbr / p->tile_h) * a.tiles_xp;
+    a.tiles_y = L.nbr / p->tile_h;
+    a.nvirt = a.tiles_y * a.tiles_xp;
     a.rs = p->radius;
     a.out_scale = ldexpf(1.0f, -p->coeff_exp);
     for (int i = 0; i < 25; ++i) a.cf[i] = p->coeff[i];
@@ -1620,6 +1650,7 @@ int launch_edge(const tcs_problem *p, int mode, const void *d_in, void *d_out, c
     a.rm_scale_out = ldexpf(1.0f, -scale_exp);
     a.rm_vec = ld % 4 == 0 && (reinterpret_cast<uintptr_t>(src) & 15) == 0 &&
                (reinterpret_cast<uintptr_t>(dst) & 15) == 0;
+    if (mode & 1) a.nvirt = (a.tiles_y + kXCD - 1) / kXCD * kXCD * a.tiles_x;  // row-band order
     hipLaunchKernelGGL(k->fn, dim3((unsigned)a.nvirt), dim3(kWG), 0, stream, a);
     TCS_HIP_CHECK(hipGetLastError());
     return TCS_OK;

@@ -93,7 +93,7 @@ def run_workload(stencil, rows, cols, method="mfma", precision="fp16", tile=None
     return out
 
 
-def run_edges(stencil, rows, cols, precision="fp16", sweeps=1, reps=20, seed=1):
+def run_edges(stencil, rows, cols, precision="fp16", sweeps=1, reps=20, seed=1, tile=None):
     """ms per call of tcs_run_f32 (fused edges) and of pack + run + unpack on
     the same row-major FP32 grid, and whether their outputs are bitwise equal."""
     import torch
@@ -101,7 +101,7 @@ def run_edges(stencil, rows, cols, precision="fp16", sweeps=1, reps=20, seed=1):
     from . import api as T
 
     B = T.gen_coeffs(stencil, seed)
-    p = T.Problem(stencil, rows, cols, B, precision=precision)
+    p = T.Problem(stencil, rows, cols, B, precision=precision, tile=tuple(tile) if tile else None)
     x = T.gen_uniform(rows, cols, seed=seed, dtype=torch.float32)
     y0, y1 = torch.empty_like(x), torch.empty_like(x)
     b0, b1 = p.alloc(), p.alloc()
@@ -177,6 +177,7 @@ def main(argv=None):
     e.add_argument("--precision", default="fp16")
     e.add_argument("--sweeps", type=int, default=1)
     e.add_argument("--reps", type=int, default=20)
+    e.add_argument("--tile", type=int, nargs=2, default=None)
     s = sub.add_parser("sweep")
     s.add_argument("--stencils", nargs="+", default=["star2d1r", "star2d2r", "box2d1r", "box2d2r"])
     s.add_argument("--methods", nargs="+", default=["mfma", "direct"])
@@ -198,7 +199,8 @@ def main(argv=None):
         return 0
 
     if a.cmd == "edges":
-        print(json.dumps(run_edges(a.stencil, a.size, a.cols or a.size, a.precision, a.sweeps, a.reps)), flush=True)
+        print(json.dumps(run_edges(a.stencil, a.size, a.cols or a.size, a.precision, a.sweeps, a.reps,
+                                   tile=a.tile)), flush=True)
         return 0
 
     rows = []

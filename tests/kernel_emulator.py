@@ -187,6 +187,26 @@ def stage_tile(inp: np.ndarray, L: Layout, G: TileGeom, gr0: int, gc0: int, chec
     return lds
 
 
+def _tiles_band(L: Layout, TH: int, TW: int):
+    """The row-band schedule of the kernels that read row-major FP32 input:
+    tile rows in bands of 8 (padded with empty rows), virtual tile v of a band
+    at tile row 8 * band + v % 8, tile column (v % (8 tiles_x)) // 8; one
+    workgroup per virtual tile, on XCD v % 8, so tile row ty stays on XCD
+    ty % 8."""
+    tiles_x, tiles_y = L.nbc // TW, L.nbr // TH
+    nvirt = -(-tiles_y // XCD) * XCD * tiles_x
+    seen = []
+    for v in range(nvirt):
+        band, rem = divmod(v, XCD * tiles_x)
+        ty, tx = XCD * band + (rem & 7), rem >> 3
+        if ty >= tiles_y:
+            continue
+        assert tx < tiles_x and ty % XCD == v % XCD, "a tile row left its XCD"
+        seen.append(ty * tiles_x + tx)
+        yield 16 * TH * ty, 16 * TW * tx
+    assert sorted(seen) == list(range(tiles_y * tiles_x)), "schedule does not cover every tile exactly once"
+
+
 def _tiles(L: Layout, TH: int, TW: int, per_xcd: int = 0):
     """The tile schedule of tcs_sweep.hip: tile rows padded to tiles_xp (a
     multiple of 8) virtual columns, virtual tiles with tx >= tiles_x empty;
@@ -393,7 +413,7 @@ def mfma_edge(mode, src, rows, cols, ld, L: Layout, TH, TW, coeff5, rs, vec, inp
     m, g = np.arange(64) & 15, np.arange(64) >> 4
     OP = 16 * TW + 8  # pitch of the row-major FP16 output image (over the input image)
     assert 16 * TH * OP <= G.PLANE
-    for gr0, gc0 in _tiles(L, TH, TW, 0):
+    for gr0, gc0 in (_tiles_band(L, TH, TW) if mode & 1 else _tiles(L, TH, TW, 0)):
         lds = stage_tile_rm(src, rows, cols, ld, G, gr0, gc0, vec) if mode & 1 else stage_tile(inp, L, G, gr0, gc0)
         edge = not (gr0 >= rs and gr0 + 16 * TH <= L.rows - rs and gc0 >= rs and gc0 + 16 * TW <= L.cols - rs)
         oimg = np.full(G.PLANE, np.nan)

@@ -91,6 +91,8 @@ def test_persistent_schedule_covers_every_tile(ntile_shape):
     for per_xcd in (0, 1, 2, 3, 7, 1000):
         origins = list(E._tiles(L, TH, TW, per_xcd))
         assert len(origins) == len(set(origins)) == (L.nbr // TH) * (L.nbc // TW)
+    band = list(E._tiles_band(L, TH, TW))
+    assert len(band) == len(set(band)) == (L.nbr // TH) * (L.nbc // TW)
 
 
 @pytest.mark.parametrize("stencil", STENCILS)
