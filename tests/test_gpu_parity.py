@@ -712,6 +712,31 @@ def test_run_f32_fused_edges_bitwise(T, torch, precision):
                         assert torch.equal(got, want), (stencil, rows, cols, off, tile, sweeps, fuse)
 
 
+def test_run_f32_tiny_grids(T, torch, oracle):
+    """Grids smaller than one brick or one tile, in one or both directions:
+    tcs_run_f32 equals pack -> run -> unpack, and one sweep equals the oracle
+    (all ring where the interior is empty)."""
+    for stencil in STENCILS:
+        B = T.gen_coeffs(stencil, 3)
+        for rows, cols in ((1, 1), (5, 3), (3, 40), (17, 1), (33, 17), (6, 6)):
+            say("case run_f32 tiny", stencil, rows, cols)
+            x = T.gen_uniform(rows, cols, seed=rows * cols, dtype=torch.float32)
+            T.round_to(x, "fp16")
+            p = T.Problem(stencil, rows, cols, B)
+            b0, b1 = p.alloc(), p.alloc()
+            p.clear(b0)
+            p.clear(b1)
+            for sweeps in (1, 2):
+                got = p.run_f32(x, sweeps, buf0=b0, buf1=b1)
+                assert torch.equal(got, three_steps(T, torch, p, x, sweeps)), (stencil, rows, cols, sweeps)
+            got = p.run_f32(x, 1, buf0=b0, buf1=b1).cpu().numpy()
+            want = oracle.sweep_f64(stencil, x.cpu().numpy().astype(np.float64), B.astype(np.float64))
+            r = 1 if stencil in ("star2d1r", "box2d1r") else 2
+            if rows <= 2 * r or cols <= 2 * r:
+                assert np.array_equal(got, x.cpu().numpy())
+            assert np.abs(got - want).max() <= 2.0 ** -9, (stencil, rows, cols)
+
+
 def test_run_f32_row_strided_views(T, torch):
     """A grid that is a column slice of a wider array (leading dimension 160
     for 150 columns) gives the result of the contiguous grid, fused and in
