@@ -572,11 +572,11 @@ constexpr bool kHighOccupancy = PREC != TCS_PREC_SPLIT && TH * TW < 64 && !PERSI
 // (looked up only in edge tiles); nothing outside the grid.  Two 16-byte
 // stores when a.rm_vec allows and the piece lies inside the grid.
 template <int PREC>
-__device__ __forceinline__ void store_rm8(const SweepArgs &a, int gr, int gc, u32x4 u, bool edge) {
+__device__ __forceinline__ void store_rm4(const SweepArgs &a, int gr, int gc, u32x2 u, bool edge) {
     if (gr >= a.rows) return;
-    float v[8];
+    float v[4];
 #pragma unroll
-    for (int e = 0; e < 8; ++e) {
+    for (int e = 0; e < 4; ++e) {
         const uint16_t bits = (uint16_t)(u[e >> 1] >> (16 * (e & 1)));
         const float x = PREC == TCS_PREC_BF16 ? static_cast<float>(__builtin_bit_cast(__bf16, bits))
                                               : static_cast<float>(__builtin_bit_cast(_Float16, bits));
@@ -585,15 +585,14 @@ __device__ __forceinline__ void store_rm8(const SweepArgs &a, int gr, int gc, u3
     const int64_t row = gr * a.rm_ld;
     if (edge) {
 #pragma unroll
-        for (int e = 0; e < 8; ++e)
+        for (int e = 0; e < 4; ++e)
             if (gc + e < a.cols && !point_is_interior(a, gr, gc + e)) v[e] = a.rm_src[row + gc + e];
     }
-    if (a.rm_vec && gc + 8 <= a.cols) {
-        *reinterpret_cast<f32x4 *>(a.rm_dst + row + gc) = f32x4{v[0], v[1], v[2], v[3]};
-        *reinterpret_cast<f32x4 *>(a.rm_dst + row + gc + 4) = f32x4{v[4], v[5], v[6], v[7]};
+    if (a.rm_vec && gc + 4 <= a.cols) {
+        __builtin_nontemporal_store(f32x4{v[0], v[1], v[2], v[3]}, reinterpret_cast<f32x4 *>(a.rm_dst + row + gc));
     } else {
 #pragma unroll
-        for (int e = 0; e < 8; ++e)
+        for (int e = 0; e < 4; ++e)
             if (gc + e < a.cols) a.rm_dst[row + gc + e] = v[e];
     }
 }
@@ -676,9 +675,10 @@ __device__ __forceinline__ void sweep_tile16(const SweepArgs &a) {
         if constexpr (MODE & 2) {
             // Row-major output through LDS: the tile's results go into a
             // row-major FP16 image (pitch OP, over the input image, which every
-            // wave has finished reading), then every thread writes 8 results
-            // of one row -- 32 contiguous bytes, consecutive threads along the
-            // row -- instead of 16-byte pieces of 16 rows per brick.
+            // wave has finished reading), then every thread writes pieces of 4
+            // results of one row, consecutive threads along the row, so each
+            // store instruction of a wave writes whole tile rows (16TW * 4
+            // contiguous bytes) instead of 16-byte pieces of 16 rows per brick.
             constexpr int OP = 16 * TW + 8;
             static_assert(16 * TH * OP <= T::PLANE, "output image must fit the input image");
             __syncthreads();
@@ -689,14 +689,14 @@ __device__ __forceinline__ void sweep_tile16(const SweepArgs &a) {
                 *reinterpret_cast<u32x2 *>(lds + (16 * br + m) * OP + 16 * bc + 4 * g) = res[q];
             }
             __syncthreads();
+            constexpr int NP = 16 * TH * 4 * TW;  // 4-result pieces of the tile
+            static_assert(NP % kWG == 0, "whole passes of the workgroup");
 #pragma unroll
-            for (int i = 0; i < T::CPT; ++i) {
+            for (int i = 0; i < NP / kWG; ++i) {
                 const int c = tid + i * kWG;
-                if (T::NCH % kWG == 0 || c < T::NCH) {
-                    const int lrow = c / (2 * TW), cc = c - lrow * (2 * TW);
-                    const u32x4 u = *reinterpret_cast<const u32x4 *>(lds + lrow * OP + 8 * cc);
-                    store_rm8<PREC>(a, cgr0 + lrow, cgc0 + 8 * cc, u, edge);
-                }
+                const int lrow = c / (4 * TW), pc = c - lrow * (4 * TW);
+                const u32x2 u = *reinterpret_cast<const u32x2 *>(lds + lrow * OP + 4 * pc);
+                store_rm4<PREC>(a, cgr0 + lrow, cgc0 + 4 * pc, u, edge);
             }
         }
         // One-tile-per-workgroup launches cover every tile with the grid itself.

@@ -86,11 +86,13 @@ __global__ void tcs_pack_kernel(const T *__restrict__ src, int64_t ld, uint16_t 
 #pragma unroll
             for (int e = 0; e < 4; ++e) en[e] = encode<T>(v[e] * scale, prec);
             const int64_t d = brick + 16 * x;
-            *reinterpret_cast<u32x2 *>(dst + d) =
-                u32x2{(uint32_t)en[0].hi | ((uint32_t)en[1].hi << 16), (uint32_t)en[2].hi | ((uint32_t)en[3].hi << 16)};
+            __builtin_nontemporal_store(
+                u32x2{(uint32_t)en[0].hi | ((uint32_t)en[1].hi << 16), (uint32_t)en[2].hi | ((uint32_t)en[3].hi << 16)},
+                reinterpret_cast<u32x2 *>(dst + d));
             if (prec == TCS_PREC_SPLIT)
-                *reinterpret_cast<u32x2 *>(dst + plane + d) = u32x2{(uint32_t)en[0].lo | ((uint32_t)en[1].lo << 16),
-                                                                    (uint32_t)en[2].lo | ((uint32_t)en[3].lo << 16)};
+                __builtin_nontemporal_store(u32x2{(uint32_t)en[0].lo | ((uint32_t)en[1].lo << 16),
+                                                  (uint32_t)en[2].lo | ((uint32_t)en[3].lo << 16)},
+                                            reinterpret_cast<u32x2 *>(dst + plane + d));
         }
     }
 }
@@ -134,7 +136,7 @@ __global__ void tcs_unpack_kernel(const uint16_t *__restrict__ src, int64_t plan
                 }
             }
             if (vec && gc + 4 <= cols) {
-                *reinterpret_cast<f32x4 *>(dst + o) = f32x4{v[0], v[1], v[2], v[3]};
+                __builtin_nontemporal_store(f32x4{v[0], v[1], v[2], v[3]}, reinterpret_cast<f32x4 *>(dst + o));
             } else {
 #pragma unroll
                 for (int e = 0; e < 4; ++e)

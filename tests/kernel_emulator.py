@@ -434,29 +434,30 @@ def mfma_edge(mode, src, rows, cols, ld, L: Layout, TH, TW, coeff5, rs, vec, inp
         if not mode & 2:
             continue
         tid = np.arange(WG)
-        for i in range(G.CPT):  # store_rm8: 8 results of one row per thread
+        NP = 16 * TH * 4 * TW  # store_rm4: pieces of 4 results of one row per thread
+        assert NP % WG == 0
+        for i in range(NP // WG):
             c = tid + i * WG
-            c = c[c < G.NCH]
-            lrow, cc = c // (2 * TW), c % (2 * TW)
-            e0 = lrow * OP + 8 * cc
-            _check(e0, G.PLANE - 7, 8, "output image read (16 B)")
-            vals = oimg[e0[:, None] + np.arange(8)[None, :]]
+            lrow, pc = c // (4 * TW), c % (4 * TW)
+            e0 = lrow * OP + 4 * pc
+            _check(e0, G.PLANE - 3, 4, "output image read (8 B)")
+            vals = oimg[e0[:, None] + np.arange(4)[None, :]]
             assert not np.isnan(vals).any(), "output image read an unwritten element"
             for t in range(c.size):
-                gr, gc = gr0 + lrow[t], gc0 + 8 * cc[t]
+                gr, gc = gr0 + lrow[t], gc0 + 4 * pc[t]
                 if gr >= rows:
                     continue
                 v = vals[t].copy()
-                for e in range(8):
+                for e in range(4):
                     if gc + e < cols and not _interior(gr, gc + e, L, rs):
                         assert edge, "ring point in a tile not marked edge"
                         v[e] = src[gr * ld + gc + e]
                 base = gr * ld + gc
-                if vec and gc + 8 <= cols:
-                    _check(np.array([base]), n - 7, 4, "row-major vector store (2 x 16 B)")
-                    es = range(8)
+                if vec and gc + 4 <= cols:
+                    _check(np.array([base]), n - 3, 4, "row-major vector store (16 B)")
+                    es = range(4)
                 else:
-                    es = [e for e in range(8) if gc + e < cols]
+                    es = [e for e in range(4) if gc + e < cols]
                     _check(np.array([base + e for e in es], dtype=np.int64), n, 1, "row-major store")
                 for e in es:
                     assert not written[base + e], "two threads store the same element"
