@@ -566,11 +566,12 @@ constexpr bool kLookahead = PREC != TCS_PREC_SPLIT;
 template <int TH, int TW, int PREC, bool PERSIST>
 constexpr bool kHighOccupancy = PREC != TCS_PREC_SPLIT && TH * TW < 64 && !PERSIST;
 
-// Row-major FP32 output of 8 results of row gr, columns gc..gc+7 (FP16/BF16
+// Row-major FP32 output of 4 results of row gr, columns gc..gc+3 (FP16/BF16
 // bits u): what the unpack kernel writes -- the stored value times
 // rm_scale_out at interior points, the caller's rm_src value on the ring
-// (looked up only in edge tiles); nothing outside the grid.  Two 16-byte
-// stores when a.rm_vec allows and the piece lies inside the grid.
+// (looked up only in edge tiles); nothing outside the grid.  One nontemporal
+// 16-byte store when a.rm_vec allows and the piece lies inside the grid (the
+// wave's stores together cover whole rows of the tile).
 template <int PREC>
 __device__ __forceinline__ void store_rm4(const SweepArgs &a, int gr, int gc, u32x2 u, bool edge) {
     if (gr >= a.rows) return;
