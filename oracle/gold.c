@@ -158,6 +158,26 @@ double oracle_check_relative_error_2d(int N, const float *output, const float *r
     return sqrt(error / ((double)(y_ub - y_lb) * (double)(x_ub - x_lb)));
 }
 
+/* common/common.hpp:94-124 (checkRelativeError3D): the same relative error over
+ * the box [z_lb,z_ub)x[y_lb,y_ub)x[x_lb,x_ub) of an N x N x N volume (z the
+ * slowest index).  No stencil in the reference has a 3D kernel; the check is
+ * kept for parity of the utility. */
+double oracle_check_relative_error_3d(int N, const float *output, const float *reference, int z_lb,
+                                      int z_ub, int y_lb, int y_ub, int x_lb, int x_ub) {
+    const double min_number = 1e-6;
+    double error = 0.0;
+    for (int z = z_lb; z < z_ub; z++)
+        for (int y = y_lb; y < y_ub; y++)
+            for (int x = x_lb; x < x_ub; x++) {
+                const ptrdiff_t i = ((ptrdiff_t)z * N + y) * N + x;
+                const float rv = reference[i];
+                double e = (double)output[i] - (double)rv;
+                e = (fabs((double)rv) < min_number) ? e / min_number : e / (double)rv;
+                error += e * e;
+            }
+    return sqrt(error / ((double)(z_ub - z_lb) * (double)(y_ub - y_lb) * (double)(x_ub - x_lb)));
+}
+
 /* SplitMix64 (Steele, Lea, Flood 2014; public-domain C by S. Vigna):
  * output k of a stream seeded with x is mix64(x + (k-1)*gamma). */
 uint64_t oracle_mix64(uint64_t z) {

@@ -66,6 +66,9 @@ double oracle_check_error_2d(int N, const float *output, const float *reference,
                              int y_lb, int y_ub, int x_lb, int x_ub);
 double oracle_check_relative_error_2d(int N, const float *output, const float *reference,
                                       int y_lb, int y_ub, int x_lb, int x_ub);
+/* Restatement of common/common.hpp:94-124 (checkRelativeError3D), without the printing. */
+double oracle_check_relative_error_3d(int N, const float *output, const float *reference, int z_lb,
+                                      int z_ub, int y_lb, int y_ub, int x_lb, int x_ub);
 
 /* SplitMix64 finaliser and the counter-based generators the product uses. */
 uint64_t oracle_mix64(uint64_t z);

@@ -58,6 +58,8 @@ def lib() -> ctypes.CDLL:
         L.oracle_check_error_2d.restype = ctypes.c_double
         L.oracle_check_relative_error_2d.argtypes = [ctypes.c_int, _F32P, _F32P] + [ctypes.c_int] * 4
         L.oracle_check_relative_error_2d.restype = ctypes.c_double
+        L.oracle_check_relative_error_3d.argtypes = [ctypes.c_int, _F32P, _F32P] + [ctypes.c_int] * 6
+        L.oracle_check_relative_error_3d.restype = ctypes.c_double
         L.oracle_mix64.argtypes = [ctypes.c_uint64]
         L.oracle_mix64.restype = ctypes.c_uint64
         L.oracle_gen_uniform_f32.argtypes = [_F32P, ctypes.c_int, ctypes.c_int, ctypes.c_uint64,
@@ -130,6 +132,15 @@ def check_error_2d(N, output, reference, y_lb, y_ub, x_lb, x_ub) -> float:
 
 def check_relative_error_2d(N, output, reference, y_lb, y_ub, x_lb, x_ub) -> float:
     return lib().oracle_check_relative_error_2d(N, _p32(output), _p32(reference), y_lb, y_ub, x_lb, x_ub)
+
+
+def check_relative_error_3d(N, output, reference, z_lb, z_ub, y_lb, y_ub, x_lb, x_ub) -> float:
+    """common/common.hpp:94-124 on an N x N x N volume (z slowest)."""
+    output = np.ascontiguousarray(output, dtype=np.float32)
+    reference = np.ascontiguousarray(reference, dtype=np.float32)
+    assert output.size == reference.size == N ** 3
+    return lib().oracle_check_relative_error_3d(N, _p32(output), _p32(reference), z_lb, z_ub, y_lb, y_ub,
+                                                x_lb, x_ub)
 
 
 def mix64(z: int) -> int:

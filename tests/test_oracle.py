@@ -165,6 +165,32 @@ def test_reference_style_norms(oracle):
     np.testing.assert_allclose(oracle.check_relative_error_2d(N, out, tiny, 0, N, 0, N), 1.0, rtol=1e-6)
 
 
+def test_reference_style_relative_error_3d(oracle):
+    """checkRelativeError3D: closed forms, the MIN_NUMBER floor, and agreement
+    with the 2D check on a one-slice box (a transposed index would break it)."""
+    N = 10
+    ref = np.full((N, N, N), 4.0, dtype=np.float32)
+    out = ref + np.float32(1.0)
+    assert oracle.check_relative_error_3d(N, out, ref, 1, N - 1, 1, N - 1, 1, N - 1) == 0.25
+    tiny = np.zeros((N, N, N), dtype=np.float32)
+    np.testing.assert_allclose(oracle.check_relative_error_3d(N, tiny + np.float32(2e-6), tiny, 0, N, 0, N, 0, N),
+                               2.0, rtol=1e-6)
+    rng = np.random.default_rng(3)
+    ref = rng.uniform(0.5, 1.5, (N, N, N)).astype(np.float32)
+    out = (ref * rng.uniform(0.9, 1.1, (N, N, N))).astype(np.float32)
+    z = 6
+    got = oracle.check_relative_error_3d(N, out, ref, z, z + 1, 2, 7, 1, 9)
+    want = oracle.check_relative_error_2d(N, out[z], ref[z], 2, 7, 1, 9)
+    assert got == want
+    # only the chosen box counts: damage outside it changes nothing
+    out2 = out.copy()
+    out2[0] = 100.0
+    assert oracle.check_relative_error_3d(N, out2, ref, z, z + 1, 2, 7, 1, 9) == got
+    sub = (out[2:5, 3:8, 1:4].astype(np.float64) - ref[2:5, 3:8, 1:4]) / ref[2:5, 3:8, 1:4]
+    np.testing.assert_allclose(oracle.check_relative_error_3d(N, out, ref, 2, 5, 3, 8, 1, 4),
+                               np.sqrt((sub ** 2).mean()), rtol=1e-12)
+
+
 def test_splitmix64_published_vector(oracle):
     # First outputs of SplitMix64 seeded with 1234567 (Vigna's reference C).
     assert oracle.mix64(1234567) == 6457827717110365317
