@@ -66,18 +66,23 @@ __global__ void tcs_pack_kernel(const T *__restrict__ src, int64_t ld, uint16_t 
         const int bj = kSpanBricks * (int)(span - (int64_t)bi * spans_per_row) + (c4 >> 2);
         if (bj >= sbc) continue;
         const int cb = 4 * (c4 & 3), gc = 16 * bj - 16 + cb;
-        const bool col_vec = sizeof(T) == 4 && vec && gc >= 0 && gc + 4 <= cols;
+        const bool col_vec = vec && gc >= 0 && gc + 4 <= cols;  // FP32 sources only (the host sets vec)
         const int64_t brick = ((int64_t)bi * sbc + bj) * 256 + cb;
 #pragma unroll
         for (int k = 0; k < 4; ++k) {
             const int x = 4 * k + (l >> 4), gr = 16 * bi - 16 + x;
             const bool row_in = (unsigned)gr < (unsigned)rows;
             T v[4];
-            if (row_in && col_vec) {
-                const f32x4 w = *reinterpret_cast<const f32x4 *>(src + gr * ld + gc);
+            bool loaded = false;
+            if constexpr (sizeof(T) == 4) {
+                if (row_in && col_vec) {
+                    const f32x4 w = *reinterpret_cast<const f32x4 *>(src + gr * ld + gc);
 #pragma unroll
-                for (int e = 0; e < 4; ++e) v[e] = (T)w[e];
-            } else {
+                    for (int e = 0; e < 4; ++e) v[e] = w[e];
+                    loaded = true;
+                }
+            }
+            if (!loaded) {
 #pragma unroll
                 for (int e = 0; e < 4; ++e)
                     v[e] = row_in && (unsigned)(gc + e) < (unsigned)cols ? src[gr * ld + gc + e] : (T)0;
