@@ -386,6 +386,28 @@ def test_full_size_box2d2r_16384(T, torch, method):
     assert res["max_abs"] <= (2.0 ** -11) * 2.0 * 1.001 + 1e-6
 
 
+@pytest.mark.parametrize("stencil", ["box2d2r", "star2d2r"])
+def test_full_size_split_16384(T, torch, stencil):
+    """The split hi+lo form at the 16384^2 size, with per-grid max-abs scaling:
+    rel-L2 vs the GPU FP64 oracle <= 1e-6 after one sweep and after ten, on the
+    sinusoid input (FP64, not rounded: the split pair carries ~22 bits)."""
+    say("case full size split", stencil)
+    N = 16384
+    B = T.gen_coeffs(stencil, 1)
+    x64 = T.gen_sinusoid(N, N, seed=1, noise=1e-3)
+    e = T.split_scale_exp(T.maxabs(x64))
+    p = T.Problem(stencil, N, N, B, precision="split")
+    b0, b1 = p.alloc(), p.alloc()
+    p.clear(b1)
+    p.pack(x64, b0, e)
+    for sweeps in (1, 10):
+        ref = T.oracle_sweeps_f64(stencil, x64, B.astype(np.float64), sweeps)
+        out = p.unpack(p.run(b0, b1, sweeps), scale_exp=e)  # b0 keeps the input: run reads it first
+        res = T.error_norms(out, ref, 2)
+        say("  ", sweeps, res)
+        assert res["rel_l2"] <= 1e-6, (sweeps, res)
+
+
 # ------------------------------------------------------- temporal blocking
 FUSED_TILES = [(2, 4), (4, 4), (4, 8), (8, 8)]
 
