@@ -184,3 +184,34 @@ def test_frag32_setup_and_validation():
         T.Problem("box2d2r", 64, 64, B, method="direct", frag=32)
     with pytest.raises(T.TcsError, match="fragment"):
         T.Problem("box2d2r", 64, 64, B, frag=24)
+
+
+def test_error_messages_are_per_thread():
+    """tcs_last_error is thread-local: an error raised on one thread does not
+    replace the message another thread is about to read (the C ABI keeps no
+    shared mutable state)."""
+    import threading
+
+    import tcstencil_amd as T
+    L = T.lib()
+    first, second = threading.Barrier(2), threading.Barrier(2)
+    seen = {}
+
+    def worker(name, bad_align):
+        lay = T._lib.Layout()
+        if name == "a":
+            assert L.tcs_layout_init(ctypes.byref(lay), 64, 64, bad_align) == T._lib.ERR_INVALID
+            first.wait()   # b now raises its own error
+            second.wait()
+        else:
+            first.wait()
+            assert L.tcs_layout_init(ctypes.byref(lay), 64, 64, bad_align) == T._lib.ERR_INVALID
+            second.wait()
+        seen[name] = L.tcs_last_error().decode()
+
+    threads = [threading.Thread(target=worker, args=("a", 3)), threading.Thread(target=worker, args=("b", 5))]
+    for t in threads:
+        t.start()
+    for t in threads:
+        t.join()
+    assert "align 3" in seen["a"] and "align 5" in seen["b"], seen
