@@ -12,7 +12,8 @@ update (4 B FP16/BF16, 8 B SPLIT, divided by the sweeps fused per launch)
 against 8 TB/s.
 
   c1  star2d1r 1024^2, FP16 U[-1,1), one sweep, MFMA and direct
-  c2  box2d1r 10240^2, FP16 U[-1,1), 100 sweeps, MFMA vs direct, errors after 1/10/100
+  c2  box2d1r 10240^2, FP16 U[-1,1), 100 sweeps, MFMA vs direct, errors after 1/10/100;
+      the loop also fused 8/16 per launch and captured as one hipGraph
   c3  star2d2r and box2d2r 16384^2, 4 sinusoids + noise, 10 sweeps, FP16/BF16/SPLIT MFMA
   c4  512 batched uneven grids (box2d1r/star2d2r, 64..2048 per side), 20 sweeps in a
       hipGraph: tile, MFMA fragment, sweeps per launch and waves per CU autotuned per
@@ -165,6 +166,10 @@ def c2():
         emit(config="c2", **rec)
     for fuse in (8, 16):
         p, rec = _measure(stencil, N, N, B, x, "mfma", "fp16", 100, fuse=fuse)
+        emit(config="c2", **rec)
+    # the 100-sweep loop captured once as a hipGraph: no launch gaps between sweeps
+    for method, fuse in (("mfma", 1), ("direct", 1), ("mfma", 16)):
+        p, rec = _measure(stencil, N, N, B, x, method, "fp16", 100, fuse=fuse, graph=True)
         emit(config="c2", **rec)
     emit(config="c2", mfma_vs_direct=round(res["mfma"] / res["direct"], 3))
 
