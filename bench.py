@@ -95,6 +95,28 @@ def cpu_baseline(np):
                       f"host {os.cpu_count()} logical CPUs, {cpu_model()}"}
 
 
+def cpu_baseline_threads(np, threads):
+    """The oracle's restatement of gold_box2d2r.cpp on row bands, one thread
+    per band (`threads` host cores), eight ping-pong 16384^2 sweeps: a multi-core
+    CPU figure beside the single-thread reference one (single-threaded, the
+    restatement and the reference's -O3 build run at the same rate: the loop
+    does not vectorise)."""
+    from oracle import oracle as O
+    A = np.random.default_rng(SEED).uniform(-1, 1, (N, N)).astype(np.float32)
+    B = np.full((5, 5), 1.0 / 25.0, dtype=np.float32)
+    C = A.copy()
+    sweeps = 8
+    t0 = time.perf_counter()
+    for _ in range(sweeps):
+        O.sweep_f32_threads(STENCIL, A, B, C, threads)
+        A, C = C, A
+    dt = time.perf_counter() - t0
+    return {"value": round(sweeps * (N - 4) ** 2 / dt / 1e9, 4), "unit": "GStencil/s", "cores": threads,
+            "kind": "port",
+            "sample": f"{sweeps} box2d2r sweeps of {N}x{N} FP32 on {threads} row bands, {dt:.1f} s "
+                      f"(oracle/_build/liboracle.so), host {os.cpu_count()} logical CPUs, {cpu_model()}"}
+
+
 def cpu_model():
     """The host CPU's model name (Linux /proc/cpuinfo), or 'unknown CPU'."""
     try:
@@ -169,11 +191,13 @@ def main():
     mfma_tflops = m["mfma_per_brick"] * bricks * (16 * 16 * 32 * 2) / (m["ms"] * 1e-3) / 1e12
     traffic = read_traffic()
 
-    cpu = None
+    cpu = cpu_mt = None
     if not args.no_cpu:
-        log("bench: timing the CPU gold (one full sweep, single thread) ...")
+        log("bench: timing the CPU gold (two full sweeps, single thread) ...")
         cpu = cpu_baseline(np)
         log(f"bench: CPU {cpu['value']} GStencil/s ({cpu['sample']})")
+        cpu_mt = cpu_baseline_threads(np, min(16, os.cpu_count() or 1))
+        log(f"bench: CPU, all host cores {cpu_mt['value']} GStencil/s ({cpu_mt['sample']})")
 
     line = {
         "metric": METRIC,
@@ -209,6 +233,7 @@ def main():
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic},
         "cpu_baseline": cpu,
+        "cpu_baseline_threads": cpu_mt,
         "gpu_clocks": "default power management, not locked (the reference locks clocks, README.md:7)",
     }
     print(json.dumps(line), flush=True)

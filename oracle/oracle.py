@@ -90,6 +90,35 @@ def gold(stencil: str, A: np.ndarray, B: np.ndarray, C: np.ndarray | None = None
     return C
 
 
+def sweep_f32_threads(stencil: str, A: np.ndarray, B: np.ndarray, C: np.ndarray, threads: int) -> None:
+    """One FP32 sweep A -> C (C seeded with A's ring by the caller) by the
+    oracle's loop on `threads` bands of rows, one Python thread per band (the
+    ctypes call releases the interpreter lock).  Each band is the oracle's
+    rectangular sweep on rows [lo - r, hi + r) of A, which writes rows
+    [lo, hi) of C: the same arithmetic per point as sweep_f32."""
+    import threading
+    assert A.dtype == C.dtype == np.float32 and A.flags.c_contiguous and C.flags.c_contiguous
+    B = np.ascontiguousarray(B, dtype=np.float32)
+    rows, cols = A.shape
+    r = RADIUS[stencil]
+    edges = np.linspace(r, rows - r, threads + 1).astype(int)
+    L, sid = lib(), STENCIL_ID[stencil]
+    rcs = []
+
+    def band(lo, hi):
+        off = (lo - r) * cols * 4
+        a = ctypes.cast(A.ctypes.data + off, _F32P)
+        c = ctypes.cast(C.ctypes.data + off, _F32P)
+        rcs.append(L.oracle_sweep_f32(sid, a, _p32(B), c, hi - lo + 2 * r, cols))
+
+    ts = [threading.Thread(target=band, args=(int(lo), int(hi))) for lo, hi in zip(edges[:-1], edges[1:]) if hi > lo]
+    for t in ts:
+        t.start()
+    for t in ts:
+        t.join()
+    assert rcs and all(rc == 0 for rc in rcs)
+
+
 def sweep_f32(stencil: str, A: np.ndarray, B: np.ndarray) -> np.ndarray:
     """One FP32 sweep on a rows x cols grid; the ring keeps A's values."""
     A = np.ascontiguousarray(A, dtype=np.float32)

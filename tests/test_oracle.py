@@ -205,3 +205,18 @@ def test_uniform_generator_range(oracle):
     k = (x.astype(np.float64) + 1.0) * 2.0 ** 23
     assert np.array_equal(k, np.round(k))
     assert not np.array_equal(x, oracle.gen_uniform_f32(64, 48, seed=2, stream=0, lo=-1.0, hi=1.0))
+
+
+@pytest.mark.parametrize("stencil", ["star2d1r", "star2d2r", "box2d1r", "box2d2r"])
+def test_threaded_bands_equal_one_sweep(oracle, stencil):
+    """The row-band threaded sweep (the multi-core CPU baseline) is bitwise the
+    single-threaded oracle sweep, for band counts that do and do not divide the
+    interior (and more bands than rows)."""
+    rng = np.random.default_rng(11)
+    A = rng.uniform(-1, 1, (97, 61)).astype(np.float32)
+    B = rng.uniform(0, 1, (5, 5) if stencil.endswith("2r") else (3, 3)).astype(np.float32)
+    want = oracle.sweep_f32(stencil, A, B)
+    for threads in (1, 3, 7, 200):
+        C = A.copy()
+        oracle.sweep_f32_threads(stencil, A, B, C, threads)
+        assert np.array_equal(C, want), threads
