@@ -138,8 +138,8 @@ int tcs_problem_init(tcs_problem *p, int stencil, int method, int precision, int
                      const float *B, int n);
 int tcs_radius(int stencil);
 /* Choose the workgroup tile (bricks, rows x cols): 2x4, 4x4, 2x8, 4x8 or 8x8.
- * Single-sweep kernels exist for all but SPLIT 8x8, fused kernels for 2x4, 4x4,
- * 4x8 and 8x8; a missing combination fails at launch with TCS_ERR_UNSUPPORTED
+ * Single-sweep kernels exist for all but SPLIT 8x8, fused kernels for all five
+ * (16x16 fragments); a missing combination fails at launch with TCS_ERR_UNSUPPORTED
  * (tcs_kernel_name / tcs_fused_kernel_name return NULL for it).  Re-derives the
  * layout's alignment, so buffers packed for another tile must be packed again. */
 int tcs_problem_set_tile(tcs_problem *p, int tile_h, int tile_w);

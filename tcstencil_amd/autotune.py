@@ -35,8 +35,8 @@ def tune(stencil, rows, cols, method="mfma", precision="fp16", sweeps=20, warmup
     for tile, wg in itertools.product(tiles or TILES, wgs):
         if fuse == 1 and precision == "split" and tile == (8, 8):
             continue  # no unfused SPLIT 8x8 tile
-        if fuse > 1 and (tile == (2, 8) or wg):
-            continue  # fused launches: tiles 2x4, 4x4, 4x8, 8x8, one tile per workgroup
+        if fuse > 1 and wg:
+            continue  # fused launches: one tile per workgroup
         p = T.Problem(stencil, rows, cols, B, method=method, precision=precision, tile=tile, wg_per_cu=wg,
                       fuse=fuse)
         b0, b1 = p.alloc(), p.alloc()

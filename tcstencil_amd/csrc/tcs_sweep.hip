@@ -1458,7 +1458,7 @@ static const KernelEntry *select_kernel(const tcs_problem *p) {
     return nullptr;
 }
 
-// Fused (temporally blocked) MFMA kernels: tiles 2x4, 4x4, 4x8, 8x8 (16x16
+// Fused (temporally blocked) MFMA kernels: tiles 2x4, 4x4, 2x8, 4x8, 8x8 (16x16
 // fragments), 4x4, 4x8, 8x8 (32x32 fragments, FP16 and BF16).
 typedef void (*fused_fn)(SweepArgs, int);
 struct FusedEntry {
@@ -1474,8 +1474,8 @@ struct FusedEntry {
 #define TCS_FUSED32_TILES(ZLO, ZHI, P, R) \
     TCS_FUSED32(4, 4, ZLO, ZHI, P, R), TCS_FUSED32(4, 8, ZLO, ZHI, P, R), TCS_FUSED32(8, 8, ZLO, ZHI, P, R)
 #define TCS_FUSED_TILES(ZLO, ZHI, P, R)                                                           \
-    TCS_FUSED(2, 4, ZLO, ZHI, P, R), TCS_FUSED(4, 4, ZLO, ZHI, P, R), TCS_FUSED(4, 8, ZLO, ZHI, P, R), \
-        TCS_FUSED(8, 8, ZLO, ZHI, P, R)
+    TCS_FUSED(2, 4, ZLO, ZHI, P, R), TCS_FUSED(4, 4, ZLO, ZHI, P, R), TCS_FUSED(2, 8, ZLO, ZHI, P, R), \
+        TCS_FUSED(4, 8, ZLO, ZHI, P, R), TCS_FUSED(8, 8, ZLO, ZHI, P, R)
 static const FusedEntry kFused[] = {
     TCS_FUSED_TILES(0, 4, 0, 2), TCS_FUSED_TILES(1, 3, 0, 1), TCS_FUSED_TILES(0, 4, 1, 2),
     TCS_FUSED_TILES(1, 3, 1, 1), TCS_FUSED_TILES(0, 4, 2, 2), TCS_FUSED_TILES(1, 3, 2, 1),

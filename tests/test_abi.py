@@ -109,7 +109,7 @@ def test_invalid_arguments_return_status():
     p.set_tile(8, 8)
     assert p.kernel_name == "" and p.fused_kernel_name.startswith("tcs_mfma_fused<8, 8")
     p.set_tile(2, 8)
-    assert p.fused_kernel_name == "" and p.kernel_name.startswith("tcs_mfma_sweep<2, 8")
+    assert p.fused_kernel_name.startswith("tcs_mfma_fused<2, 8") and p.kernel_name.startswith("tcs_mfma_sweep<2, 8")
     with pytest.raises(T.TcsError, match="not offered"):
         p.set_tile(3, 8)
     with pytest.raises(T.TcsError, match="non-finite"):

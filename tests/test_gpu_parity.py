@@ -409,7 +409,7 @@ def test_full_size_split_16384(T, torch, stencil):
 
 
 # ------------------------------------------------------- temporal blocking
-FUSED_TILES = [(2, 4), (4, 4), (4, 8), (8, 8)]
+FUSED_TILES = [(2, 4), (4, 4), (2, 8), (4, 8), (8, 8)]
 
 
 def run_buffers(T, torch, p, A, sweeps, scale_exp=0, graph=False):
@@ -726,7 +726,7 @@ def test_run_f32_fused_edges_bitwise(T, torch, precision):
                     b0, b1 = p.alloc(), p.alloc()
                     p.clear(b0)
                     p.clear(b1)
-                    for sweeps, fuse in ((1, 1), (2, 1), (3, 1), (4, 1 if tile == (2, 8) else 2)):  # no fused 2x8
+                    for sweeps, fuse in ((1, 1), (2, 1), (3, 1), (4, 2)):
                         say("case run_f32", stencil, precision, rows, cols, off, tile, sweeps, fuse)
                         p.fuse = fuse
                         got = p.run_f32(src, sweeps, buf0=b0, buf1=b1)
