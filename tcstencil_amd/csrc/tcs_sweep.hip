@@ -1621,7 +1621,7 @@ struct EdgeEntry {
     TCS_EDGE(TH, TW, ZLO, ZHI, P, R, 1), TCS_EDGE(TH, TW, ZLO, ZHI, P, R, 2), TCS_EDGE(TH, TW, ZLO, ZHI, P, R, 3)
 #define TCS_EDGE_TILES(ZLO, ZHI, P, R)                                                                   \
     TCS_EDGE_MODES(2, 4, ZLO, ZHI, P, R), TCS_EDGE_MODES(4, 4, ZLO, ZHI, P, R), TCS_EDGE_MODES(2, 8, ZLO, ZHI, P, R), \
-        TCS_EDGE_MODES(4, 8, ZLO, ZHI, P, R)
+        TCS_EDGE_MODES(4, 8, ZLO, ZHI, P, R), TCS_EDGE_MODES(8, 8, ZLO, ZHI, P, R)
 static const EdgeEntry kEdge[] = {
     TCS_EDGE_TILES(0, 4, 0, 2), TCS_EDGE_TILES(1, 3, 0, 1), TCS_EDGE_TILES(0, 4, 1, 2), TCS_EDGE_TILES(1, 3, 1, 1),
 };

@@ -721,7 +721,7 @@ def test_run_f32_fused_edges_bitwise(T, torch, precision):
             for off in (0, 1):
                 src = base[off:off + rows * cols].view(rows, cols)
                 src.copy_(T.gen_uniform(rows, cols, seed=rows + cols, dtype=torch.float32))
-                for tile in ((2, 4), (4, 4), (2, 8), (4, 8)):
+                for tile in ((2, 4), (4, 4), (2, 8), (4, 8), (8, 8)):
                     p = T.Problem(stencil, rows, cols, B, precision=precision, tile=tile)
                     b0, b1 = p.alloc(), p.alloc()
                     p.clear(b0)
@@ -784,15 +784,15 @@ def test_run_f32_row_strided_views(T, torch):
 
 
 def test_run_f32_other_problems_run_three_steps(T, torch):
-    """Problems without an edge kernel (SPLIT, direct, 8x8 tiles, 32x32
-    fragments, persistent grids) take the three-step path through tcs_run_f32;
-    a fused single sweep needs no storage buffers."""
+    """Problems without an edge kernel (SPLIT, direct, 32x32 fragments,
+    persistent grids) take the three-step path through tcs_run_f32; a fused
+    single sweep needs no storage buffers."""
     rows, cols = 70, 150
     B = T.gen_coeffs("box2d2r", 3)
     src = T.gen_uniform(rows, cols, seed=9, dtype=torch.float32)
     e = T.split_scale_exp(1.0)
-    for kw, scale in (({"precision": "split"}, e), ({"method": "direct"}, 0), ({"tile": (8, 8)}, 0),
-                      ({"frag": 32}, 0), ({"wg_per_cu": 2}, 0)):
+    for kw, scale in (({"precision": "split"}, e), ({"method": "direct"}, 0), ({"frag": 32}, 0),
+                      ({"wg_per_cu": 2}, 0)):
         say("case run_f32 three steps", kw)
         p = T.Problem("box2d2r", rows, cols, B, **kw)
         b0, b1 = p.alloc(), p.alloc()
