@@ -158,14 +158,18 @@ __device__ __forceinline__ void load_tile(const SweepArgs &a, TileRegs<G> &R, in
 }
 
 // Storage-format bits of a row-major FP32 value (times rm_scale_in), rounded
-// to nearest as the pack kernel rounds: FP16 or BF16.
+// to nearest as the pack kernel rounds: FP16 or BF16, or for SPLIT the FP16
+// pair hi = fp16(x), lo = fp16(x - hi) (plane 0 and plane 1).
 template <int PREC>
-__device__ __forceinline__ uint16_t rm_encode(const SweepArgs &a, float x) {
+__device__ __forceinline__ uint16_t rm_encode(const SweepArgs &a, float x, int plane = 0) {
     x *= a.rm_scale_in;
-    if constexpr (PREC == TCS_PREC_BF16)
+    if constexpr (PREC == TCS_PREC_BF16) {
         return __builtin_bit_cast(uint16_t, static_cast<__bf16>(x));
-    else
-        return __builtin_bit_cast(uint16_t, static_cast<_Float16>(x));
+    } else {
+        const _Float16 h = static_cast<_Float16>(x);
+        if (plane == 0) return __builtin_bit_cast(uint16_t, h);
+        return __builtin_bit_cast(uint16_t, static_cast<_Float16>(x - static_cast<float>(h)));
+    }
 }
 
 // load_tile from the row-major FP32 grid a.rm_src instead of bricks (FP16/BF16,
@@ -177,7 +181,7 @@ __device__ __forceinline__ uint16_t rm_encode(const SweepArgs &a, float x) {
 // the grid); write_tile<G, true> stores them with the same order.
 template <class G, int PREC>
 __device__ __forceinline__ void load_tile_rm(const SweepArgs &a, TileRegs<G> &R, int r0, int c0, int tid) {
-    static_assert(G::NPL == 1, "row-major edges: FP16/BF16 storage");
+    static_assert(G::NPL == 1 || PREC == TCS_PREC_SPLIT, "one storage plane, or the SPLIT pair");
 #pragma unroll
     for (int i = 0; i < G::CPT; ++i) {
         const int c = tid + i * kWG;
@@ -200,8 +204,10 @@ __device__ __forceinline__ void load_tile_rm(const SweepArgs &a, TileRegs<G> &R,
                     x[e] = gr < a.rows && gc + e < a.cols ? a.rm_src[gr * a.rm_ld + gc + e] : 0.0f;
             }
 #pragma unroll
-            for (int q = 0; q < 4; ++q)
-                R.cv[0][i][q] = pack2(rm_encode<PREC>(a, x[2 * q]), rm_encode<PREC>(a, x[2 * q + 1]));
+            for (int p = 0; p < G::NPL; ++p)
+#pragma unroll
+                for (int q = 0; q < 4; ++q)
+                    R.cv[p][i][q] = pack2(rm_encode<PREC>(a, x[2 * q], p), rm_encode<PREC>(a, x[2 * q + 1], p));
         }
     }
 #pragma unroll
@@ -216,7 +222,8 @@ __device__ __forceinline__ void load_tile_rm(const SweepArgs &a, TileRegs<G> &R,
 #pragma unroll
             for (int e = 0; e < 2; ++e)
                 x[e] = row_in && (unsigned)(gc + e) < (unsigned)a.cols ? a.rm_src[gr * a.rm_ld + gc + e] : 0.0f;
-            R.hv[0][i] = pack2(rm_encode<PREC>(a, x[0]), rm_encode<PREC>(a, x[1]));
+#pragma unroll
+            for (int p = 0; p < G::NPL; ++p) R.hv[p][i] = pack2(rm_encode<PREC>(a, x[0], p), rm_encode<PREC>(a, x[1], p));
         }
     }
 }
@@ -567,20 +574,22 @@ template <int TH, int TW, int PREC, bool PERSIST>
 constexpr bool kHighOccupancy = PREC != TCS_PREC_SPLIT && TH * TW < 64 && !PERSIST;
 
 // Row-major FP32 output of 4 results of row gr, columns gc..gc+3 (FP16/BF16
-// bits u): what the unpack kernel writes -- the stored value times
+// bits u; SPLIT: the hi bits u and lo bits ul): what the unpack kernel writes -- the stored value times
 // rm_scale_out at interior points, the caller's rm_src value on the ring
 // (looked up only in edge tiles); nothing outside the grid.  One nontemporal
 // 16-byte store when a.rm_vec allows and the piece lies inside the grid (the
 // wave's stores together cover whole rows of the tile).
 template <int PREC>
-__device__ __forceinline__ void store_rm4(const SweepArgs &a, int gr, int gc, u32x2 u, bool edge) {
+__device__ __forceinline__ void store_rm4(const SweepArgs &a, int gr, int gc, u32x2 u, u32x2 ul, bool edge) {
     if (gr >= a.rows) return;
     float v[4];
 #pragma unroll
     for (int e = 0; e < 4; ++e) {
         const uint16_t bits = (uint16_t)(u[e >> 1] >> (16 * (e & 1)));
-        const float x = PREC == TCS_PREC_BF16 ? static_cast<float>(__builtin_bit_cast(__bf16, bits))
-                                              : static_cast<float>(__builtin_bit_cast(_Float16, bits));
+        float x = PREC == TCS_PREC_BF16 ? static_cast<float>(__builtin_bit_cast(__bf16, bits))
+                                        : static_cast<float>(__builtin_bit_cast(_Float16, bits));
+        if constexpr (PREC == TCS_PREC_SPLIT)  // hi + lo, as the unpack kernel decodes the pair
+            x += static_cast<float>(__builtin_bit_cast(_Float16, (uint16_t)(ul[e >> 1] >> (16 * (e & 1)))));
         v[e] = x * a.rm_scale_out;
     }
     const int64_t row = gr * a.rm_ld;
@@ -605,7 +614,7 @@ template <int TH, int TW, int ZLO, int ZHI, int PREC, bool PERSIST, int MODE>
 __device__ __forceinline__ void sweep_tile16(const SweepArgs &a) {
     constexpr int NPL = PREC == TCS_PREC_SPLIT ? 2 : 1;
     constexpr bool LOOKAHEAD = kLookahead<PREC> && !kHighOccupancy<TH, TW, PREC, PERSIST>;
-    static_assert(MODE == 0 || (!PERSIST && NPL == 1), "row-major edges: one tile per workgroup, FP16/BF16");
+    static_assert(MODE == 0 || !PERSIST, "row-major edges: one tile per workgroup");
     using T = Tile<TH, TW, NPL, true>;
     __shared__ __attribute__((aligned(16))) uint16_t lds[NPL * T::PLANE];
 
@@ -628,7 +637,7 @@ __device__ __forceinline__ void sweep_tile16(const SweepArgs &a) {
     C.load(a.cw, m, g);
     const typename Ops<T, ZLO, ZHI, PREC>::Offsets off(m, g);
     constexpr int NB = TH * TW / 4;
-    u32x2 res[MODE & 2 ? NB : 1];  // results held for the row-major output
+    u32x2 res[MODE & 2 ? NB : 1][NPL];  // results held for the row-major output
 
     for (;;) {
         write_tile<T, (MODE & 1) != 0>(R, lds, tid);
@@ -665,7 +674,8 @@ __device__ __forceinline__ void sweep_tile16(const SweepArgs &a) {
             u32x2 w[NPL];
             mfma_brick<T, ZLO, ZHI, PREC>(a, lds, C, cur, br, bc, m, g, edge, gr, gcb, w);
             if constexpr (MODE & 2) {
-                res[q] = w[0];
+#pragma unroll
+                for (int p = 0; p < NPL; ++p) res[q][p] = w[p];
             } else {
                 const int64_t s = stor_index(gr, gcb, a.sbc);
 #pragma unroll
@@ -687,7 +697,10 @@ __device__ __forceinline__ void sweep_tile16(const SweepArgs &a) {
             for (int q = 0; q < NB; ++q) {
                 int br, bc;
                 brick_of<TW, 0>(wave + 4 * q, br, bc);
-                *reinterpret_cast<u32x2 *>(lds + (16 * br + m) * OP + 16 * bc + 4 * g) = res[q];
+#pragma unroll
+                for (int p = 0; p < NPL; ++p)  // plane p's image over input plane p
+                    *reinterpret_cast<u32x2 *>(lds + p * T::PLANE + (16 * br + m) * OP + 16 * bc + 4 * g) =
+                        res[q][p];
             }
             __syncthreads();
             constexpr int NP = 16 * TH * 4 * TW;  // 4-result pieces of the tile
@@ -697,7 +710,9 @@ __device__ __forceinline__ void sweep_tile16(const SweepArgs &a) {
                 const int c = tid + i * kWG;
                 const int lrow = c / (4 * TW), pc = c - lrow * (4 * TW);
                 const u32x2 u = *reinterpret_cast<const u32x2 *>(lds + lrow * OP + 4 * pc);
-                store_rm4<PREC>(a, cgr0 + lrow, cgc0 + 4 * pc, u, edge);
+                u32x2 ul = {0u, 0u};
+                if constexpr (NPL == 2) ul = *reinterpret_cast<const u32x2 *>(lds + T::PLANE + lrow * OP + 4 * pc);
+                store_rm4<PREC>(a, cgr0 + lrow, cgc0 + 4 * pc, u, ul, edge);
             }
         }
         // One-tile-per-workgroup launches cover every tile with the grid itself.
@@ -723,7 +738,8 @@ void tcs_mfma_sweep(SweepArgs a) {
 // sweep: 0.50 ms unstaged at 8 waves, 0.86 ms staged with spills, 0.45 ms).
 template <int TH, int TW, int ZLO, int ZHI, int PREC, int MODE>
 __global__ __launch_bounds__(kWG)
-__attribute__((amdgpu_waves_per_eu((MODE & 2) ? 4 : (kHighOccupancy<TH, TW, PREC, false> ? 8 : 1), 8)))
+__attribute__((amdgpu_waves_per_eu((MODE & 2) ? (PREC == TCS_PREC_SPLIT ? 3 : 4)
+                                                : (kHighOccupancy<TH, TW, PREC, false> ? 8 : 1), 8)))
 void tcs_mfma_edge(SweepArgs a) {
     sweep_tile16<TH, TW, ZLO, ZHI, PREC, false, MODE>(a);
 }
@@ -1622,8 +1638,12 @@ struct EdgeEntry {
 #define TCS_EDGE_TILES(ZLO, ZHI, P, R)                                                                   \
     TCS_EDGE_MODES(2, 4, ZLO, ZHI, P, R), TCS_EDGE_MODES(4, 4, ZLO, ZHI, P, R), TCS_EDGE_MODES(2, 8, ZLO, ZHI, P, R), \
         TCS_EDGE_MODES(4, 8, ZLO, ZHI, P, R), TCS_EDGE_MODES(8, 8, ZLO, ZHI, P, R)
+#define TCS_EDGE_SPLIT_TILES(ZLO, ZHI, R)                                                                \
+    TCS_EDGE_MODES(2, 4, ZLO, ZHI, 2, R), TCS_EDGE_MODES(4, 4, ZLO, ZHI, 2, R), TCS_EDGE_MODES(2, 8, ZLO, ZHI, 2, R), \
+        TCS_EDGE_MODES(4, 8, ZLO, ZHI, 2, R)
 static const EdgeEntry kEdge[] = {
     TCS_EDGE_TILES(0, 4, 0, 2), TCS_EDGE_TILES(1, 3, 0, 1), TCS_EDGE_TILES(0, 4, 1, 2), TCS_EDGE_TILES(1, 3, 1, 1),
+    TCS_EDGE_SPLIT_TILES(0, 4, 2), TCS_EDGE_SPLIT_TILES(1, 3, 1),
 };
 
 static const EdgeEntry *select_edge(const tcs_problem *p, int mode) {

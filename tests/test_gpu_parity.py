@@ -707,13 +707,15 @@ def three_steps(T, torch, p, src, sweeps, scale_exp=0):
     return p.unpack(p.run(b0, b1, sweeps), scale_exp=scale_exp, ring=src)
 
 
-@pytest.mark.parametrize("precision", ["fp16", "bf16"])
+@pytest.mark.parametrize("precision", ["fp16", "bf16", "split"])
 def test_run_f32_fused_edges_bitwise(T, torch, precision):
     """tcs_run_f32 with the layout transforms fused into the first and last sweep
     (tcs_mfma_edge) equals pack -> run -> unpack bitwise: random inputs, every
     stencil and edge tile, 1-3 sweeps (and 4 with fused middle sweeps), widths
     that do and do not allow 16-byte row accesses, and a source 4 bytes off a
-    16-byte boundary."""
+    16-byte boundary.  SPLIT (no 8x8 tile) with a per-grid scale exponent."""
+    e = T.split_scale_exp(1.0) if precision == "split" else 0
+    tiles = ((2, 4), (4, 4), (2, 8), (4, 8)) + (() if precision == "split" else ((8, 8),))
     for stencil in STENCILS:
         B = T.gen_coeffs(stencil, 5)
         for rows, cols in ((70, 150), (97, 517), (64, 128)):
@@ -721,7 +723,7 @@ def test_run_f32_fused_edges_bitwise(T, torch, precision):
             for off in (0, 1):
                 src = base[off:off + rows * cols].view(rows, cols)
                 src.copy_(T.gen_uniform(rows, cols, seed=rows + cols, dtype=torch.float32))
-                for tile in ((2, 4), (4, 4), (2, 8), (4, 8), (8, 8)):
+                for tile in tiles:
                     p = T.Problem(stencil, rows, cols, B, precision=precision, tile=tile)
                     b0, b1 = p.alloc(), p.alloc()
                     p.clear(b0)
@@ -729,8 +731,8 @@ def test_run_f32_fused_edges_bitwise(T, torch, precision):
                     for sweeps, fuse in ((1, 1), (2, 1), (3, 1), (4, 2)):
                         say("case run_f32", stencil, precision, rows, cols, off, tile, sweeps, fuse)
                         p.fuse = fuse
-                        got = p.run_f32(src, sweeps, buf0=b0, buf1=b1)
-                        want = three_steps(T, torch, p, src, sweeps)
+                        got = p.run_f32(src, sweeps, buf0=b0, buf1=b1, scale_exp=e)
+                        want = three_steps(T, torch, p, src, sweeps, e)
                         assert torch.equal(got, want), (stencil, rows, cols, off, tile, sweeps, fuse)
 
 
@@ -784,15 +786,15 @@ def test_run_f32_row_strided_views(T, torch):
 
 
 def test_run_f32_other_problems_run_three_steps(T, torch):
-    """Problems without an edge kernel (SPLIT, direct, 32x32 fragments,
-    persistent grids) take the three-step path through tcs_run_f32; a fused
-    single sweep needs no storage buffers."""
+    """Problems without an edge kernel (direct, 32x32 fragments, persistent
+    grids) take the three-step path through tcs_run_f32; a fused single sweep
+    needs no storage buffers."""
     rows, cols = 70, 150
     B = T.gen_coeffs("box2d2r", 3)
     src = T.gen_uniform(rows, cols, seed=9, dtype=torch.float32)
     e = T.split_scale_exp(1.0)
-    for kw, scale in (({"precision": "split"}, e), ({"method": "direct"}, 0), ({"frag": 32}, 0),
-                      ({"wg_per_cu": 2}, 0)):
+    for kw, scale in (({"method": "direct"}, 0), ({"frag": 32}, 0), ({"wg_per_cu": 2}, 0),
+                      ({"precision": "split", "wg_per_cu": 2}, e)):
         say("case run_f32 three steps", kw)
         p = T.Problem("box2d2r", rows, cols, B, **kw)
         b0, b1 = p.alloc(), p.alloc()
