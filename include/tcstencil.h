@@ -189,7 +189,8 @@ int tcs_run(const tcs_problem *p, void *d_buf0, void *d_buf1, int sweeps, void *
  * result of tcs_pack_f32(d_src -> d_buf0, scale_exp), tcs_run(d_buf0, d_buf1,
  * sweeps) and tcs_unpack_f32(result -> d_dst, scale_exp, d_ring = d_src).  For
  * MFMA problems with 16x16 fragments and one tile per workgroup (FP16/BF16 on
- * any of the five tiles, SPLIT on all but 8x8), the pack is fused into the first sweep and the unpack
+ * any of the five tiles, SPLIT on all but 8x8; the two edge sweeps may run on a
+ * smaller tile that divides the layout), the pack is fused into the first sweep and the unpack
  * into the last, so one sweep is one kernel reading and writing FP32 and
  * touching no storage buffer (d_buf0/d_buf1 may then be NULL); otherwise the
  * three steps run.  Storage buffers must have a zero ghost ring (tcs_clear).

@@ -1637,7 +1637,7 @@ struct EdgeEntry {
     TCS_EDGE(TH, TW, ZLO, ZHI, P, R, 1), TCS_EDGE(TH, TW, ZLO, ZHI, P, R, 2), TCS_EDGE(TH, TW, ZLO, ZHI, P, R, 3)
 #define TCS_EDGE_TILES(ZLO, ZHI, P, R)                                                                   \
     TCS_EDGE_MODES(2, 4, ZLO, ZHI, P, R), TCS_EDGE_MODES(4, 4, ZLO, ZHI, P, R), TCS_EDGE_MODES(2, 8, ZLO, ZHI, P, R), \
-        TCS_EDGE_MODES(4, 8, ZLO, ZHI, P, R), TCS_EDGE_MODES(8, 8, ZLO, ZHI, P, R)
+        TCS_EDGE_MODES(4, 8, ZLO, ZHI, P, R)
 #define TCS_EDGE_SPLIT_TILES(ZLO, ZHI, R)                                                                \
     TCS_EDGE_MODES(2, 4, ZLO, ZHI, 2, R), TCS_EDGE_MODES(4, 4, ZLO, ZHI, 2, R), TCS_EDGE_MODES(2, 8, ZLO, ZHI, 2, R), \
         TCS_EDGE_MODES(4, 8, ZLO, ZHI, 2, R)
@@ -1655,17 +1655,19 @@ static const EdgeEntry *select_edge(const tcs_problem *p, int mode) {
     return nullptr;
 }
 
-// The problem the edge kernels run: the caller's, except that SPLIT tiles of
-// 32 or more bricks run as 4x4 tiles (they divide the layout, and brick
-// results do not depend on the tile): holding both planes' results for the
-// staged output costs registers, and 4x4 measured fastest (box2d2r 16384^2,
-// one sweep: 4x8 0.62 ms, 2x8 0.61, 4x4 0.57, 2x4 0.61).
+// The problem the edge kernels run: the caller's, except that larger tiles
+// run as the measured-fastest edge tile, which divides the layout (brick
+// results do not depend on the tile).  SPLIT: 4x4 for tiles of 32 or more
+// bricks -- holding both planes' results for the staged output costs
+// registers (box2d2r 16384^2, one sweep: 4x8 0.62 ms, 2x8 0.61, 4x4 0.57, 2x4
+// 0.61).  FP16/BF16: 4x8 for 8x8 tiles (0.40 against 0.44 ms).
 static tcs_problem edge_problem(const tcs_problem *p) {
     tcs_problem q = *p;
-    if (p->precision == TCS_PREC_SPLIT && p->tile_h * p->tile_w >= 32 && p->layout.nbr % 4 == 0 &&
-        p->layout.nbc % 4 == 0) {
+    const bool split = p->precision == TCS_PREC_SPLIT;
+    const int tw = split ? 4 : 8;
+    if (p->tile_h * p->tile_w >= (split ? 32 : 64) && p->layout.nbr % 4 == 0 && p->layout.nbc % tw == 0) {
         q.tile_h = 4;
-        q.tile_w = 4;
+        q.tile_w = tw;
     }
     return q;
 }
