@@ -761,6 +761,40 @@ def test_run_f32_tiny_grids(T, torch, oracle):
             assert np.abs(got - want).max() <= 2.0 ** -9, (stencil, rows, cols)
 
 
+@pytest.mark.parametrize("precision", ["fp16", "split"])
+def test_run_f32_captured_in_a_graph(T, torch, precision):
+    """tcs_run_f32 launches only kernels on the caller's stream, so a caller can
+    capture it (here in a torch CUDA graph): every replay, with eager work in
+    between, equals the eager result."""
+    rows, cols = 200, 300
+    B = T.gen_coeffs("box2d2r", 2)
+    x = T.gen_uniform(rows, cols, seed=12, dtype=torch.float32)
+    e = T.split_scale_exp(1.0) if precision == "split" else 0
+    p = T.Problem("box2d2r", rows, cols, B, precision=precision)
+    b0, b1 = p.alloc(), p.alloc()
+    p.clear(b0)
+    p.clear(b1)
+    eager = p.run_f32(x, 3, buf0=b0, buf1=b1, scale_exp=e).clone()
+    out = torch.empty_like(x)
+    side = torch.cuda.Stream()
+    side.wait_stream(torch.cuda.current_stream())
+    with torch.cuda.stream(side):
+        p.run_f32(x, 3, out=out, buf0=b0, buf1=b1, scale_exp=e)
+    torch.cuda.current_stream().wait_stream(side)
+    torch.cuda.synchronize()
+    g = torch.cuda.CUDAGraph()
+    with torch.cuda.graph(g):
+        p.run_f32(x, 3, out=out, buf0=b0, buf1=b1, scale_exp=e)
+    for rep in range(4):
+        out.fill_(-1.0)
+        p.run_f32(x, 1, buf0=b0, buf1=b1, scale_exp=e)  # eager work between replays
+        torch.cuda.synchronize()
+        g.replay()
+        torch.cuda.synchronize()
+        assert torch.equal(out, eager), rep
+    del g
+
+
 def test_run_f32_row_strided_views(T, torch):
     """A grid that is a column slice of a wider array (leading dimension 160
     for 150 columns) gives the result of the contiguous grid, fused and in
