@@ -2,7 +2,7 @@
  * CPU oracle for the 2D stencil path -- TEST INFRASTRUCTURE ONLY.
  *
  * Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg may load
- * this library, and only as the checker (or the timed CPU baseline); the product
+ * this library, and only to check results (or as the timed CPU baseline); the product
  * library (tcstencil_amd/) never links, loads or calls it.
  *
  * Every function restates the reference's CPU gold loop with the same loop order

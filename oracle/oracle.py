@@ -1,7 +1,7 @@
 """ctypes front end of the CPU oracle -- TEST INFRASTRUCTURE ONLY.
 
 Only tests/, ``__graft_entry__.smoke()`` and bench.py's cpu_baseline leg import
-this module, and only as the checker (or the timed CPU baseline).  The product
+this module, and only to check results (or as the timed CPU baseline).  The product
 package ``tcstencil_amd`` never imports it.
 
 The arithmetic lives in ``oracle/gold.c`` (a restatement of the reference's
