@@ -138,8 +138,8 @@ int tcs_problem_init(tcs_problem *p, int stencil, int method, int precision, int
                      const float *B, int n);
 int tcs_radius(int stencil);
 /* Choose the workgroup tile (bricks, rows x cols): 2x4, 4x4, 2x8, 4x8 or 8x8.
- * Single-sweep kernels exist for all but SPLIT 8x8, fused kernels for all five
- * (16x16 fragments); a missing combination fails at launch with TCS_ERR_UNSUPPORTED
+ * Single-sweep and fused kernels exist for all five (16x16 fragments); a
+ * missing combination (32x32 fragments: see tcs_problem_set_frag) fails at launch with TCS_ERR_UNSUPPORTED
  * (tcs_kernel_name / tcs_fused_kernel_name return NULL for it).  Re-derives the
  * layout's alignment, so buffers packed for another tile must be packed again. */
 int tcs_problem_set_tile(tcs_problem *p, int tile_h, int tile_w);
@@ -188,9 +188,9 @@ int tcs_run(const tcs_problem *p, void *d_buf0, void *d_buf1, int sweeps, void *
  * (d_dst; both rows x cols with leading dimension ld, distinct): bitwise the
  * result of tcs_pack_f32(d_src -> d_buf0, scale_exp), tcs_run(d_buf0, d_buf1,
  * sweeps) and tcs_unpack_f32(result -> d_dst, scale_exp, d_ring = d_src).  For
- * MFMA problems with 16x16 fragments and one tile per workgroup (FP16/BF16 on
- * any of the five tiles, SPLIT on all but 8x8; the two edge sweeps may run on a
- * smaller tile that divides the layout), the pack is fused into the first sweep and the unpack
+ * MFMA problems with 16x16 fragments and one tile per workgroup (any of the
+ * five tiles; the two edge sweeps may run on a smaller tile that divides the
+ * layout), the pack is fused into the first sweep and the unpack
  * into the last, so one sweep is one kernel reading and writing FP32 and
  * touching no storage buffer (d_buf0/d_buf1 may then be NULL); otherwise the
  * three steps run.  Storage buffers must have a zero ghost ring (tcs_clear).

@@ -33,8 +33,6 @@ def tune(stencil, rows, cols, method="mfma", precision="fp16", sweeps=20, warmup
     e = T.split_scale_exp(1.0) if precision == "split" else 0
     res = []
     for tile, wg in itertools.product(tiles or TILES, wgs):
-        if fuse == 1 and precision == "split" and tile == (8, 8):
-            continue  # no unfused SPLIT 8x8 tile
         if fuse > 1 and wg:
             continue  # fused launches: one tile per workgroup
         p = T.Problem(stencil, rows, cols, B, method=method, precision=precision, tile=tile, wg_per_cu=wg,

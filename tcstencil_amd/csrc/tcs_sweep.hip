@@ -1432,7 +1432,7 @@ struct KernelEntry {
     TCS_MFMA_TILES(0, 4, 0, 2, PS), TCS_MFMA(8, 8, 0, 4, 0, 2, PS), TCS_MFMA_TILES(1, 3, 0, 1, PS), \
     TCS_MFMA(8, 8, 1, 3, 0, 1, PS), TCS_MFMA_TILES(0, 4, 1, 2, PS), TCS_MFMA(8, 8, 0, 4, 1, 2, PS), \
     TCS_MFMA_TILES(1, 3, 1, 1, PS), TCS_MFMA(8, 8, 1, 3, 1, 1, PS), TCS_MFMA_TILES(0, 4, 2, 2, PS), \
-    TCS_MFMA_TILES(1, 3, 2, 1, PS)
+    TCS_MFMA(8, 8, 0, 4, 2, 2, PS), TCS_MFMA_TILES(1, 3, 2, 1, PS), TCS_MFMA(8, 8, 1, 3, 2, 1, PS)
 #define TCS_DIRECT(TH, TW, S, PS) \
     {TCS_METHOD_DIRECT, TCS_PREC_FP16, -1, S, TH, TW, PS, 0, tcs_direct_sweep<TH, TW, S, PS>, \
      "tcs_direct_sweep<" #TH ", " #TW ", " #S ", " #PS ">"}
@@ -1448,8 +1448,8 @@ struct KernelEntry {
     TCS_MFMA32(8, 8, ZLO, ZHI, P, R)
 
 // Tile shapes offered (bricks, TH x TW), each one-tile-per-workgroup (false)
-// and persistent (true).  SPLIT has no 8x8 tile: two planes of 132 x 136
-// elements would need 71.8 KB of LDS.  32x32 fragments: FP16 and BF16, tiles
+// and persistent (true); SPLIT 8x8 (two planes, 72 KB of LDS) runs two
+// workgroups per CU.  32x32 fragments: FP16 and BF16, tiles
 // whose 2x2-brick blocks split evenly over the 4 waves, one tile per workgroup.
 static const KernelEntry kKernels[] = {
     TCS_MFMA_ALL(false), TCS_MFMA_ALL(true),

@@ -107,7 +107,7 @@ def test_invalid_arguments_return_status():
         T.Problem("box2d1r", 10, 10, np.ones((3, 3), np.float32), method="direct", precision="bf16")
     p = T.Problem("box2d1r", 64, 64, np.ones((3, 3), np.float32), precision="split")
     p.set_tile(8, 8)
-    assert p.kernel_name == "" and p.fused_kernel_name.startswith("tcs_mfma_fused<8, 8")
+    assert p.kernel_name.startswith("tcs_mfma_sweep<8, 8, 1, 3, 2") and p.fused_kernel_name.startswith("tcs_mfma_fused<8, 8")
     p.set_tile(2, 8)
     assert p.fused_kernel_name.startswith("tcs_mfma_fused<2, 8") and p.kernel_name.startswith("tcs_mfma_sweep<2, 8")
     with pytest.raises(T.TcsError, match="not offered"):

@@ -443,9 +443,7 @@ def test_fused_bitwise_equals_unfused(T, torch, stencil, precision):
             sweeps = 2 * fuse + 1  # several launches, uneven split, odd parity
             say("case fused", stencil, precision, tile, fuse, sweeps)
             if sweeps not in ref:
-                # brick results do not depend on the tile; SPLIT has no unfused 8x8
-                pu = T.Problem(stencil, 300, 517, B, precision=precision,
-                               tile=None if precision == "split" and tile == (8, 8) else tile)
+                pu = T.Problem(stencil, 300, 517, B, precision=precision, tile=tile)
                 ref[sweeps] = run_buffers(T, torch, pu, x, sweeps, e)
                 ref[sweeps] = pu.unpack(ref[sweeps], scale_exp=e)
             pf = T.Problem(stencil, 300, 517, B, precision=precision, tile=tile, fuse=fuse)
@@ -713,9 +711,9 @@ def test_run_f32_fused_edges_bitwise(T, torch, precision):
     (tcs_mfma_edge) equals pack -> run -> unpack bitwise: random inputs, every
     stencil and edge tile, 1-3 sweeps (and 4 with fused middle sweeps), widths
     that do and do not allow 16-byte row accesses, and a source 4 bytes off a
-    16-byte boundary.  SPLIT (no 8x8 tile) with a per-grid scale exponent."""
+    16-byte boundary.  SPLIT with a per-grid scale exponent."""
     e = T.split_scale_exp(1.0) if precision == "split" else 0
-    tiles = ((2, 4), (4, 4), (2, 8), (4, 8)) + (() if precision == "split" else ((8, 8),))
+    tiles = ((2, 4), (4, 4), (2, 8), (4, 8), (8, 8))
     for stencil in STENCILS:
         B = T.gen_coeffs(stencil, 5)
         for rows, cols in ((70, 150), (97, 517), (64, 128)):
