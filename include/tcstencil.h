@@ -138,10 +138,11 @@ int tcs_problem_init(tcs_problem *p, int stencil, int method, int precision, int
                      const float *B, int n);
 int tcs_radius(int stencil);
 /* Choose the workgroup tile (bricks, rows x cols): 2x4, 4x4, 2x8, 4x8 or 8x8.
- * Single-sweep and fused kernels exist for all five (16x16 fragments); a
- * missing combination (32x32 fragments: see tcs_problem_set_frag) fails at launch with TCS_ERR_UNSUPPORTED
- * (tcs_kernel_name / tcs_fused_kernel_name return NULL for it).  Re-derives the
- * layout's alignment, so buffers packed for another tile must be packed again. */
+ * With 16x16 fragments single-sweep and fused kernels exist for all five; a
+ * missing combination (32x32 fragments: see tcs_problem_set_frag) fails at
+ * launch with TCS_ERR_UNSUPPORTED (tcs_kernel_name / tcs_fused_kernel_name
+ * return NULL for it).  Re-derives the layout's alignment, so buffers packed
+ * for another tile must be packed again. */
 int tcs_problem_set_tile(tcs_problem *p, int tile_h, int tile_w);
 /* Choose the MFMA fragment (16 or 32, see tcs_problem.frag) and re-derive the
  * default tile for it (32: 4x4, 2x8, 4x8 or 8x8 single-sweep tiles; 4x4, 4x8, 8x8
