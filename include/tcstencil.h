@@ -189,8 +189,8 @@ int tcs_run(const tcs_problem *p, void *d_buf0, void *d_buf1, int sweeps, void *
  * (d_dst; both rows x cols with leading dimension ld, distinct): bitwise the
  * result of tcs_pack_f32(d_src -> d_buf0, scale_exp), tcs_run(d_buf0, d_buf1,
  * sweeps) and tcs_unpack_f32(result -> d_dst, scale_exp, d_ring = d_src).  For
- * MFMA problems with 16x16 fragments and one tile per workgroup (any of the
- * five tiles; the two edge sweeps may run on a smaller tile that divides the
+ * MFMA problems with 16x16 fragments (any of the five tiles; the two edge
+ * sweeps run one tile per workgroup, maybe on a smaller tile that divides the
  * layout), the pack is fused into the first sweep and the unpack
  * into the last, so one sweep is one kernel reading and writing FP32 and
  * touching no storage buffer (d_buf0/d_buf1 may then be NULL); otherwise the

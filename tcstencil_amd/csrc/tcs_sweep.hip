@@ -1655,14 +1655,15 @@ static const EdgeEntry *select_edge(const tcs_problem *p, int mode) {
     return nullptr;
 }
 
-// The problem the edge kernels run: the caller's, except that larger tiles
-// run as the measured-fastest edge tile, which divides the layout (brick
-// results do not depend on the tile).  SPLIT: 4x4 for tiles of 32 or more
+// The problem the edge kernels run: the caller's on one tile per workgroup,
+// except that larger tiles run as the measured-fastest edge tile, which
+// divides the layout (brick results depend on neither).  SPLIT: 4x4 for tiles of 32 or more
 // bricks -- holding both planes' results for the staged output costs
 // registers (box2d2r 16384^2, one sweep: 4x8 0.62 ms, 2x8 0.61, 4x4 0.57, 2x4
 // 0.61).  FP16/BF16: 4x8 for 8x8 tiles (0.40 against 0.44 ms).
 static tcs_problem edge_problem(const tcs_problem *p) {
     tcs_problem q = *p;
+    q.wg_per_cu = 0;  // one tile per workgroup: persistence does not change results
     const bool split = p->precision == TCS_PREC_SPLIT;
     const int tw = split ? 4 : 8;
     if (p->tile_h * p->tile_w >= (split ? 32 : 64) && p->layout.nbr % 4 == 0 && p->layout.nbc % tw == 0) {

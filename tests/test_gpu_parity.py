@@ -818,9 +818,10 @@ def test_run_f32_row_strided_views(T, torch):
 
 
 def test_run_f32_other_problems_run_three_steps(T, torch):
-    """Problems without an edge kernel (direct, 32x32 fragments, persistent
-    grids) take the three-step path through tcs_run_f32; a fused single sweep
-    needs no storage buffers."""
+    """Problems without an edge kernel (direct, 32x32 fragments) take the
+    three-step path through tcs_run_f32; persistent-grid problems run their edge
+    sweeps one tile per workgroup; a fused single sweep needs no storage
+    buffers."""
     rows, cols = 70, 150
     B = T.gen_coeffs("box2d2r", 3)
     src = T.gen_uniform(rows, cols, seed=9, dtype=torch.float32)
