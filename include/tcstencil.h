@@ -44,9 +44,10 @@ enum {
 };
 
 /* Problem parameters of the reference (common/param.hpp:1-13), same names and
- * defaults; tcs_params_from_env() overrides each from TCS_<NAME>.  TILE_SIZE
- * (bricks per block along x in the reference) maps to tile_w; 0 means "use the
- * tuned default". */
+ * defaults; tcs_params_from_env() overrides each from TCS_<NAME>.  TILE_SIZE is
+ * the reference's bricks per block along x (1..15 in its sweep); the host
+ * drivers apply it with tcs_problem_set_tile_size; 0 means "use the tuned
+ * default". */
 typedef struct tcs_params {
     double TOLERANCE;  /* 0.01 */
     double MIN_NUMBER; /* 1e-6 */
@@ -144,6 +145,14 @@ int tcs_radius(int stencil);
  * return NULL for it).  Re-derives the layout's alignment, so buffers packed
  * for another tile must be packed again. */
 int tcs_problem_set_tile(tcs_problem *p, int tile_h, int tile_w);
+/* Apply the reference's TILE_SIZE (common/param.hpp:11-13: bricks one thread
+ * block walks along x on blocks one brick tall, layout16/host_box2d2r.cu:166-168;
+ * swept over 1..15 by script/param.py:5).  Every value maps to an offered tile:
+ * 1..5 -> 4 bricks wide, 6 and above -> 8 bricks wide, at the current tile
+ * height where that shape is offered (else the nearest offered height).
+ * tile_size <= 0 keeps the tuned default.  Brick results do not depend on the
+ * tile, so any TILE_SIZE gives the same output bits. */
+int tcs_problem_set_tile_size(tcs_problem *p, int tile_size);
 /* Choose the MFMA fragment (16 or 32, see tcs_problem.frag) and re-derive the
  * default tile for it (32: 4x4, 2x8, 4x8 or 8x8 single-sweep tiles; 4x4, 4x8, 8x8
  * fused).  32 needs method MFMA and FP16 or BF16. */
@@ -194,7 +203,9 @@ int tcs_run(const tcs_problem *p, void *d_buf0, void *d_buf1, int sweeps, void *
  * layout), the pack is fused into the first sweep and the unpack
  * into the last, so one sweep is one kernel reading and writing FP32 and
  * touching no storage buffer (d_buf0/d_buf1 may then be NULL); otherwise the
- * three steps run.  Storage buffers must have a zero ghost ring (tcs_clear).
+ * three steps run.  The storage buffers' prior contents do not matter: their
+ * ghost rings are cleared here (a fill kernel, safe under graph capture) and
+ * every data brick is written before it is read.
  * This is the GPU-resident edge of the reference's host driver: its host-side
  * conversion, padding and brick layout (layout16/host_box2d2r.cu:131-141) and
  * the reverse (:196-204), run on the device and fused into the sweeps. */

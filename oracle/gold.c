@@ -126,13 +126,16 @@ void oracle_error_norms(const float *o, const double *ref, int rows, int cols, i
     out[3] = sqrt(sr);
 }
 
-/* common/common.hpp:38-64: RMS of |output-reference| over [y_lb,y_ub)x[x_lb,x_ub). */
+/* common/common.hpp:38-64: RMS of |output-reference| over [y_lb,y_ub)x[x_lb,x_ub).
+ * As in the reference (:47), the difference of the two floats is taken in
+ * single precision and then widened; only the sums run in double. */
 double oracle_check_error_2d(int N, const float *output, const float *reference,
                              int y_lb, int y_ub, int x_lb, int x_ub) {
     double error = 0.0;
     for (int j = y_lb; j < y_ub; j++)
         for (int k = x_lb; k < x_ub; k++) {
-            double e = (double)output[(ptrdiff_t)j * N + k] - (double)reference[(ptrdiff_t)j * N + k];
+            const float d = output[(ptrdiff_t)j * N + k] - reference[(ptrdiff_t)j * N + k];
+            double e = (double)d;
             e = e < 0.0 ? -e : e;
             error += e * e;
         }
@@ -151,7 +154,8 @@ double oracle_check_relative_error_2d(int N, const float *output, const float *r
     for (int j = y_lb; j < y_ub; j++)
         for (int k = x_lb; k < x_ub; k++) {
             const float rv = reference[(ptrdiff_t)j * N + k];
-            double e = (double)output[(ptrdiff_t)j * N + k] - (double)rv;
+            const float d = output[(ptrdiff_t)j * N + k] - rv; /* single precision, :75 */
+            double e = (double)d;
             e = (fabs((double)rv) < min_number) ? e / min_number : e / (double)rv;
             error += e * e;
         }
@@ -171,7 +175,8 @@ double oracle_check_relative_error_3d(int N, const float *output, const float *r
             for (int x = x_lb; x < x_ub; x++) {
                 const ptrdiff_t i = ((ptrdiff_t)z * N + y) * N + x;
                 const float rv = reference[i];
-                double e = (double)output[i] - (double)rv;
+                const float d = output[i] - rv; /* single precision, :105 */
+                double e = (double)d;
                 e = (fabs((double)rv) < min_number) ? e / min_number : e / (double)rv;
                 error += e * e;
             }

@@ -79,6 +79,7 @@ SIGNATURES = {
     "tcs_problem_init": (_i, [_prob, _i, _i, _i, _i, _i, _fp, _i]),
     "tcs_radius": (_i, [_i]),
     "tcs_problem_set_tile": (_i, [_prob, _i, _i]),
+    "tcs_problem_set_tile_size": (_i, [_prob, _i]),
     "tcs_problem_set_frag": (_i, [_prob, _i]),
     "tcs_buffer_bytes": (_i64, [_prob]),
     "tcs_pack_f32": (_i, [_prob, _vp, _i64, _vp, _i, _vp]),

@@ -82,6 +82,24 @@ class Problem:
     def set_tile(self, tile_h: int, tile_w: int) -> None:
         check(lib().tcs_problem_set_tile(ctypes.byref(self.s), tile_h, tile_w))
 
+    def set_tile_size(self, tile_size: int) -> None:
+        """Apply the reference's TILE_SIZE (bricks per block along x, 1..15)."""
+        check(lib().tcs_problem_set_tile_size(ctypes.byref(self.s), int(tile_size)))
+
+    def check_buf(self, buf, what: str = "storage buffer") -> None:
+        """A storage buffer of this problem must hold the whole layout (which
+        set_tile and the fragment choice re-derive): the kernels address all of it."""
+        if buf is None:
+            raise ValueError(f"{what}: missing")
+        nbytes = buf.numel() * buf.element_size()
+        if nbytes < self.buffer_bytes:
+            raise ValueError(f"{what}: {nbytes} bytes, this problem's layout needs {self.buffer_bytes}")
+
+    def buf_ptr(self, buf, what: str = "storage buffer") -> ctypes.c_void_p:
+        """Device pointer of a checked storage buffer of this problem."""
+        self.check_buf(buf, what)
+        return _ptr(buf)
+
     @property
     def ref(self):
         return ctypes.byref(self.s)
@@ -155,22 +173,22 @@ class Problem:
         return torch.empty(self.buffer_bytes, dtype=torch.uint8, device=device)
 
     def clear(self, buf, stream=None) -> None:
-        check(lib().tcs_clear(self.ref, _ptr(buf), _stream(stream)))
+        check(lib().tcs_clear(self.ref, self.buf_ptr(buf), _stream(stream)))
 
     def pack(self, src, buf, scale_exp: int = 0, stream=None) -> None:
         """Row-major FP32/FP64 device grid -> brick storage."""
         torch = _torch()
         sp = _grid_ptr(src, self.rows, self.cols, "pack source")
-        if buf.numel() < self.buffer_bytes:
-            raise ValueError("storage buffer too small")
+        bp = self.buf_ptr(buf, "pack destination")
         fn = {torch.float32: lib().tcs_pack_f32, torch.float64: lib().tcs_pack_f64}.get(src.dtype)
         if fn is None:
             raise TypeError("pack: source must be float32 or float64")
-        check(fn(self.ref, sp, src.stride(0), _ptr(buf), scale_exp, _stream(stream)))
+        check(fn(self.ref, sp, src.stride(0), bp, scale_exp, _stream(stream)))
 
     def unpack(self, buf, out=None, scale_exp: int = 0, ring=None, stream=None):
         """Brick storage -> row-major FP32 device grid (ring copied from `ring`)."""
         torch = _torch()
+        bp = self.buf_ptr(buf, "unpack source")
         if out is None:
             out = torch.empty((self.rows, self.cols), dtype=torch.float32, device=buf.device)
         if out.dtype != torch.float32:
@@ -181,15 +199,19 @@ class Problem:
             ringp = _grid_ptr(ring, self.rows, self.cols, "unpack ring")
             if ring.stride(0) != out.stride(0) or ring.dtype != torch.float32:
                 raise ValueError("unpack: ring must be float32 with the output's row stride")
-        check(lib().tcs_unpack_f32(self.ref, _ptr(buf), op, out.stride(0), scale_exp, ringp, _stream(stream)))
+        check(lib().tcs_unpack_f32(self.ref, bp, op, out.stride(0), scale_exp, ringp, _stream(stream)))
         return out
 
     # -- sweeps -------------------------------------------------------------
     def sweep(self, src, dst, stream=None) -> None:
+        self.check_buf(src, "sweep input")
+        self.check_buf(dst, "sweep output")
         check(lib().tcs_sweep(self.ref, _ptr(src), _ptr(dst), _stream(stream)))
 
     def run(self, buf0, buf1, sweeps: int, stream=None):
         """Ping-pong sweeps; returns the buffer holding the result."""
+        self.check_buf(buf0, "buf0")
+        self.check_buf(buf1, "buf1")
         check(lib().tcs_run(self.ref, _ptr(buf0), _ptr(buf1), sweeps, _stream(stream)))
         return buf1 if sweeps % 2 else buf0
 
@@ -205,6 +227,9 @@ class Problem:
             out = torch.empty_like(src)
         if out.stride(0) != src.stride(0) or out.stride(1) != 1 or src.stride(1) != 1:
             raise ValueError("run_f32: source and output need the same row stride and unit column stride")
+        for b, name in ((buf0, "buf0"), (buf1, "buf1")):
+            if b is not None:
+                self.check_buf(b, name)
         b0 = _ptr(buf0) if buf0 is not None else None
         b1 = _ptr(buf1) if buf1 is not None else None
         sp = _grid_ptr(src, self.rows, self.cols, "run_f32 source")
@@ -215,8 +240,9 @@ class Problem:
 
     def graph(self, buf0, buf1, sweeps: int, stream=None) -> "Graph":
         h = ctypes.c_void_p()
-        check(lib().tcs_graph_create(self.ref, _ptr(buf0), _ptr(buf1), sweeps, _stream(stream),
-                                     ctypes.byref(h)))
+        self.check_buf(buf0, "buf0")
+        self.check_buf(buf1, "buf1")
+        check(lib().tcs_graph_create(self.ref, _ptr(buf0), _ptr(buf1), sweeps, _stream(stream), ctypes.byref(h)))
         return Graph(h)
 
 
@@ -250,6 +276,9 @@ class Batch:
         if not (n == len(bufs0) == len(bufs1)) or n == 0:
             raise ValueError("batch: need one problem and two buffers per grid")
         arr = (L.ProblemStruct * n)(*[p.s for p in problems])
+        for i, (p, b0, b1) in enumerate(zip(problems, bufs0, bufs1)):
+            p.check_buf(b0, f"grid {i} buf0")
+            p.check_buf(b1, f"grid {i} buf1")
         p0 = (ctypes.c_void_p * n)(*[_ptr(b).value for b in bufs0])
         p1 = (ctypes.c_void_p * n)(*[_ptr(b).value for b in bufs1])
         self.h = ctypes.c_void_p()
@@ -285,8 +314,19 @@ class Batch:
 
 
 # ---------------------------------------------------------------- utilities
+def _same_grid(a, b, dtype_a, dtype_b, what: str) -> None:
+    """Both tensors are contiguous 2-D grids of one shape with the given dtypes
+    (the C functions address rows x cols elements of each)."""
+    if a.dim() != 2 or tuple(a.shape) != tuple(b.shape):
+        raise ValueError(f"{what}: grids of shapes {tuple(a.shape)} and {tuple(b.shape)}")
+    if a.dtype != dtype_a or b.dtype != dtype_b:
+        raise TypeError(f"{what}: expected {dtype_a} and {dtype_b}, got {a.dtype} and {b.dtype}")
+
+
 def oracle_sweep_f64(stencil, src, B, dst, stream=None) -> None:
     """One FP64 sweep (gold's order, no FMA) on row-major device grids."""
+    torch = _torch()
+    _same_grid(src, dst, torch.float64, torch.float64, "oracle_sweep_f64")
     B = np.ascontiguousarray(B, dtype=np.float64)
     Bp = B.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
     check(lib().tcs_oracle_sweep_f64(stencil_id(stencil), _ptr(src), Bp, B.shape[0], _ptr(dst),
@@ -304,6 +344,7 @@ def oracle_sweeps_f64(stencil, x64, B, sweeps: int, stream=None):
 
 
 def error_norms(out32, ref64, r: int, stream=None) -> dict:
+    _same_grid(out32, ref64, _torch().float32, _torch().float64, "error_norms")
     res = np.zeros(4, dtype=np.float64)
     check(lib().tcs_error_norms(_ptr(out32), _ptr(ref64), out32.shape[0], out32.shape[1], r,
                                 res.ctypes.data_as(ctypes.POINTER(ctypes.c_double)), _stream(stream)))

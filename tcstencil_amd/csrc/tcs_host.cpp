@@ -179,7 +179,9 @@ static bool frag32(const tcs_problem *p) { return p->frag == 32; }
 // average; a 1024-tile threshold was 4.9% off.
 static void default_tile(const tcs_problem *p, int *th, int *tw) {
     static const int kPref[5][2] = {{8, 8}, {4, 8}, {4, 4}, {2, 8}, {2, 4}};
-    const int last = frag32(p) ? 3 : 4;  // 32x32 fragments need 2x2-brick blocks, 4 per tile
+    // 32x32 fragments need 2x2-brick blocks, 4 per tile; 4x4 is the smallest
+    // tile with single-sweep, fused and batched 32x32 kernels.
+    const int last = frag32(p) ? 2 : 4;
     int first;
     if (p->method == TCS_METHOD_MFMA)
         first = frag32(p) ? 0 : 1;  // 8x8 (32x32 fragments), else 4x8
@@ -208,6 +210,25 @@ extern "C" int tcs_problem_set_tile(tcs_problem *p, int tile_h, int tile_w) {
     p->tile_h = tile_h;
     p->tile_w = tile_w;
     return TCS_OK;
+}
+
+// The reference's TILE_SIZE (common/param.hpp:11-13) is the number of bricks a
+// thread block walks along x, on blocks one brick tall (layout16/host_box2d2r.cu:
+// 166-168); its sweep covers 1..15 (script/param.py:5).  Here it selects the
+// offered tile whose width is nearest: 1..5 -> 4 bricks wide, 6 and above -> 8,
+// keeping the current tile height where that shape is offered (else the
+// nearest offered height).  Brick results do not depend on the tile.
+extern "C" int tcs_problem_set_tile_size(tcs_problem *p, int tile_size) {
+    if (!p) return set_error(TCS_ERR_INVALID, "set_tile_size: null problem");
+    if (tile_size <= 0) return TCS_OK;  // the tuned default stays
+    const int tw = tile_size < 6 ? 4 : 8;
+    static const int kHeights[3] = {2, 4, 8};
+    int th = p->tile_h > 0 ? p->tile_h : 4, best = -1;
+    for (int h : kHeights) {
+        const bool ok = tile_offered(h, tw) && (!frag32(p) || (h % 2 == 0 && (h / 2) * (tw / 2) % 4 == 0));
+        if (ok && (best < 0 || abs(h - th) < abs(best - th))) best = h;
+    }
+    return tcs_problem_set_tile(p, best, tw);
 }
 
 extern "C" int tcs_problem_init(tcs_problem *p, int stencil, int method, int precision, int rows, int cols,
@@ -333,10 +354,16 @@ extern "C" int tcs_run_f32(const tcs_problem *p, const float *d_src, float *d_ds
     if (ld < p->layout.cols)
         return set_error(TCS_ERR_INVALID, "run_f32: ld %lld < cols %d", (long long)ld, p->layout.cols);
     const hipStream_t s = (hipStream_t)stream;
+    // The storage buffers' prior contents do not matter: every data brick is
+    // written before it is read, and the ghost rings (read as the frame of the
+    // edge tiles, where the MFMA operands multiply them, if only by zero
+    // coefficients) are cleared here.
     if (sweeps >= 1 && edge_supported(p)) {
         if (sweeps == 1) return launch_edge(p, 3, nullptr, nullptr, d_src, d_dst, ld, scale_exp, s);
         if (!d_buf0 || !d_buf1 || d_buf0 == d_buf1)
             return set_error(TCS_ERR_INVALID, "run_f32: %d sweeps need two distinct storage buffers", sweeps);
+        TCS_TRY(clear_ring(p, d_buf0, s));
+        if (sweeps > 2) TCS_TRY(clear_ring(p, d_buf1, s));
         TCS_TRY(launch_edge(p, 1, nullptr, d_buf0, d_src, nullptr, ld, scale_exp, s));
         // Sweeps 2 .. sweeps-1 ping-pong from d_buf0; the last reads the result.
         TCS_TRY(tcs_run(p, d_buf0, d_buf1, sweeps - 2, stream));
@@ -346,6 +373,7 @@ extern "C" int tcs_run_f32(const tcs_problem *p, const float *d_src, float *d_ds
     if (!d_buf0 || !d_buf1 || d_buf0 == d_buf1)
         return set_error(TCS_ERR_INVALID, "run_f32: this problem needs two distinct storage buffers");
     TCS_TRY(tcs_pack_f32(p, d_src, ld, d_buf0, scale_exp, stream));
+    TCS_TRY(clear_ring(p, d_buf1, s));
     TCS_TRY(tcs_run(p, d_buf0, d_buf1, sweeps, stream));
     return tcs_unpack_f32(p, sweeps % 2 ? d_buf1 : d_buf0, d_dst, ld, scale_exp, d_src, stream);
 }
@@ -515,7 +543,7 @@ static int host_sweep_impl(int stencil, int method, int precision, const float *
         tcs_params_from_env(&prm);
     tcs_problem p;
     TCS_TRY(tcs_problem_init(&p, stencil, method, precision, rows, cols, h_B, n));
-    if (prm.TILE_SIZE > 0) TCS_TRY(tcs_problem_set_tile(&p, p.tile_h, prm.TILE_SIZE));
+    TCS_TRY(tcs_problem_set_tile_size(&p, prm.TILE_SIZE));
 
     DeviceScope ds;
     TCS_HIP_CHECK(hipStreamCreateWithFlags(&ds.stream, hipStreamNonBlocking));

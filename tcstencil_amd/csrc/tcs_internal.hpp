@@ -94,6 +94,8 @@ bool edge_supported(const tcs_problem *p);
 int launch_edge(const tcs_problem *p, int mode, const void *d_in, void *d_out, const float *src, float *dst,
                 int64_t ld, int scale_exp, hipStream_t stream);
 int launch_fused(const tcs_problem *p, const void *d_in, void *d_out, int nsweeps, hipStream_t stream);
+// Zero the ghost ring of a storage buffer of this problem (tcs_transform.hip).
+int clear_ring(const tcs_problem *p, void *d_buf, hipStream_t stream);
 const char *sweep_kernel_name(const tcs_problem *p);
 const char *fused_kernel_name(const tcs_problem *p);
 int mfma_per_brick(const tcs_problem *p);

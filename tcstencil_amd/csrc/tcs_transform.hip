@@ -158,6 +158,29 @@ __global__ void tcs_fill_kernel(u32x4 *__restrict__ p, int64_t n16) {
         p[t] = u32x4{0u, 0u, 0u, 0u};
 }
 
+// Zero the one-brick ghost ring of a storage buffer (every plane): ring brick
+// b < 2*sbc is in stored brick row 0 or sbr-1, the rest in stored brick column
+// 0 or sbc-1 of rows 1..sbr-2.  One thread per 16-byte chunk (32 per brick).
+__global__ void tcs_clear_ring_kernel(uint16_t *__restrict__ buf, int64_t plane, int nplanes, int sbr, int sbc) {
+    const int64_t nring = 2 * (int64_t)sbc + 2 * (int64_t)(sbr - 2);
+    const int64_t total = nring * 32;
+    for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < total;
+         t += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t b = t >> 5;
+        int64_t bi, bj;
+        if (b < 2 * (int64_t)sbc) {
+            bi = b < sbc ? 0 : sbr - 1;
+            bj = b < sbc ? b : b - sbc;
+        } else {
+            const int64_t q = b - 2 * (int64_t)sbc;
+            bi = 1 + (q < sbr - 2 ? q : q - (sbr - 2));
+            bj = q < sbr - 2 ? 0 : sbc - 1;
+        }
+        const int64_t e = (bi * sbc + bj) * 256 + 8 * (t & 31);
+        for (int p = 0; p < nplanes; ++p) *reinterpret_cast<u32x4 *>(buf + p * plane + e) = u32x4{0u, 0u, 0u, 0u};
+    }
+}
+
 // Streaming copy: the measured HBM ceiling next to the 8 TB/s spec figure, and
 // the known-size kernel that calibrates the FETCH_SIZE / WRITE_SIZE counters.
 // One pass, four 16-byte chunks per thread (all loads before the stores),
@@ -344,6 +367,15 @@ extern "C" int tcs_clear(const tcs_problem *p, void *d_buf, void *stream) {
     if (!d_buf) return set_error(TCS_ERR_INVALID, "clear: null pointer");
     const int64_t n16 = tcs_buffer_bytes(p) / 16;
     hipLaunchKernelGGL(tcs_fill_kernel, dim3(grid_for(n16)), dim3(256), 0, (hipStream_t)stream, (u32x4 *)d_buf, n16);
+    TCS_HIP_CHECK(hipGetLastError());
+    return TCS_OK;
+}
+
+int tcs::clear_ring(const tcs_problem *p, void *d_buf, hipStream_t stream) {
+    const tcs_layout &L = p->layout;
+    const int64_t total = (2 * (int64_t)L.sbc + 2 * (int64_t)(L.sbr - 2)) * 32;
+    hipLaunchKernelGGL(tcs_clear_ring_kernel, dim3(grid_for(total)), dim3(256), 0, stream, (uint16_t *)d_buf,
+                       L.plane_elems, p->precision == TCS_PREC_SPLIT ? 2 : 1, L.sbr, L.sbc);
     TCS_HIP_CHECK(hipGetLastError());
     return TCS_OK;
 }

@@ -215,3 +215,86 @@ def test_error_messages_are_per_thread():
     for t in threads:
         t.join()
     assert "align 3" in seen["a"] and "align 5" in seen["b"], seen
+
+
+@pytest.mark.parametrize("frag", [16, 32])
+def test_reference_tile_size_maps_to_an_offered_tile(frag):
+    """TILE_SIZE of common/param.hpp:11-13 over the reference's sweep 1..15
+    (script/param.py:5), its default 1 and values beyond: every value selects
+    an offered tile with a kernel (no TCS_ERR_UNSUPPORTED, so the host_<stencil>
+    drivers never exit for it); widths 4 for 1..5, 8 from 6 on."""
+    import tcstencil_amd as T
+    B = T.gen_coeffs("box2d2r", 1)
+    for ts in list(range(1, 16)) + [16, 64]:
+        p = T.Problem("box2d2r", 4096, 4096, B, frag=frag)
+        default = p.tile
+        p.set_tile_size(ts)
+        th, tw = p.tile
+        assert tw == (4 if ts < 6 else 8), (ts, p.tile)
+        assert p.kernel_name, (ts, p.tile)
+        assert p.fused_kernel_name or frag == 32 and (th, tw) == (2, 8), (ts, p.tile)
+        if frag == 16:
+            assert th == min(default[0], 4 if tw == 4 else 8)
+    p = T.Problem("box2d1r", 300, 300, T.gen_coeffs("box2d1r", 1))
+    t0 = p.tile
+    p.set_tile_size(0)
+    assert p.tile == t0
+
+
+def test_small_frag32_problems_have_fused_and_batch_tiles():
+    """32x32-fragment defaults stop at 4x4, a tile with single-sweep, fused and
+    batched kernels, also on small grids."""
+    import tcstencil_amd as T
+    for n in (64, 300, 1024, 2048):
+        p = T.Problem("box2d2r", n, n, T.gen_coeffs("box2d2r", 1), frag=32)
+        assert p.tile in ((4, 4), (4, 8), (8, 8)), (n, p.tile)
+        assert p.kernel_name and p.fused_kernel_name, (n, p.tile)
+
+
+class _FakeDeviceTensor:
+    """Duck-typed stand-in for a device tensor: enough for the argument checks,
+    which must reject it before any library call."""
+
+    is_cuda = True
+
+    def __init__(self, nbytes):
+        self.nbytes = nbytes
+
+    def numel(self):
+        return self.nbytes
+
+    def element_size(self):
+        return 1
+
+    def is_contiguous(self):
+        return True
+
+    def data_ptr(self):
+        raise AssertionError("the checks must reject the buffer before using its address")
+
+
+def test_storage_buffers_too_small_are_rejected_before_any_launch():
+    import tcstencil_amd as T
+    B = T.gen_coeffs("box2d2r", 1)
+    p = T.Problem("box2d2r", 100, 200, B)
+    small = _FakeDeviceTensor(p.buffer_bytes - 1)
+    with pytest.raises(ValueError, match="needs"):
+        p.clear(small)
+    with pytest.raises(ValueError, match="needs"):
+        p.sweep(small, small)
+    with pytest.raises(ValueError, match="needs"):
+        p.run(small, small, 3)
+    with pytest.raises(ValueError, match="needs"):
+        p.graph(small, small, 3)
+    with pytest.raises(ValueError, match="needs"):
+        p.unpack(small)
+    # a buffer sized for a smaller tile is too small once the layout is re-aligned
+    q = T.Problem("box2d2r", 100, 300, B, tile=(2, 4))
+    old = _FakeDeviceTensor(q.buffer_bytes)
+    q.set_tile(8, 8)
+    assert q.buffer_bytes > old.nbytes
+    with pytest.raises(ValueError, match="needs"):
+        q.clear(old)
+    with pytest.raises(ValueError, match="grid 0 buf1"):
+        T.Batch([p, p], [_FakeDeviceTensor(p.buffer_bytes)] * 2,
+                [small, _FakeDeviceTensor(p.buffer_bytes)])

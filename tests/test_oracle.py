@@ -165,6 +165,24 @@ def test_reference_style_norms(oracle):
     np.testing.assert_allclose(oracle.check_relative_error_2d(N, out, tiny, 0, N, 0, N), 1.0, rtol=1e-6)
 
 
+def test_reference_style_norms_subtract_in_single_precision(oracle):
+    """common/common.hpp:47,75,105 subtract two floats in float and widen the
+    result.  1.0f - 2^-25 is a tie in float and rounds to 1.0f (even mantissa);
+    a difference taken in double would be 1 - 2^-25."""
+    N = 3
+    out = np.ones((N, N), dtype=np.float32)
+    ref = np.full((N, N), 2.0 ** -25, dtype=np.float32)
+    assert np.float32(1.0) - np.float32(2.0 ** -25) == np.float32(1.0)
+    assert oracle.check_error_2d(N, out, ref, 1, 2, 1, 2) == 1.0
+    # |ref| < MIN_NUMBER: the difference is divided by MIN_NUMBER (common.hpp:76)
+    assert oracle.check_relative_error_2d(N, out, ref, 1, 2, 1, 2) == 1.0 / 1e-6
+    out3 = np.ones((N, N, N), dtype=np.float32)
+    ref3 = np.full((N, N, N), 2.0 ** -25, dtype=np.float32)
+    assert oracle.check_relative_error_3d(N, out3, ref3, 1, 2, 1, 2, 1, 2) == 1.0 / 1e-6
+    # a difference that is exact in float is unchanged: 1.5 - 0.25
+    assert oracle.check_error_2d(N, out + np.float32(0.5), out - np.float32(0.75), 0, N, 0, N) == 1.25
+
+
 def test_reference_style_relative_error_3d(oracle):
     """checkRelativeError3D: closed forms, the MIN_NUMBER floor, and agreement
     with the 2D check on a one-slice box (a transposed index would break it)."""
@@ -186,7 +204,7 @@ def test_reference_style_relative_error_3d(oracle):
     out2 = out.copy()
     out2[0] = 100.0
     assert oracle.check_relative_error_3d(N, out2, ref, z, z + 1, 2, 7, 1, 9) == got
-    sub = (out[2:5, 3:8, 1:4].astype(np.float64) - ref[2:5, 3:8, 1:4]) / ref[2:5, 3:8, 1:4]
+    sub = (out[2:5, 3:8, 1:4] - ref[2:5, 3:8, 1:4]).astype(np.float64) / ref[2:5, 3:8, 1:4]
     np.testing.assert_allclose(oracle.check_relative_error_3d(N, out, ref, 2, 5, 3, 8, 1, 4),
                                np.sqrt((sub ** 2).mean()), rtol=1e-12)
 
