@@ -817,6 +817,49 @@ def test_run_f32_row_strided_views(T, torch):
             assert bool(out[:, cols:].eq(-5.0).all())
 
 
+@pytest.mark.parametrize("precision", ["fp16", "split"])
+def test_run_f32_ignores_prior_buffer_contents(T, torch, precision):
+    """tcs_run_f32 gives the three-step result whatever the storage buffers
+    held before: both filled with 0xFF bytes (a NaN in every FP16 element,
+    ghost ring included), 2 and 3 sweeps, fused edges and the three-step path
+    (direct method)."""
+    rows, cols = 97, 150
+    B = T.gen_coeffs("box2d2r", 7)
+    src = T.gen_uniform(rows, cols, seed=13, dtype=torch.float32)
+    e = T.split_scale_exp(1.0) if precision == "split" else 0
+    kinds = [{"tile": (4, 8)}, {"tile": (2, 4)}] + ([{"method": "direct"}] if precision == "fp16" else [])
+    for kw in kinds:
+        p = T.Problem("box2d2r", rows, cols, B, precision=precision, **kw)
+        for sweeps in (2, 3):
+            say("case run_f32 dirty buffers", precision, kw, sweeps)
+            want = three_steps(T, torch, p, src, sweeps, e)
+            b0, b1 = p.alloc(), p.alloc()
+            b0.fill_(255)
+            b1.fill_(255)
+            got = p.run_f32(src, sweeps, buf0=b0, buf1=b1, scale_exp=e)
+            assert torch.equal(got, want), (kw, sweeps)
+
+
+@pytest.mark.parametrize("tile_size", ["1", "15"])
+def test_dropin_driver_reference_tile_size(tile_size):
+    """The reference's TILE_SIZE values (default 1, sweep up to 15,
+    common/param.hpp:11-13, script/param.py:5) through the drop-in driver:
+    mapped to an offered tile, same checks as the default."""
+    import os
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    for method in ("mfma", "direct"):
+        exe = os.path.join(root, "tests", "dropin", "_build", f"main_{method}_box2d2r")
+        env = dict(os.environ, TCS_RUN_TIMES="5", TCS_SKIP_TIMES="2", TCS_TILE_SIZE=tile_size)
+        out = subprocess.run([exe, "200", "5"], capture_output=True, text=True, timeout=120, env=env)
+        say(method, tile_size, out.stdout)
+        assert out.returncode == 0, out.stdout + out.stderr
+        lines = dict(l.split(":", 1) for l in out.stdout.splitlines() if l.startswith("[Test]"))
+        assert int(lines["[Test] Ring mismatches"]) == 0
+        max_err = float(lines["[Test] Max Error"].split()[0])
+        max_ref = float(lines["[Test] Max Error"].split()[3].rstrip(")"))
+        assert max_err <= 2.0 ** -11 * (0.5 * 578.0 + max_ref) * 1.01 + 1e-6
+
+
 def test_run_f32_other_problems_run_three_steps(T, torch):
     """Problems without an edge kernel (direct, 32x32 fragments) take the
     three-step path through tcs_run_f32; persistent-grid problems run their edge
