@@ -254,6 +254,17 @@ int tcs_oracle_sweep_f64(int stencil, const double *d_in, const double *B, int n
 int tcs_error_norms(const float *d_out, const double *d_ref, int rows, int cols, int r,
                     double *h_out4, void *stream);
 
+/* No counterpart in the reference: tcs_error_norms for callers that cannot
+ * allocate (code under hipGraph capture, serving loops, harnesses that forbid
+ * hipMalloc).  The same two-pass FP64 reduction and the same four results, but
+ * the partials go to a static 48 KiB device scratch array of the library (one
+ * per GPU, held until the process ends) and the results to d_out4 (4 doubles,
+ * device memory); nothing is allocated, copied to the host or synchronised.
+ * Any grid size (the partial pass uses at most 2048 blocks).  Calls that
+ * overlap on one GPU share that GPU's scratch and race: serialise them. */
+int tcs_error_norms_noalloc(const float *d_out, const double *d_ref, int rows, int cols, int r,
+                            double *d_out4, void *stream);
+
 /* ------------------------------------------------------------- generators */
 /* Counter-based (SplitMix64) fills, identical on every launch shape.
  * uniform: element (i,j) = (T)(lo + (hi-lo) * u), u = top 24 bits of

@@ -104,6 +104,7 @@ SIGNATURES = {
     "tcs_batch_launches_per_sweep": (_i, [_vp]),
     "tcs_oracle_sweep_f64": (_i, [_i, _vp, _dp, _i, _vp, _i, _i, _vp]),
     "tcs_error_norms": (_i, [_vp, _vp, _i, _i, _i, _dp, _vp]),
+    "tcs_error_norms_noalloc": (_i, [_vp, _vp, _i, _i, _i, _vp, _vp]),
     "tcs_gen_uniform_f32": (_i, [_vp, _i, _i, _u64, _u64, ctypes.c_float, ctypes.c_float, _vp]),
     "tcs_gen_uniform_f64": (_i, [_vp, _i, _i, _u64, _u64, ctypes.c_double, ctypes.c_double, _vp]),
     "tcs_gen_sinusoid_f64": (_i, [_vp, _i, _i, _u64, ctypes.c_double, _vp]),

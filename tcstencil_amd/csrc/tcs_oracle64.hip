@@ -7,6 +7,8 @@
 // bitwise equal to the CPU gold loop run in FP64 with -ffp-contract=off.
 #include <math.h>
 
+#include <mutex>
+
 #include "tcs_internal.hpp"
 
 #pragma clang fp contract(off)
@@ -101,9 +103,48 @@ __global__ void tcs_norm_final_kernel(const double *__restrict__ part, int nbloc
     out[3] = sqrt(sr);
 }
 
+// Partial blocks of the norm reduction: at most this many (the grid of the
+// partial kernel is capped here whatever the grid size).
+constexpr int kNormMaxBlocks = 2048;
+
+// Scratch of tcs_error_norms_noalloc: the per-block partials, one copy per GPU
+// for the life of the process (48 KiB).  Host code never uses this name as an
+// address: it takes the device address from hipGetSymbolAddress (norm_scratch).
+static __device__ __align__(256) double scratch[3 * kNormMaxBlocks];
+
 }  // namespace tcs
 
 using namespace tcs;
+
+namespace {
+
+// Device address of `scratch` on the current device, looked up once per device
+// (hipGetSymbolAddress with that device current) and kept.
+int norm_scratch(double **out) {
+    constexpr int kMaxDevices = 64;
+    static std::mutex mu;
+    static double *addr[kMaxDevices] = {};
+    int dev = 0;
+    TCS_HIP_CHECK(hipGetDevice(&dev));
+    if (dev < 0 || dev >= kMaxDevices) return set_error(TCS_ERR_UNSUPPORTED, "error_norms_noalloc: device %d", dev);
+    std::lock_guard<std::mutex> lock(mu);
+    if (!addr[dev]) {
+        void *p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess) return hip_error(e, "hipGetSymbolAddress(scratch)");
+        addr[dev] = static_cast<double *>(p);
+    }
+    *out = addr[dev];
+    return TCS_OK;
+}
+
+// Blocks of the partial pass for an interior of `total` points.
+int norm_blocks(int64_t total) {
+    const int64_t nb = (total + 255) / 256;
+    return nb > kNormMaxBlocks ? kNormMaxBlocks : (int)nb;
+}
+
+}  // namespace
 
 extern "C" int tcs_oracle_sweep_f64(int stencil, const double *d_in, const double *B, int n, double *d_out,
                                     int rows, int cols, void *stream) {
@@ -134,18 +175,44 @@ extern "C" int tcs_error_norms(const float *d_out, const double *d_ref, int rows
     if (!d_out || !d_ref || !h_out4) return set_error(TCS_ERR_INVALID, "error_norms: null pointer");
     if (r < 0 || rows <= 2 * r || cols <= 2 * r) return set_error(TCS_ERR_INVALID, "error_norms: no interior");
     const int64_t total = (int64_t)(rows - 2 * r) * (cols - 2 * r);
-    int nb = (int)((total + 255) / 256);
-    if (nb > 2048) nb = 2048;
+    const int nb = norm_blocks(total);
     hipStream_t s = (hipStream_t)stream;
-    double *scratch = nullptr;
-    if (hipMalloc(&scratch, sizeof(double) * (3 * (size_t)nb + 4)) != hipSuccess)
+    double *buf = nullptr;
+    if (hipMalloc(&buf, sizeof(double) * (3 * (size_t)nb + 4)) != hipSuccess)
         return set_error(TCS_ERR_ALLOC, "error_norms: hipMalloc failed");
-    hipLaunchKernelGGL(tcs_norm_partial_kernel, dim3(nb), dim3(256), 0, s, d_out, d_ref, rows, cols, r, scratch);
-    hipLaunchKernelGGL(tcs_norm_final_kernel, dim3(1), dim3(64), 0, s, scratch, nb, total, scratch + 3 * nb);
+    hipLaunchKernelGGL(tcs_norm_partial_kernel, dim3(nb), dim3(256), 0, s, d_out, d_ref, rows, cols, r, buf);
+    hipLaunchKernelGGL(tcs_norm_final_kernel, dim3(1), dim3(64), 0, s, buf, nb, total, buf + 3 * nb);
     hipError_t e = hipGetLastError();
-    if (e == hipSuccess) e = hipMemcpyAsync(h_out4, scratch + 3 * nb, 4 * sizeof(double), hipMemcpyDeviceToHost, s);
+    if (e == hipSuccess) e = hipMemcpyAsync(h_out4, buf + 3 * nb, 4 * sizeof(double), hipMemcpyDeviceToHost, s);
     if (e == hipSuccess) e = hipStreamSynchronize(s);
-    (void)hipFree(scratch);
+    (void)hipFree(buf);
     if (e != hipSuccess) return hip_error(e, "error_norms");
+    return TCS_OK;
+}
+
+// The same two passes with the partials in the library's static scratch and
+// the four results written to the caller's device memory: no allocation, no
+// copy to the host and no synchronisation, so the call can be captured in a
+// graph or issued from a loop that must not allocate.  Every call on a GPU
+// uses that GPU's one scratch array: calls that overlap on one GPU (other
+// streams, other threads) race and must be serialised by the caller.  The
+// partial pass writes every partial the final pass reads, so nothing needs
+// clearing between calls.
+extern "C" int tcs_error_norms_noalloc(const float *d_out, const double *d_ref, int rows, int cols, int r,
+                                       double *d_out4, void *stream) {
+    if (!d_out || !d_ref || !d_out4) return set_error(TCS_ERR_INVALID, "error_norms_noalloc: null pointer");
+    if (r < 0 || rows <= 2 * r || cols <= 2 * r)
+        return set_error(TCS_ERR_INVALID, "error_norms_noalloc: no interior");
+    const int64_t total = (int64_t)(rows - 2 * r) * (cols - 2 * r);
+    const int nb = norm_blocks(total);
+    if (nb > kNormMaxBlocks)
+        return set_error(TCS_ERR_INVALID, "error_norms_noalloc: %d partial blocks exceed the scratch (%d)", nb,
+                         kNormMaxBlocks);
+    double *part = nullptr;
+    TCS_TRY(norm_scratch(&part));
+    hipStream_t s = (hipStream_t)stream;
+    hipLaunchKernelGGL(tcs_norm_partial_kernel, dim3(nb), dim3(256), 0, s, d_out, d_ref, rows, cols, r, part);
+    hipLaunchKernelGGL(tcs_norm_final_kernel, dim3(1), dim3(64), 0, s, part, nb, total, d_out4);
+    TCS_HIP_CHECK(hipGetLastError());
     return TCS_OK;
 }
