@@ -89,6 +89,12 @@ typedef struct tcs_problem {
     int32_t frag;            /* MFMA fragment edge: 16 (v_mfma_f32_16x16x32, one brick per
                                 product; the default) or 32 (v_mfma_f32_32x32x16 over
                                 2x2-brick blocks, FP16/BF16); set with tcs_problem_set_frag */
+    int32_t coef_lo;         /* MFMA FP16/BF16 with 16x16 fragments: 1 = each FP32 tap c is
+                                carried as hi = round(c) plus lo = round(c - hi) in the
+                                storage format, a second MFMA pass per fragment on the same
+                                data (taps exact to 2^-22 FP16 / 2^-16 BF16 instead of
+                                2^-11 / 2^-8); 0 = rounded taps only.  Ignored by SPLIT
+                                (which splits its taps anyway) and the direct method. */
 } tcs_problem;
 
 /* ---------------------------------------------------------- status, params */
@@ -157,6 +163,10 @@ int tcs_problem_set_tile_size(tcs_problem *p, int tile_size);
  * default tile for it (32: 4x4, 2x8, 4x8 or 8x8 single-sweep tiles; 4x4, 4x8, 8x8
  * fused).  32 needs method MFMA and FP16 or BF16. */
 int tcs_problem_set_frag(tcs_problem *p, int frag);
+/* Set coef_lo (hi+lo coefficients, see tcs_problem.coef_lo) and re-derive the
+ * default tile for it (8x8 where the grid allows: the hi+lo kernels run 4 waves
+ * per SIMD).  Buffers packed for another tile must be packed again. */
+int tcs_problem_set_coef_lo(tcs_problem *p, int on);
 /* Bytes of one storage buffer for this problem (2 B/elem; SPLIT: two planes). */
 int64_t tcs_buffer_bytes(const tcs_problem *p);
 

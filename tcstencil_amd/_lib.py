@@ -41,7 +41,8 @@ class ProblemStruct(ctypes.Structure):
                 ("coeff_hi", ctypes.c_uint16 * 25),
                 ("coeff_lo", ctypes.c_uint16 * 25),
                 ("coeff_exp", ctypes.c_int32), ("wg_per_cu", ctypes.c_int32),
-                ("fuse", ctypes.c_int32), ("frag", ctypes.c_int32)]
+                ("fuse", ctypes.c_int32), ("frag", ctypes.c_int32),
+                ("coef_lo", ctypes.c_int32)]
 
 
 class Params(ctypes.Structure):
@@ -81,6 +82,7 @@ SIGNATURES = {
     "tcs_problem_set_tile": (_i, [_prob, _i, _i]),
     "tcs_problem_set_tile_size": (_i, [_prob, _i]),
     "tcs_problem_set_frag": (_i, [_prob, _i]),
+    "tcs_problem_set_coef_lo": (_i, [_prob, _i]),
     "tcs_buffer_bytes": (_i64, [_prob]),
     "tcs_pack_f32": (_i, [_prob, _vp, _i64, _vp, _i, _vp]),
     "tcs_pack_f64": (_i, [_prob, _vp, _i64, _vp, _i, _vp]),

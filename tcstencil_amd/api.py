@@ -62,7 +62,7 @@ class Problem:
     """A tcs_problem descriptor: stencil, method, precision, grid, coefficients."""
 
     def __init__(self, stencil, rows: int, cols: int, B, method="mfma", precision="fp16",
-                 tile=None, wg_per_cu: int = 0, fuse: int = 1, frag: int = 16):
+                 tile=None, wg_per_cu: int = 0, fuse: int = 1, frag: int = 16, coef_lo: bool = False):
         self.stencil = stencil_id(stencil)
         self.method = L.METHOD_ID[method] if isinstance(method, str) else int(method)
         self.precision = L.PREC_ID[precision] if isinstance(precision, str) else int(precision)
@@ -73,6 +73,8 @@ class Problem:
                                      self.precision, rows, cols, _fp(self.B), n))
         if frag != 16:
             check(lib().tcs_problem_set_frag(ctypes.byref(self.s), frag))
+        if coef_lo:
+            check(lib().tcs_problem_set_coef_lo(ctypes.byref(self.s), 1))
         if tile is not None:
             self.set_tile(*tile)
         self.s.wg_per_cu = wg_per_cu
@@ -131,6 +133,14 @@ class Problem:
     @fuse.setter
     def fuse(self, k: int) -> None:
         self.s.fuse = int(k)
+
+    @property
+    def coef_lo(self) -> bool:
+        return bool(self.s.coef_lo)
+
+    def set_coef_lo(self, on: bool) -> None:
+        """Hi+lo coefficients on or off; re-derives the default tile."""
+        check(lib().tcs_problem_set_coef_lo(ctypes.byref(self.s), int(bool(on))))
 
     @property
     def fused_kernel_name(self) -> str:

@@ -45,9 +45,9 @@ extern "C" int tcs_batch_create(tcs_batch **out, int nprob, const tcs_problem *p
     if (!out) return set_error(TCS_ERR_INVALID, "batch: null output");
     *out = nullptr;
     if (nprob < 1 || !probs || !d_buf0 || !d_buf1) return set_error(TCS_ERR_INVALID, "batch: empty or null input");
-    const int prec = probs[0].precision;
-    if (prec != TCS_PREC_FP16 && prec != TCS_PREC_BF16)
+    if (probs[0].precision != TCS_PREC_FP16 && probs[0].precision != TCS_PREC_BF16)
         return set_error(TCS_ERR_UNSUPPORTED, "batch: FP16 or BF16 storage only");
+    const int prec = kernel_prec(&probs[0]);  // with hi+lo coefficients or not
     std::vector<BatchProblem> hp(nprob);
     std::vector<std::vector<BatchTile>> tiles;
     struct Shape {
@@ -60,8 +60,11 @@ extern "C" int tcs_batch_create(tcs_batch **out, int nprob, const tcs_problem *p
     int64_t interior = 0;
     for (int i = 0; i < nprob; ++i) {
         const tcs_problem &p = probs[i];
-        if (p.method != TCS_METHOD_MFMA || p.precision != prec)
-            return set_error(TCS_ERR_INVALID, "batch: problem %d is not MFMA with the batch's precision", i);
+        if (p.method != TCS_METHOD_MFMA || kernel_prec(&p) != prec)
+            return set_error(TCS_ERR_INVALID,
+                             "batch: problem %d is not MFMA with the batch's precision and coefficient form", i);
+        if (has_coef_lo(prec) && frag_of(p) != 16)
+            return set_error(TCS_ERR_UNSUPPORTED, "batch: hi+lo coefficients need 16x16 fragments (problem %d)", i);
         const int frag = frag_of(p);
         if (!batch_tile_offered(p.tile_h, p.tile_w, frag))
             return set_error(TCS_ERR_UNSUPPORTED,

@@ -16,6 +16,14 @@ typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
 typedef uint32_t u32x2 __attribute__((ext_vector_type(2)));
 
 constexpr int kWG = 256;  // threads per workgroup of every tiled kernel (4 wave64)
+
+// Kernel precision ids: the storage precision (TCS_PREC_FP16, _BF16, _SPLIT)
+// in the low two bits, plus kCoefLo for FP16/BF16 MFMA kernels that carry each
+// coefficient as a hi+lo pair of that format (a second MFMA pass per fragment
+// with the lo taps, on the same data operand, into the same accumulator).
+constexpr int kCoefLo = 4;
+__host__ __device__ constexpr int base_prec(int kp) { return kp & 3; }
+__host__ __device__ constexpr bool has_coef_lo(int kp) { return (kp & kCoefLo) != 0; }
 constexpr int kXCD = 8;   // MI355X: 8 XCDs, workgroups dispatched round-robin
 
 // Storage element of global grid point (gr, gc), gr, gc >= -16: bricks are
@@ -109,6 +117,11 @@ int hip_error(hipError_t e, const char *what);
 uint16_t f32_to_f16_bits(float f);
 uint16_t f32_to_bf16_bits(float f);
 float f16_bits_to_f32(uint16_t h);
+float bf16_bits_to_f32(uint16_t h);
+
+// Kernel precision id of a problem (tcs_host.cpp): its precision, with
+// kCoefLo for MFMA FP16/BF16 problems with coef_lo set.
+int kernel_prec(const tcs_problem *p);
 
 // MFMA operand windows of a problem's coefficients (tcs_host.cpp).
 void build_coef_windows(const tcs_problem *p, CoefWindows *cw);

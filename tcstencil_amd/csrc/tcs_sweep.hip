@@ -163,7 +163,7 @@ __device__ __forceinline__ void load_tile(const SweepArgs &a, TileRegs<G> &R, in
 template <int PREC>
 __device__ __forceinline__ uint16_t rm_encode(const SweepArgs &a, float x, int plane = 0) {
     x *= a.rm_scale_in;
-    if constexpr (PREC == TCS_PREC_BF16) {
+    if constexpr (base_prec(PREC) == TCS_PREC_BF16) {
         return __builtin_bit_cast(uint16_t, static_cast<__bf16>(x));
     } else {
         const _Float16 h = static_cast<_Float16>(x);
@@ -181,7 +181,7 @@ __device__ __forceinline__ uint16_t rm_encode(const SweepArgs &a, float x, int p
 // the grid); write_tile<G, true> stores them with the same order.
 template <class G, int PREC>
 __device__ __forceinline__ void load_tile_rm(const SweepArgs &a, TileRegs<G> &R, int r0, int c0, int tid) {
-    static_assert(G::NPL == 1 || PREC == TCS_PREC_SPLIT, "one storage plane, or the SPLIT pair");
+    static_assert(G::NPL == 1 || base_prec(PREC) == TCS_PREC_SPLIT, "one storage plane, or the SPLIT pair");
 #pragma unroll
     for (int i = 0; i < G::CPT; ++i) {
         const int c = tid + i * kWG;
@@ -366,6 +366,10 @@ struct Frag<TCS_PREC_BF16> {
 };
 template <>
 struct Frag<TCS_PREC_SPLIT> : Frag<TCS_PREC_FP16> {};
+template <>
+struct Frag<TCS_PREC_FP16 | kCoefLo> : Frag<TCS_PREC_FP16> {};
+template <>
+struct Frag<TCS_PREC_BF16 | kCoefLo> : Frag<TCS_PREC_BF16> {};
 
 // K slots of the 16x16x32 MFMAs of one brick.  K slot group g (k = 8g..8g+7)
 // of MFMA f multiplies haloed row m + ZLO + zi, columns 8kg..8kg+7 (operand
@@ -413,7 +417,8 @@ struct KMap {
 // Coefficient fragments: lane (j, g) of MFMA f holds c[ZLO + zi][8kg + e - j],
 // e = 0..7 (zero outside the taps) -- the window at d0 = 8kg - j of the
 // zero-padded tap row, loaded whole from the host-built window table
-// (CoefWindows; window 12 is all zeros).
+// (CoefWindows; window 12 is all zeros).  lo: the second plane of the table
+// (SPLIT, and FP16/BF16 with hi+lo coefficients).
 template <int ZLO, int ZHI, int PREC>
 struct Coef {
     typedef typename Frag<PREC>::type frag_t;
@@ -430,7 +435,7 @@ struct Coef {
             const int j = real && d0 >= -7 && d0 <= 4 ? d0 + 7 : 12;
             const uint32_t *wh = cw.w[0][ZLO + zi][j];
             hi[f] = __builtin_bit_cast(frag_t, u32x4{wh[0], wh[1], wh[2], wh[3]});
-            if constexpr (PREC == TCS_PREC_SPLIT) {
+            if constexpr (base_prec(PREC) == TCS_PREC_SPLIT || has_coef_lo(PREC)) {
                 const uint32_t *wl = cw.w[1][ZLO + zi][j];
                 lo[f] = __builtin_bit_cast(frag_t, u32x4{wl[0], wl[1], wl[2], wl[3]});
             }
@@ -447,7 +452,7 @@ struct Ops {
     static_assert(G::STRIP, "the K slot order is chosen for the STRIP LDS layout");
     using K = KMap<ZLO, ZHI>;
     u32x4 h[K::NF];
-    u32x4 l[PREC == TCS_PREC_SPLIT ? K::NF : 1];
+    u32x4 l[base_prec(PREC) == TCS_PREC_SPLIT ? K::NF : 1];
 
     // Per-lane element offsets of the fragments within a brick's window.
     struct Offsets {
@@ -471,7 +476,7 @@ struct Ops {
         for (int f = 0; f < K::NF; ++f) {
             const int o = K::PACKED ? off.o[f] : off.o[0] + G::at(f, 0);
             h[f] = *reinterpret_cast<const u32x4 *>(base + o);
-            if constexpr (PREC == TCS_PREC_SPLIT) l[f] = *reinterpret_cast<const u32x4 *>(base + G::PLANE + o);
+            if constexpr (base_prec(PREC) == TCS_PREC_SPLIT) l[f] = *reinterpret_cast<const u32x4 *>(base + G::PLANE + o);
         }
     }
 };
@@ -494,6 +499,11 @@ __device__ __forceinline__ void mfma_brick(const SweepArgs &a, const uint16_t *l
     for (int f = 0; f < NF; ++f) {
         const frag_t bh = __builtin_bit_cast(frag_t, op.h[f]);
         acc = F::mma(C.hi[f], bh, acc);
+        // Hi+lo coefficients: the lo taps (c - round(c), rounded) on the same
+        // data, so the accumulator sees each tap to 2^-22 (FP16) or 2^-16
+        // (BF16) of its value instead of 2^-11 or 2^-8: no drift from the
+        // rounded taps' sum over many sweeps.
+        if constexpr (has_coef_lo(PREC)) acc = F::mma(C.lo[f], bh, acc);
         if constexpr (PREC == TCS_PREC_SPLIT) {
             // c*x = (ch + cl)(xh + xl) without the cl*xl term: |cl| <= 2^-11 |c|
             // and |xl| <= 2^-11 |x| (round to nearest), so the dropped part of
@@ -508,12 +518,12 @@ __device__ __forceinline__ void mfma_brick(const SweepArgs &a, const uint16_t *l
     uint16_t o[NPL][4];
 #pragma unroll
     for (int v = 0; v < 4; ++v) {
-        if constexpr (PREC == TCS_PREC_SPLIT) {
+        if constexpr (base_prec(PREC) == TCS_PREC_SPLIT) {
             const float sv = acc[v] * a.out_scale;
             const _Float16 hi = static_cast<_Float16>(sv);
             o[0][v] = __builtin_bit_cast(uint16_t, hi);
             o[1][v] = f16_bits(sv - static_cast<float>(hi));
-        } else if constexpr (PREC == TCS_PREC_BF16) {
+        } else if constexpr (base_prec(PREC) == TCS_PREC_BF16) {
             o[0][v] = bf16_bits(acc[v]);
         } else {
             o[0][v] = f16_bits(acc[v]);
@@ -562,16 +572,18 @@ __device__ __forceinline__ void block_brick(int wave, int q, int &br, int &bc) {
 // pays for FP16/BF16 and measured slower for SPLIT, whose three MFMA passes per
 // shift already hide the LDS latency.
 template <int PREC>
-constexpr bool kLookahead = PREC != TCS_PREC_SPLIT;
+constexpr bool kLookahead = base_prec(PREC) != TCS_PREC_SPLIT;
 
 // Single-sweep FP16/BF16 kernels on tiles below 8x8 bricks (LDS for up to 8
 // workgroups per CU) trade the look-ahead for 8 waves per SIMD (at most 64
 // VGPRs): other workgroups then hide the LDS latency, and the extra resident
 // tiles keep more loads in flight (box2d2r 16384^2, 4x8 tile: 193 -> 180 us).
 // 8x8 tiles are held to 4 workgroups per CU by LDS and keep the look-ahead;
-// persistent grids hold the next tile in registers and keep it too.
+// persistent grids hold the next tile in registers and keep it too, and so do
+// kernels with hi+lo coefficients (their lo fragments do not fit in 64 VGPRs).
 template <int TH, int TW, int PREC, bool PERSIST>
-constexpr bool kHighOccupancy = PREC != TCS_PREC_SPLIT && TH * TW < 64 && !PERSIST;
+constexpr bool kHighOccupancy = base_prec(PREC) == PREC && base_prec(PREC) != TCS_PREC_SPLIT && TH * TW < 64 &&
+                                 !PERSIST;
 
 // Row-major FP32 output of 4 results of row gr, columns gc..gc+3 (FP16/BF16
 // bits u; SPLIT: the hi bits u and lo bits ul): what the unpack kernel writes -- the stored value times
@@ -586,9 +598,9 @@ __device__ __forceinline__ void store_rm4(const SweepArgs &a, int gr, int gc, u3
 #pragma unroll
     for (int e = 0; e < 4; ++e) {
         const uint16_t bits = (uint16_t)(u[e >> 1] >> (16 * (e & 1)));
-        float x = PREC == TCS_PREC_BF16 ? static_cast<float>(__builtin_bit_cast(__bf16, bits))
+        float x = base_prec(PREC) == TCS_PREC_BF16 ? static_cast<float>(__builtin_bit_cast(__bf16, bits))
                                         : static_cast<float>(__builtin_bit_cast(_Float16, bits));
-        if constexpr (PREC == TCS_PREC_SPLIT)  // hi + lo, as the unpack kernel decodes the pair
+        if constexpr (base_prec(PREC) == TCS_PREC_SPLIT)  // hi + lo, as the unpack kernel decodes the pair
             x += static_cast<float>(__builtin_bit_cast(_Float16, (uint16_t)(ul[e >> 1] >> (16 * (e & 1)))));
         v[e] = x * a.rm_scale_out;
     }
@@ -612,7 +624,7 @@ __device__ __forceinline__ void store_rm4(const SweepArgs &a, int gr, int gc, u3
 // output written to the row-major FP32 grid a.rm_dst (tcs_mfma_edge).
 template <int TH, int TW, int ZLO, int ZHI, int PREC, bool PERSIST, int MODE>
 __device__ __forceinline__ void sweep_tile16(const SweepArgs &a) {
-    constexpr int NPL = PREC == TCS_PREC_SPLIT ? 2 : 1;
+    constexpr int NPL = base_prec(PREC) == TCS_PREC_SPLIT ? 2 : 1;
     constexpr bool LOOKAHEAD = kLookahead<PREC> && !kHighOccupancy<TH, TW, PREC, PERSIST>;
     static_assert(MODE == 0 || !PERSIST, "row-major edges: one tile per workgroup");
     using T = Tile<TH, TW, NPL, true>;
@@ -738,7 +750,7 @@ void tcs_mfma_sweep(SweepArgs a) {
 // sweep: 0.50 ms unstaged at 8 waves, 0.86 ms staged with spills, 0.45 ms).
 template <int TH, int TW, int ZLO, int ZHI, int PREC, int MODE>
 __global__ __launch_bounds__(kWG)
-__attribute__((amdgpu_waves_per_eu((MODE & 2) ? (PREC == TCS_PREC_SPLIT ? 3 : 4)
+__attribute__((amdgpu_waves_per_eu((MODE & 2) ? (base_prec(PREC) == TCS_PREC_SPLIT ? 3 : 4)
                                                 : (kHighOccupancy<TH, TW, PREC, false> ? 8 : 1), 8)))
 void tcs_mfma_edge(SweepArgs a) {
     sweep_tile16<TH, TW, ZLO, ZHI, PREC, false, MODE>(a);
@@ -760,12 +772,12 @@ using FusedGeom = Geom<TH + 2, TW + 2, NPL, STRIP>;
 template <int TH, int TW, int ZLO, int ZHI, int PREC>
 __device__ __forceinline__ void fused_tile(const SweepArgs &a, const CoefWindows &cw, uint16_t *lds, int gr0,
                                            int gc0, int nsweeps) {
-    constexpr int NPL = PREC == TCS_PREC_SPLIT ? 2 : 1;
+    constexpr int NPL = base_prec(PREC) == TCS_PREC_SPLIT ? 2 : 1;
     // Operand look-ahead fits the 4-waves-per-SIMD register budget of the
     // FP16/BF16 kernels (box2d2r 16384^2, 4x8 tile, 8 fused sweeps: 118 ->
     // 115 us per sweep; box2d1r 16 fused: 94 -> 90 us); it measured slightly
     // slower on 4x4 tiles, and SPLIT has no registers left for it.
-    constexpr bool LOOKAHEAD = PREC != TCS_PREC_SPLIT && TH * TW >= 32;
+    constexpr bool LOOKAHEAD = base_prec(PREC) != TCS_PREC_SPLIT && TH * TW >= 32;
     using G = FusedGeom<TH, TW, NPL, true>;
     constexpr int QA = G::BH * G::BW / 4;  // bricks per wave, whole image
     constexpr int QC = TH * TW / 4;        // bricks per wave, core only
@@ -872,9 +884,9 @@ __device__ __forceinline__ void fused_tile(const SweepArgs &a, const CoefWindows
 // which LDS allows for tiles up to 4x8 (box2d2r 16384^2, 4x8 tile, 8 sweeps
 // fused: 142 -> 134 us per sweep against 2 waves per SIMD).
 template <int TH, int TW, int ZLO, int ZHI, int PREC>
-__global__ __launch_bounds__(kWG) __attribute__((amdgpu_waves_per_eu(PREC == TCS_PREC_SPLIT ? 1 : 4, 8)))
+__global__ __launch_bounds__(kWG) __attribute__((amdgpu_waves_per_eu(base_prec(PREC) == TCS_PREC_SPLIT ? 1 : 4, 8)))
 void tcs_mfma_fused(SweepArgs a, int nsweeps) {
-    constexpr int NPL = PREC == TCS_PREC_SPLIT ? 2 : 1;
+    constexpr int NPL = base_prec(PREC) == TCS_PREC_SPLIT ? 2 : 1;
     __shared__ __attribute__((aligned(16))) uint16_t lds[NPL * FusedGeom<TH, TW, NPL, true>::PLANE];
     const Schedule sc = schedule(a);
     if (sc.tile >= sc.end) return;
@@ -1179,7 +1191,7 @@ __global__ __launch_bounds__(kWG) void tcs_mfma32_fused(SweepArgs a, int nsweeps
 template <int TH, int TW, int ZLO, int ZHI, int PREC>
 __device__ __forceinline__ void batch_tile(const SweepArgs &a, const CoefWindows &cw, uint16_t *lds, int gr0,
                                            int gc0) {
-    constexpr int NPL = PREC == TCS_PREC_SPLIT ? 2 : 1;
+    constexpr int NPL = base_prec(PREC) == TCS_PREC_SPLIT ? 2 : 1;
     constexpr bool LOOKAHEAD = kLookahead<PREC> && !kHighOccupancy<TH, TW, PREC, false>;
     using T = Tile<TH, TW, NPL, true>;
     const int tid = threadIdx.x;
@@ -1283,13 +1295,13 @@ __device__ __forceinline__ void batch_body(const BatchProblem *__restrict__ prob
 // which needs no loop state and lets small tiles run 8 waves per SIMD.
 template <int TH, int TW, int PREC, int FRAG, bool FUSED, bool PERSIST>
 __global__ __launch_bounds__(kWG)
-__attribute__((amdgpu_waves_per_eu(FRAG == 16 && !PERSIST && PREC != TCS_PREC_SPLIT
+__attribute__((amdgpu_waves_per_eu(FRAG == 16 && !PERSIST && base_prec(PREC) != TCS_PREC_SPLIT
                                        ? (FUSED ? 4 : (kHighOccupancy<TH, TW, PREC, false> ? 8 : 1))
                                        : 1,
                                    8)))
 void tcs_mfma_batch(const BatchProblem *__restrict__ probs, const BatchTile *__restrict__ tiles, int ntiles,
                     int tiles_per_xcd, int parity, int nsweeps) {
-    constexpr int NPL = PREC == TCS_PREC_SPLIT ? 2 : 1;
+    constexpr int NPL = base_prec(PREC) == TCS_PREC_SPLIT ? 2 : 1;
     constexpr int PLANE = FUSED ? FusedGeom<TH, TW, NPL, FRAG == 16>::PLANE : Tile<TH, TW, NPL, FRAG == 16>::PLANE;
     __shared__ __attribute__((aligned(16))) uint16_t lds[NPL * PLANE];
     const int x = blockIdx.x % kXCD, step = gridDim.x / kXCD;
@@ -1412,34 +1424,80 @@ __global__ __launch_bounds__(kWG) void tcs_direct_sweep(SweepArgs a) {
 }
 
 // ---------------------------------------------------------------- dispatch
+// This file is compiled once per part (-DTCS_PART=n): parts 1..6 each
+// instantiate one group of kernels and export its tables, part 0 holds the
+// selection and launch code over all of them.  The parts compile in parallel.
+//   1 FP16, 2 BF16, 3 SPLIT (16x16 fragments: single sweep, fused, edge, batch)
+//   4 FP16 and 5 BF16 with hi+lo coefficients (the same kernel kinds)
+//   6 32x32 fragments (single sweep and fused) and the direct-FMA kernel
+#ifndef TCS_PART
+#error "compile with -DTCS_PART=0..6"
+#endif
+
 typedef void (*sweep_fn)(SweepArgs);
+typedef void (*fused_fn)(SweepArgs, int);
+typedef void (*batch_fn)(const BatchProblem *, const BatchTile *, int, int, int, int);
 
 struct KernelEntry {
-    int method, prec, radius, stencil, th, tw;  // stencil = -1: any stencil of that radius
+    int method, prec, radius, stencil, th, tw;  // prec: kernel precision id; stencil = -1: any of that radius
     bool persist;
     int frag;  // MFMA fragment edge (16 or 32); 0 for the direct kernel
     sweep_fn fn;
     const char *name;  // as rocprofv3 reports it (demangled, without the namespace)
 };
 
+// Fused (temporally blocked) MFMA kernels.
+struct FusedEntry {
+    int prec, radius, th, tw, frag;
+    fused_fn fn;
+    const char *name;
+};
+
+// Batched kernels (16x16 and 32x32 fragments, one sweep or several fused).
+struct BatchEntry {
+    int prec, th, tw, frag;
+    bool fused, persist;
+    batch_fn fn;
+    const char *name;
+};
+
+// Edge kernels (tcs_run_f32): 16x16 fragments, tiles 2x4, 4x4, 2x8, 4x8, one
+// tile per workgroup.
+struct EdgeEntry {
+    int prec, radius, th, tw, mode;
+    sweep_fn fn;
+};
+
+struct PartTables {
+    const KernelEntry *sweep;
+    int nsweep;
+    const FusedEntry *fused;
+    int nfused;
+    const BatchEntry *batch;
+    int nbatch;
+    const EdgeEntry *edge;
+    int nedge;
+};
+
+#define TCS_N(a) (int)(sizeof(a) / sizeof((a)[0]))
+
 #define TCS_MFMA(TH, TW, ZLO, ZHI, P, R, PS) \
     {TCS_METHOD_MFMA, P, R, -1, TH, TW, PS, 16, tcs_mfma_sweep<TH, TW, ZLO, ZHI, P, PS>, \
      "tcs_mfma_sweep<" #TH ", " #TW ", " #ZLO ", " #ZHI ", " #P ", " #PS ">"}
 #define TCS_MFMA_TILES(ZLO, ZHI, P, R, PS) \
     TCS_MFMA(2, 4, ZLO, ZHI, P, R, PS), TCS_MFMA(4, 4, ZLO, ZHI, P, R, PS), TCS_MFMA(2, 8, ZLO, ZHI, P, R, PS), \
-    TCS_MFMA(4, 8, ZLO, ZHI, P, R, PS)
-#define TCS_MFMA_ALL(PS)                                                                  \
-    TCS_MFMA_TILES(0, 4, 0, 2, PS), TCS_MFMA(8, 8, 0, 4, 0, 2, PS), TCS_MFMA_TILES(1, 3, 0, 1, PS), \
-    TCS_MFMA(8, 8, 1, 3, 0, 1, PS), TCS_MFMA_TILES(0, 4, 1, 2, PS), TCS_MFMA(8, 8, 0, 4, 1, 2, PS), \
-    TCS_MFMA_TILES(1, 3, 1, 1, PS), TCS_MFMA(8, 8, 1, 3, 1, 1, PS), TCS_MFMA_TILES(0, 4, 2, 2, PS), \
-    TCS_MFMA(8, 8, 0, 4, 2, 2, PS), TCS_MFMA_TILES(1, 3, 2, 1, PS), TCS_MFMA(8, 8, 1, 3, 2, 1, PS)
+    TCS_MFMA(4, 8, ZLO, ZHI, P, R, PS), TCS_MFMA(8, 8, ZLO, ZHI, P, R, PS)
+// Every 16x16 single-sweep tile of precision P, both radii, one tile per
+// workgroup and persistent.
+#define TCS_MFMA_PREC(P)                                                        \
+    TCS_MFMA_TILES(0, 4, P, 2, false), TCS_MFMA_TILES(1, 3, P, 1, false), \
+        TCS_MFMA_TILES(0, 4, P, 2, true), TCS_MFMA_TILES(1, 3, P, 1, true)
 #define TCS_DIRECT(TH, TW, S, PS) \
     {TCS_METHOD_DIRECT, TCS_PREC_FP16, -1, S, TH, TW, PS, 0, tcs_direct_sweep<TH, TW, S, PS>, \
      "tcs_direct_sweep<" #TH ", " #TW ", " #S ", " #PS ">"}
 #define TCS_DIRECT_TILES(S, PS) \
     TCS_DIRECT(2, 4, S, PS), TCS_DIRECT(4, 4, S, PS), TCS_DIRECT(2, 8, S, PS), TCS_DIRECT(4, 8, S, PS), \
     TCS_DIRECT(8, 8, S, PS)
-
 #define TCS_MFMA32(TH, TW, ZLO, ZHI, P, R) \
     {TCS_METHOD_MFMA, P, R, -1, TH, TW, false, 32, tcs_mfma32_sweep<TH, TW, ZLO, ZHI, P>, \
      "tcs_mfma32_sweep<" #TH ", " #TW ", " #ZLO ", " #ZHI ", " #P ">"}
@@ -1447,93 +1505,124 @@ struct KernelEntry {
     TCS_MFMA32(4, 4, ZLO, ZHI, P, R), TCS_MFMA32(2, 8, ZLO, ZHI, P, R), TCS_MFMA32(4, 8, ZLO, ZHI, P, R), \
     TCS_MFMA32(8, 8, ZLO, ZHI, P, R)
 
-// Tile shapes offered (bricks, TH x TW), each one-tile-per-workgroup (false)
-// and persistent (true); SPLIT 8x8 (two planes, 72 KB of LDS) runs two
-// workgroups per CU.  32x32 fragments: FP16 and BF16, tiles
-// whose 2x2-brick blocks split evenly over the 4 waves, one tile per workgroup.
-static const KernelEntry kKernels[] = {
-    TCS_MFMA_ALL(false), TCS_MFMA_ALL(true),
-    TCS_MFMA32_TILES(0, 4, 0, 2), TCS_MFMA32_TILES(1, 3, 0, 1), TCS_MFMA32_TILES(0, 4, 1, 2),
-    TCS_MFMA32_TILES(1, 3, 1, 1),
-    TCS_DIRECT_TILES(0, false), TCS_DIRECT_TILES(1, false), TCS_DIRECT_TILES(2, false), TCS_DIRECT_TILES(3, false),
-    TCS_DIRECT_TILES(0, true), TCS_DIRECT_TILES(1, true), TCS_DIRECT_TILES(2, true), TCS_DIRECT_TILES(3, true),
-};
-static_assert(TCS_PREC_FP16 == 0 && TCS_PREC_BF16 == 1 && TCS_PREC_SPLIT == 2, "precision ids");
-static_assert(TCS_STAR2D1R == 0 && TCS_STAR2D2R == 1 && TCS_BOX2D1R == 2 && TCS_BOX2D2R == 3, "stencil ids");
-
-static int frag_of(const tcs_problem *p) { return p->frag == 32 ? 32 : 16; }
-
-static const KernelEntry *select_kernel(const tcs_problem *p) {
-    for (const KernelEntry &k : kKernels) {
-        if (k.method != p->method || k.prec != p->precision || k.th != p->tile_h || k.tw != p->tile_w ||
-            k.persist != (p->wg_per_cu > 0))
-            continue;
-        if (k.method == TCS_METHOD_MFMA && k.frag != frag_of(p)) continue;
-        if (k.method == TCS_METHOD_MFMA ? k.radius == p->radius : k.stencil == p->stencil) return &k;
-    }
-    return nullptr;
-}
-
-// Fused (temporally blocked) MFMA kernels: tiles 2x4, 4x4, 2x8, 4x8, 8x8 (16x16
-// fragments), 4x4, 4x8, 8x8 (32x32 fragments, FP16 and BF16).
-typedef void (*fused_fn)(SweepArgs, int);
-struct FusedEntry {
-    int prec, radius, th, tw, frag;
-    fused_fn fn;
-    const char *name;
-};
 #define TCS_FUSED(TH, TW, ZLO, ZHI, P, R) \
     {P, R, TH, TW, 16, tcs_mfma_fused<TH, TW, ZLO, ZHI, P>, "tcs_mfma_fused<" #TH ", " #TW ", " #ZLO ", " #ZHI ", " #P ">"}
+#define TCS_FUSED_TILES(ZLO, ZHI, P, R)                                                           \
+    TCS_FUSED(2, 4, ZLO, ZHI, P, R), TCS_FUSED(4, 4, ZLO, ZHI, P, R), TCS_FUSED(2, 8, ZLO, ZHI, P, R), \
+        TCS_FUSED(4, 8, ZLO, ZHI, P, R), TCS_FUSED(8, 8, ZLO, ZHI, P, R)
+#define TCS_FUSED_PREC(P) TCS_FUSED_TILES(0, 4, P, 2), TCS_FUSED_TILES(1, 3, P, 1)
 #define TCS_FUSED32(TH, TW, ZLO, ZHI, P, R) \
     {P, R, TH, TW, 32, tcs_mfma32_fused<TH, TW, ZLO, ZHI, P>, \
      "tcs_mfma32_fused<" #TH ", " #TW ", " #ZLO ", " #ZHI ", " #P ">"}
 #define TCS_FUSED32_TILES(ZLO, ZHI, P, R) \
     TCS_FUSED32(4, 4, ZLO, ZHI, P, R), TCS_FUSED32(4, 8, ZLO, ZHI, P, R), TCS_FUSED32(8, 8, ZLO, ZHI, P, R)
-#define TCS_FUSED_TILES(ZLO, ZHI, P, R)                                                           \
-    TCS_FUSED(2, 4, ZLO, ZHI, P, R), TCS_FUSED(4, 4, ZLO, ZHI, P, R), TCS_FUSED(2, 8, ZLO, ZHI, P, R), \
-        TCS_FUSED(4, 8, ZLO, ZHI, P, R), TCS_FUSED(8, 8, ZLO, ZHI, P, R)
-static const FusedEntry kFused[] = {
-    TCS_FUSED_TILES(0, 4, 0, 2), TCS_FUSED_TILES(1, 3, 0, 1), TCS_FUSED_TILES(0, 4, 1, 2),
-    TCS_FUSED_TILES(1, 3, 1, 1), TCS_FUSED_TILES(0, 4, 2, 2), TCS_FUSED_TILES(1, 3, 2, 1),
-    TCS_FUSED32_TILES(0, 4, 0, 2), TCS_FUSED32_TILES(1, 3, 0, 1), TCS_FUSED32_TILES(0, 4, 1, 2),
-    TCS_FUSED32_TILES(1, 3, 1, 1),
-};
 
-static const FusedEntry *select_fused(const tcs_problem *p) {
-    if (p->method != TCS_METHOD_MFMA) return nullptr;
-    for (const FusedEntry &k : kFused)
-        if (k.prec == p->precision && k.radius == p->radius && k.th == p->tile_h && k.tw == p->tile_w &&
-            k.frag == frag_of(p))
-            return &k;
-    return nullptr;
-}
-
-// Batched kernels: FP16 and BF16; tiles 2x4, 4x4, 4x8 (16x16 fragments) and
-// 4x4, 4x8 (32x32 fragments); one sweep or several fused per launch.
-typedef void (*batch_fn)(const BatchProblem *, const BatchTile *, int, int, int, int);
-struct BatchEntry {
-    int prec, th, tw, frag;
-    bool fused, persist;
-    batch_fn fn;
-    const char *name;
-};
 #define TCS_BATCH(TH, TW, P, F, FU, PS)                                          \
     {P, TH, TW, F, FU, PS, tcs_mfma_batch<TH, TW, P, F, FU, PS>,               \
      "tcs_mfma_batch<" #TH ", " #TW ", " #P ", " #F ", " #FU ", " #PS ">"}
-#define TCS_BATCH_SET(P, FU, PS)                                                                              \
-    TCS_BATCH(2, 4, P, 16, FU, PS), TCS_BATCH(4, 4, P, 16, FU, PS), TCS_BATCH(4, 8, P, 16, FU, PS),             \
-        TCS_BATCH(4, 4, P, 32, FU, PS), TCS_BATCH(4, 8, P, 32, FU, PS)
-static const BatchEntry kBatch[] = {
-    TCS_BATCH_SET(0, false, false), TCS_BATCH_SET(1, false, false), TCS_BATCH_SET(0, true, false),
-    TCS_BATCH_SET(1, true, false),  TCS_BATCH_SET(0, false, true),  TCS_BATCH_SET(1, false, true),
-    TCS_BATCH_SET(0, true, true),   TCS_BATCH_SET(1, true, true),
+#define TCS_BATCH16(P, FU, PS) TCS_BATCH(2, 4, P, 16, FU, PS), TCS_BATCH(4, 4, P, 16, FU, PS), TCS_BATCH(4, 8, P, 16, FU, PS)
+#define TCS_BATCH32(P, FU, PS) TCS_BATCH(4, 4, P, 32, FU, PS), TCS_BATCH(4, 8, P, 32, FU, PS)
+#define TCS_BATCH16_PREC(P) \
+    TCS_BATCH16(P, false, false), TCS_BATCH16(P, true, false), TCS_BATCH16(P, false, true), TCS_BATCH16(P, true, true)
+#define TCS_BATCH32_PREC(P) \
+    TCS_BATCH32(P, false, false), TCS_BATCH32(P, true, false), TCS_BATCH32(P, false, true), TCS_BATCH32(P, true, true)
+
+#define TCS_EDGE(TH, TW, ZLO, ZHI, P, R, M) {P, R, TH, TW, M, tcs_mfma_edge<TH, TW, ZLO, ZHI, P, M>}
+#define TCS_EDGE_MODES(TH, TW, ZLO, ZHI, P, R) \
+    TCS_EDGE(TH, TW, ZLO, ZHI, P, R, 1), TCS_EDGE(TH, TW, ZLO, ZHI, P, R, 2), TCS_EDGE(TH, TW, ZLO, ZHI, P, R, 3)
+#define TCS_EDGE_TILES(ZLO, ZHI, P, R)                                                                   \
+    TCS_EDGE_MODES(2, 4, ZLO, ZHI, P, R), TCS_EDGE_MODES(4, 4, ZLO, ZHI, P, R), TCS_EDGE_MODES(2, 8, ZLO, ZHI, P, R), \
+        TCS_EDGE_MODES(4, 8, ZLO, ZHI, P, R)
+#define TCS_EDGE_PREC(P) TCS_EDGE_TILES(0, 4, P, 2), TCS_EDGE_TILES(1, 3, P, 1)
+
+// The tables below spell kernel precision ids as literals (0 FP16, 1 BF16,
+// 2 SPLIT, 4 and 5 FP16 and BF16 with hi+lo coefficients) so that the kernel
+// names carry the numbers rocprofv3 prints.
+static_assert(TCS_PREC_FP16 == 0 && TCS_PREC_BF16 == 1 && TCS_PREC_SPLIT == 2 && kCoefLo == 4, "precision ids");
+static_assert(TCS_STAR2D1R == 0 && TCS_STAR2D2R == 1 && TCS_BOX2D1R == 2 && TCS_BOX2D2R == 3, "stencil ids");
+
+// The 16x16-fragment kernels of one kernel precision: single sweep, fused,
+// edge, and (FP16/BF16, with or without hi+lo coefficients) batched.
+#define TCS_PART_16(NAME, P, BATCH...)                                                        \
+    static const KernelEntry kSweep##NAME[] = {TCS_MFMA_PREC(P)};                             \
+    static const FusedEntry kFused##NAME[] = {TCS_FUSED_PREC(P)};                             \
+    static const EdgeEntry kEdge##NAME[] = {TCS_EDGE_PREC(P)};                                \
+    static const BatchEntry kBatch##NAME[] = {BATCH};                                         \
+    extern const PartTables kPart##NAME = {kSweep##NAME, TCS_N(kSweep##NAME), kFused##NAME,   \
+                                           TCS_N(kFused##NAME), kBatch##NAME, TCS_N(kBatch##NAME), \
+                                           kEdge##NAME, TCS_N(kEdge##NAME)};
+
+#if TCS_PART == 1
+TCS_PART_16(Fp16, 0, TCS_BATCH16_PREC(0), TCS_BATCH32_PREC(0))
+#elif TCS_PART == 2
+TCS_PART_16(Bf16, 1, TCS_BATCH16_PREC(1), TCS_BATCH32_PREC(1))
+#elif TCS_PART == 3
+// SPLIT (8x8: two workgroups per CU at 72 KB of LDS); no batched kernels.
+static const KernelEntry kSweepSplit[] = {TCS_MFMA_PREC(2)};
+static const FusedEntry kFusedSplit[] = {TCS_FUSED_PREC(2)};
+static const EdgeEntry kEdgeSplit[] = {TCS_EDGE_PREC(2)};
+extern const PartTables kPartSplit = {kSweepSplit, TCS_N(kSweepSplit), kFusedSplit, TCS_N(kFusedSplit),
+                                      nullptr, 0, kEdgeSplit, TCS_N(kEdgeSplit)};
+#elif TCS_PART == 4
+TCS_PART_16(Fp16Lo, 4, TCS_BATCH16_PREC(4))
+#elif TCS_PART == 5
+TCS_PART_16(Bf16Lo, 5, TCS_BATCH16_PREC(5))
+#elif TCS_PART == 6
+// 32x32 fragments (FP16 and BF16, tiles whose 2x2-brick blocks split evenly over
+// the 4 waves, one tile per workgroup) and the direct-FMA kernel.
+static const KernelEntry kSweepOther[] = {
+    TCS_MFMA32_TILES(0, 4, 0, 2), TCS_MFMA32_TILES(1, 3, 0, 1), TCS_MFMA32_TILES(0, 4, 1, 2), TCS_MFMA32_TILES(1, 3, 1, 1),
+    TCS_DIRECT_TILES(0, false), TCS_DIRECT_TILES(1, false), TCS_DIRECT_TILES(2, false), TCS_DIRECT_TILES(3, false),
+    TCS_DIRECT_TILES(0, true), TCS_DIRECT_TILES(1, true), TCS_DIRECT_TILES(2, true), TCS_DIRECT_TILES(3, true),
 };
+static const FusedEntry kFusedOther[] = {
+    TCS_FUSED32_TILES(0, 4, 0, 2), TCS_FUSED32_TILES(1, 3, 0, 1), TCS_FUSED32_TILES(0, 4, 1, 2), TCS_FUSED32_TILES(1, 3, 1, 1),
+};
+extern const PartTables kPartOther = {kSweepOther, TCS_N(kSweepOther), kFusedOther, TCS_N(kFusedOther),
+                                      nullptr, 0, nullptr, 0};
+#elif TCS_PART == 0
+extern const PartTables kPartFp16, kPartBf16, kPartSplit, kPartFp16Lo, kPartBf16Lo, kPartOther;
+static const PartTables *const kParts[] = {&kPartFp16, &kPartBf16, &kPartSplit, &kPartFp16Lo, &kPartBf16Lo,
+                                           &kPartOther};
+
+static int frag_of(const tcs_problem *p) { return p->frag == 32 ? 32 : 16; }
+
+static const KernelEntry *select_kernel(const tcs_problem *p) {
+    const int kp = kernel_prec(p);
+    for (const PartTables *t : kParts)
+        for (int i = 0; i < t->nsweep; ++i) {
+            const KernelEntry &k = t->sweep[i];
+            if (k.method != p->method || k.th != p->tile_h || k.tw != p->tile_w || k.persist != (p->wg_per_cu > 0))
+                continue;
+            if (k.method == TCS_METHOD_MFMA ? (k.prec != kp || k.frag != frag_of(p) || k.radius != p->radius)
+                                            : k.stencil != p->stencil)
+                continue;
+            return &k;
+        }
+    return nullptr;
+}
+
+static const FusedEntry *select_fused(const tcs_problem *p) {
+    if (p->method != TCS_METHOD_MFMA) return nullptr;
+    const int kp = kernel_prec(p);
+    for (const PartTables *t : kParts)
+        for (int i = 0; i < t->nfused; ++i) {
+            const FusedEntry &k = t->fused[i];
+            if (k.prec == kp && k.radius == p->radius && k.th == p->tile_h && k.tw == p->tile_w &&
+                k.frag == frag_of(p))
+                return &k;
+        }
+    return nullptr;
+}
 
 static const BatchEntry *select_batch(int prec, int th, int tw, int frag, bool fused, bool persist) {
-    for (const BatchEntry &k : kBatch)
-        if (k.prec == prec && k.th == th && k.tw == tw && k.frag == frag && k.fused == fused &&
-            k.persist == persist)
-            return &k;
+    for (const PartTables *t : kParts)
+        for (int i = 0; i < t->nbatch; ++i) {
+            const BatchEntry &k = t->batch[i];
+            if (k.prec == prec && k.th == th && k.tw == tw && k.frag == frag && k.fused == fused &&
+                k.persist == persist)
+                return &k;
+        }
     return nullptr;
 }
 
@@ -1571,9 +1660,10 @@ int mfma_per_brick(const tcs_problem *p) {
     if (p->method != TCS_METHOD_MFMA) return 0;
     if (frag_of(p) == 32) return p->radius == 2 ? Fold32<0, 4>::NMF : Fold32<1, 3>::NMF;  // per 2x2-brick block
     // Radius 2 packs its K slots into 4 MFMAs per brick (3 for radius 1);
-    // SPLIT issues 3 per fragment (hi*hi, hi*lo, lo*hi).
+    // SPLIT issues 3 per fragment (hi*hi, hi*lo, lo*hi), hi+lo coefficients 2.
     const int nf = p->radius == 2 ? KMap<0, 4>::NF : KMap<1, 3>::NF;
-    return p->precision == TCS_PREC_SPLIT ? 3 * nf : nf;
+    const int kp = kernel_prec(p);
+    return base_prec(kp) == TCS_PREC_SPLIT ? 3 * nf : (has_coef_lo(kp) ? 2 * nf : nf);
 }
 
 int64_t mfma_flops_per_brick(const tcs_problem *p) {
@@ -1614,7 +1704,7 @@ int launch_fused(const tcs_problem *p, const void *d_in, void *d_out, int nsweep
     const FusedEntry *k = select_fused(p);
     if (!k)
         return set_error(TCS_ERR_UNSUPPORTED, "no fused kernel for method %d precision %d tile %dx%d", p->method,
-                         p->precision, p->tile_h, p->tile_w);
+                         kernel_prec(p), p->tile_h, p->tile_w);
     if (nsweeps < 1 || p->radius * (nsweeps - 1) > 16)
         return set_error(TCS_ERR_INVALID, "fused launch of %d sweeps: radius %d allows 1..%d", nsweeps, p->radius,
                          1 + 16 / p->radius);
@@ -1626,32 +1716,15 @@ int launch_fused(const tcs_problem *p, const void *d_in, void *d_out, int nsweep
     return TCS_OK;
 }
 
-// Edge kernels (tcs_run_f32): FP16 and BF16, 16x16 fragments, the tiles the
-// default rule picks (2x4, 4x4, 2x8, 4x8), one tile per workgroup.
-struct EdgeEntry {
-    int prec, radius, th, tw, mode;
-    sweep_fn fn;
-};
-#define TCS_EDGE(TH, TW, ZLO, ZHI, P, R, M) {P, R, TH, TW, M, tcs_mfma_edge<TH, TW, ZLO, ZHI, P, M>}
-#define TCS_EDGE_MODES(TH, TW, ZLO, ZHI, P, R) \
-    TCS_EDGE(TH, TW, ZLO, ZHI, P, R, 1), TCS_EDGE(TH, TW, ZLO, ZHI, P, R, 2), TCS_EDGE(TH, TW, ZLO, ZHI, P, R, 3)
-#define TCS_EDGE_TILES(ZLO, ZHI, P, R)                                                                   \
-    TCS_EDGE_MODES(2, 4, ZLO, ZHI, P, R), TCS_EDGE_MODES(4, 4, ZLO, ZHI, P, R), TCS_EDGE_MODES(2, 8, ZLO, ZHI, P, R), \
-        TCS_EDGE_MODES(4, 8, ZLO, ZHI, P, R)
-#define TCS_EDGE_SPLIT_TILES(ZLO, ZHI, R)                                                                \
-    TCS_EDGE_MODES(2, 4, ZLO, ZHI, 2, R), TCS_EDGE_MODES(4, 4, ZLO, ZHI, 2, R), TCS_EDGE_MODES(2, 8, ZLO, ZHI, 2, R), \
-        TCS_EDGE_MODES(4, 8, ZLO, ZHI, 2, R)
-static const EdgeEntry kEdge[] = {
-    TCS_EDGE_TILES(0, 4, 0, 2), TCS_EDGE_TILES(1, 3, 0, 1), TCS_EDGE_TILES(0, 4, 1, 2), TCS_EDGE_TILES(1, 3, 1, 1),
-    TCS_EDGE_SPLIT_TILES(0, 4, 2), TCS_EDGE_SPLIT_TILES(1, 3, 1),
-};
-
 static const EdgeEntry *select_edge(const tcs_problem *p, int mode) {
     if (p->method != TCS_METHOD_MFMA || frag_of(p) != 16 || p->wg_per_cu > 0) return nullptr;
-    for (const EdgeEntry &k : kEdge)
-        if (k.prec == p->precision && k.radius == p->radius && k.th == p->tile_h && k.tw == p->tile_w &&
-            k.mode == mode)
-            return &k;
+    const int kp = kernel_prec(p);
+    for (const PartTables *t : kParts)
+        for (int i = 0; i < t->nedge; ++i) {
+            const EdgeEntry &k = t->edge[i];
+            if (k.prec == kp && k.radius == p->radius && k.th == p->tile_h && k.tw == p->tile_w && k.mode == mode)
+                return &k;
+        }
     return nullptr;
 }
 
@@ -1704,7 +1777,7 @@ int launch_sweep(const tcs_problem *p, const void *d_in, void *d_out, hipStream_
     const KernelEntry *k = select_kernel(p);
     if (!k)
         return set_error(TCS_ERR_UNSUPPORTED, "no kernel for stencil %d method %d precision %d tile %dx%d",
-                         p->stencil, p->method, p->precision, p->tile_h, p->tile_w);
+                         p->stencil, p->method, kernel_prec(p), p->tile_h, p->tile_w);
     SweepArgs a;
     int st = make_args(p, d_in, d_out, a);
     if (st) return st;
@@ -1723,5 +1796,6 @@ int launch_sweep(const tcs_problem *p, const void *d_in, void *d_out, hipStream_
     TCS_HIP_CHECK(hipGetLastError());
     return TCS_OK;
 }
+#endif  // TCS_PART
 
 }  // namespace tcs

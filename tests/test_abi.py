@@ -48,7 +48,7 @@ def test_host_code_shims_export_host_code(stencil):
 def test_struct_sizes_match_c():
     from tcstencil_amd import _lib
     assert ctypes.sizeof(_lib.Layout) == 40
-    assert ctypes.sizeof(_lib.ProblemStruct) == 280
+    assert ctypes.sizeof(_lib.ProblemStruct) == 288
     assert ctypes.sizeof(_lib.Params) == 32
 
 
@@ -298,3 +298,28 @@ def test_storage_buffers_too_small_are_rejected_before_any_launch():
     with pytest.raises(ValueError, match="grid 0 buf1"):
         T.Batch([p, p], [_FakeDeviceTensor(p.buffer_bytes)] * 2,
                 [small, _FakeDeviceTensor(p.buffer_bytes)])
+
+
+def test_hi_lo_coefficient_option():
+    """coef_lo selects the hi+lo coefficient kernels (kernel precision ids 4 and
+    5: two MFMAs per fragment) for MFMA FP16/BF16 with 16x16 fragments, prefers
+    8x8 tiles, and is ignored by SPLIT and the direct method; 32x32 fragments
+    have no such kernel (UNSUPPORTED at launch: no kernel name)."""
+    import tcstencil_amd as T
+    B = T.gen_coeffs("box2d2r", 1)
+    for prec, kp in (("fp16", 4), ("bf16", 5)):
+        p = T.Problem("box2d2r", 16384, 16384, B, precision=prec, coef_lo=True)
+        assert p.coef_lo and p.tile == (8, 8) and p.mfma_per_brick == 8
+        assert p.kernel_name == f"tcs_mfma_sweep<8, 8, 0, 4, {kp}, false>"
+        assert p.fused_kernel_name == f"tcs_mfma_fused<8, 8, 0, 4, {kp}>"
+        q = T.Problem("box2d1r", 16384, 16384, T.gen_coeffs("box2d1r", 1), precision=prec, coef_lo=True)
+        assert q.mfma_per_brick == 6
+    s = T.Problem("box2d2r", 1024, 1024, B, precision="split", coef_lo=True)
+    assert s.mfma_per_brick == 12 and ", 2, false>" in s.kernel_name
+    d = T.Problem("box2d2r", 1024, 1024, B, method="direct", coef_lo=True)
+    assert d.kernel_name.startswith("tcs_direct_sweep")
+    f = T.Problem("box2d2r", 1024, 1024, B, frag=32, coef_lo=True)
+    assert f.kernel_name == "" and f.fused_kernel_name == ""
+    p = T.Problem("box2d2r", 1024, 1024, B)
+    p.set_coef_lo(True)
+    assert p.coef_lo and ", 4, false>" in p.kernel_name
