@@ -193,6 +193,11 @@ int tcs_maxabs_f32(const float *d_src, int rows, int cols, int64_t ld, double *d
 int tcs_maxabs_f64(const double *d_src, int rows, int cols, int64_t ld, double *d_out, void *stream);
 /* Per-grid scale exponent for SPLIT storage given max|x| (host helper). */
 int tcs_split_scale_exp(double maxabs);
+/* The same for a run of `sweeps` sweeps of problem p: taps whose absolute sum S
+ * exceeds 1 can grow max|x| by S per sweep, so the exponent is that of
+ * max|x| * S^sweeps (growth capped at 2^60), keeping every stored value below
+ * binary16's largest finite value.  With S <= 1 it equals tcs_split_scale_exp. */
+int tcs_split_scale_exp_for(const tcs_problem *p, double maxabs, int sweeps);
 
 /* ----------------------------------------------------------------- sweeping */
 /* One sweep d_in -> d_out (distinct storage buffers of this problem; d_out's

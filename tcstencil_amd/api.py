@@ -378,8 +378,12 @@ def maxabs(x, stream=None) -> float:
     return float(d.item())
 
 
-def split_scale_exp(mx: float) -> int:
-    return int(lib().tcs_split_scale_exp(mx))
+def split_scale_exp(mx: float, problem: "Problem | None" = None, sweeps: int = 1) -> int:
+    """SPLIT's per-grid scale exponent for max|x| = mx; with a problem, also
+    for the growth its taps allow over `sweeps` sweeps (tcs_split_scale_exp_for)."""
+    if problem is None:
+        return int(lib().tcs_split_scale_exp(mx))
+    return int(lib().tcs_split_scale_exp_for(problem.ref, mx, sweeps))
 
 
 def gen_uniform(rows: int, cols: int, seed: int, stream_id: int = 0, lo=-1.0, hi=1.0,

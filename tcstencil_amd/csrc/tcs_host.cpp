@@ -325,6 +325,20 @@ extern "C" int64_t tcs_buffer_bytes(const tcs_problem *p) {
     return p->layout.plane_elems * 2 * (p->precision == TCS_PREC_SPLIT ? 2 : 1);
 }
 
+// SPLIT storage holds x * 2^s as an FP16 pair, so every value a run produces
+// must stay below binary16's 65504 after scaling: with taps of absolute sum
+// S > 1 a sweep can grow max|x| by S, so the exponent is taken for
+// max|x| * max(1, S)^sweeps (growth capped at 2^60).
+extern "C" int tcs_split_scale_exp_for(const tcs_problem *p, double maxabs, int sweeps) {
+    if (!p) return tcs_split_scale_exp(maxabs);
+    double s = 0.0;
+    for (int k = 0; k < 25; ++k) s += fabs((double)p->coeff[k]);
+    double growth = 1.0;
+    for (int i = 0; i < sweeps && s > 1.0 && growth < 0x1p60; ++i) growth *= s;
+    if (growth > 0x1p60) growth = 0x1p60;
+    return tcs_split_scale_exp(maxabs * growth);
+}
+
 extern "C" int tcs_split_scale_exp(double maxabs) {
     // Bring max|x| to [2^9, 2^10): the lo parts keep full precision and a
     // stencil with |sum of coefficients| up to 32 cannot overflow binary16.
@@ -599,7 +613,7 @@ static int host_sweep_impl(int stencil, int method, int precision, const float *
         double mx = 0.0;
         TCS_HIP_CHECK(hipMemcpyAsync(&mx, d_max, sizeof mx, hipMemcpyDeviceToHost, s));
         TCS_HIP_CHECK(hipStreamSynchronize(s));
-        scale_exp = tcs_split_scale_exp(mx);
+        scale_exp = tcs_split_scale_exp_for(&p, mx, sweeps);
     }
     TCS_TRY(tcs_pack_f32(&p, (const float *)d_src, cols, buf0, scale_exp, s));
     TCS_TRY(tcs_clear(&p, buf1, s));

@@ -127,6 +127,26 @@ def test_split_scale_exponent():
     assert T.split_scale_exp(0.0) == 0
 
 
+def test_split_scale_exponent_covers_tap_growth(oracle):
+    """Taps with absolute sum S > 1 (the reference's init_B: S = 386 for
+    star2d2r) grow max|x| by up to S per sweep; the problem-aware exponent keeps
+    every stored value of the run, scaled, below binary16's 65504, while
+    normalised taps (S = 1) keep the plain exponent."""
+    import tcstencil_amd as T
+    for stencil in ("star2d2r", "box2d1r", "box2d2r"):
+        B = oracle.init_B(stencil)
+        n = B.shape[0]
+        p = T.Problem(stencil, 64, 64, B, precision="split")
+        S = float(np.abs(np.array(p.s.coeff, dtype=np.float64)).sum())
+        for sweeps in (1, 3):
+            e = T.split_scale_exp(1.0, p, sweeps)
+            assert 1.0 * S ** sweeps * 2.0 ** e < 2 ** 10, (stencil, sweeps, e)
+            assert 1.0 * S ** sweeps * 2.0 ** e >= 2 ** 9
+        assert n == 2 * T.radius(stencil) + 1
+    q = T.Problem("box2d2r", 64, 64, T.gen_coeffs("box2d2r", 1), precision="split")
+    assert T.split_scale_exp(0.7, q, 10) == T.split_scale_exp(0.7)
+
+
 def test_gen_coeffs_normalised():
     import tcstencil_amd as T
     for s in T.STENCIL_NAMES:
