@@ -167,6 +167,35 @@ int tcs_problem_set_frag(tcs_problem *p, int frag);
  * default tile for it (8x8 where the grid allows: the hi+lo kernels run 4 waves
  * per SIMD).  Buffers packed for another tile must be packed again. */
 int tcs_problem_set_coef_lo(tcs_problem *p, int on);
+/* ------------------------------------------------------------ tuning cache */
+/* Parameters the autotuner (python -m tcstencil_amd.autotune --write-cache)
+ * measured fastest, kept in a text file per device: "device <arch> <CUs>", then
+ * "entry <stencil> <method> <precision> <coef_lo> <batched> <bucket> <frag>
+ * <tile_h> <tile_w> <wg_per_cu> <fuse>" lines.  This replaces the reference's
+ * compile-time TILE_SIZE (common/param.hpp:11-13) swept offline
+ * (script/param.py:5).  Tile, wg_per_cu and fuse never change a result; the
+ * fragment (MFMA only) changes the grouping of the FP32 sums. */
+typedef struct tcs_tuning {
+    int32_t tile_h, tile_w, wg_per_cu, fuse, frag;
+} tcs_tuning;
+/* Size bucket of a rows x cols grid: the smaller side in bricks in [0,16),
+ * [16,64), [64,256) or [256,inf) -> 0..3. */
+int tcs_tune_bucket(int rows, int cols);
+/* The cached parameters for problem p (single grid: batched = 0; a grid of a
+ * tcs_batch: 1) from the file at `path` (NULL or "": $TCS_TUNE_CACHE).  Entries
+ * apply only on the device their file names.  TCS_ERR_UNSUPPORTED: no file or
+ * no matching entry; TCS_ERR_INVALID: unreadable or malformed file. */
+int tcs_tune_lookup(const char *path, const tcs_problem *p, int batched, tcs_tuning *out);
+/* Set p's tile, wg_per_cu and (MFMA) fragment and fuse from its cache entry; unchanged when
+ * there is none.  Re-derives the layout, so do it before allocating buffers.
+ * tcs_problem_init, tcs_problem_set_frag and tcs_problem_set_coef_lo apply the
+ * single-grid entry of $TCS_TUNE_CACHE themselves when it is set (re-deriving
+ * tile, wg_per_cu and fuse together; an explicit fragment is kept); batched grids
+ * call this with batched = 1 before allocating their buffers (tcs_batch_create
+ * takes each grid's descriptor, tile included, as given: the tile fixes the
+ * grid's storage layout). */
+int tcs_problem_apply_tuning(tcs_problem *p, const char *path, int batched);
+
 /* Bytes of one storage buffer for this problem (2 B/elem; SPLIT: two planes). */
 int64_t tcs_buffer_bytes(const tcs_problem *p);
 

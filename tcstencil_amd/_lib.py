@@ -45,6 +45,11 @@ class ProblemStruct(ctypes.Structure):
                 ("coef_lo", ctypes.c_int32)]
 
 
+class Tuning(ctypes.Structure):
+    _fields_ = [("tile_h", ctypes.c_int32), ("tile_w", ctypes.c_int32), ("wg_per_cu", ctypes.c_int32),
+                ("fuse", ctypes.c_int32), ("frag", ctypes.c_int32)]
+
+
 class Params(ctypes.Structure):
     """common/param.hpp:1-13 as a runtime struct (same names and defaults)."""
     _fields_ = [("TOLERANCE", ctypes.c_double), ("MIN_NUMBER", ctypes.c_double),
@@ -84,6 +89,9 @@ SIGNATURES = {
     "tcs_problem_set_frag": (_i, [_prob, _i]),
     "tcs_problem_set_coef_lo": (_i, [_prob, _i]),
     "tcs_buffer_bytes": (_i64, [_prob]),
+    "tcs_tune_bucket": (_i, [_i, _i]),
+    "tcs_tune_lookup": (_i, [ctypes.c_char_p, _prob, _i, _P(Tuning)]),
+    "tcs_problem_apply_tuning": (_i, [_prob, ctypes.c_char_p, _i]),
     "tcs_pack_f32": (_i, [_prob, _vp, _i64, _vp, _i, _vp]),
     "tcs_pack_f64": (_i, [_prob, _vp, _i64, _vp, _i, _vp]),
     "tcs_unpack_f32": (_i, [_prob, _vp, _vp, _i64, _i, _vp, _vp]),

@@ -62,7 +62,7 @@ class Problem:
     """A tcs_problem descriptor: stencil, method, precision, grid, coefficients."""
 
     def __init__(self, stencil, rows: int, cols: int, B, method="mfma", precision="fp16",
-                 tile=None, wg_per_cu: int = 0, fuse: int = 1, frag: int = 16, coef_lo: bool = False):
+                 tile=None, wg_per_cu=None, fuse=None, frag: int = 16, coef_lo: bool = False, tuned=None):
         self.stencil = stencil_id(stencil)
         self.method = L.METHOD_ID[method] if isinstance(method, str) else int(method)
         self.precision = L.PREC_ID[precision] if isinstance(precision, str) else int(precision)
@@ -75,14 +75,24 @@ class Problem:
             check(lib().tcs_problem_set_frag(ctypes.byref(self.s), frag))
         if coef_lo:
             check(lib().tcs_problem_set_coef_lo(ctypes.byref(self.s), 1))
+        if tuned is not None:
+            # tuned=(path or "", batched): the cached parameters of that cache file
+            self.apply_tuning(*tuned)
         if tile is not None:
             self.set_tile(*tile)
-        self.s.wg_per_cu = wg_per_cu
-        self.s.fuse = fuse
+        if wg_per_cu is not None:
+            self.s.wg_per_cu = wg_per_cu
+        if fuse is not None:
+            self.s.fuse = fuse
 
     # -- descriptor ---------------------------------------------------------
     def set_tile(self, tile_h: int, tile_w: int) -> None:
         check(lib().tcs_problem_set_tile(ctypes.byref(self.s), tile_h, tile_w))
+
+    def apply_tuning(self, path: str = "", batched: bool = False) -> None:
+        """Tile, wg_per_cu and sweeps per launch from a tuning cache file ("" or
+        None: $TCS_TUNE_CACHE); unchanged when it has no entry for this problem."""
+        check(lib().tcs_problem_apply_tuning(ctypes.byref(self.s), (path or "").encode(), int(bool(batched))))
 
     def set_tile_size(self, tile_size: int) -> None:
         """Apply the reference's TILE_SIZE (bricks per block along x, 1..15)."""

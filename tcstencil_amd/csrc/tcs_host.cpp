@@ -223,6 +223,22 @@ static void default_tile(const tcs_problem *p, int *th, int *tw) {
     *tw = kPref[pick][1];
 }
 
+extern "C" int tcs_problem_set_tile(tcs_problem *p, int tile_h, int tile_w);
+
+// The default tile of the problem's method, precision and fragment, then the
+// single-grid entry of $TCS_TUNE_CACHE when that names a file (keeping the
+// fragment when want_frag is set).
+static int derive_tile(tcs_problem *p, int want_frag = 0) {
+    int th, tw;
+    default_tile(p, &th, &tw);
+    TCS_TRY(tcs_problem_set_tile(p, th, tw));
+    const char *cache = getenv("TCS_TUNE_CACHE");
+    if (!cache || !*cache) return TCS_OK;
+    p->wg_per_cu = 0;  // the launch parameters are re-derived with the tile
+    p->fuse = 0;
+    return apply_tuning(p, cache, 0, want_frag);
+}
+
 extern "C" int tcs_problem_set_tile(tcs_problem *p, int tile_h, int tile_w) {
     if (!p) return set_error(TCS_ERR_INVALID, "set_tile: null problem");
     if (!tile_offered(tile_h, tile_w))
@@ -296,9 +312,7 @@ extern "C" int tcs_problem_init(tcs_problem *p, int stencil, int method, int pre
             p->coeff_lo[k] = f32_to_f16_bits(c - f16_bits_to_f32(hb));
         }
     }
-    int th, tw;
-    default_tile(p, &th, &tw);
-    return tcs_problem_set_tile(p, th, tw);
+    return derive_tile(p);
 }
 
 extern "C" int tcs_problem_set_frag(tcs_problem *p, int frag) {
@@ -307,17 +321,13 @@ extern "C" int tcs_problem_set_frag(tcs_problem *p, int frag) {
     if (frag == 32 && (p->method != TCS_METHOD_MFMA || p->precision == TCS_PREC_SPLIT))
         return set_error(TCS_ERR_UNSUPPORTED, "set_frag: 32x32 fragments need method MFMA with FP16 or BF16");
     p->frag = frag;
-    int th, tw;
-    default_tile(p, &th, &tw);
-    return tcs_problem_set_tile(p, th, tw);
+    return derive_tile(p, frag);
 }
 
 extern "C" int tcs_problem_set_coef_lo(tcs_problem *p, int on) {
     if (!p) return set_error(TCS_ERR_INVALID, "set_coef_lo: null problem");
     p->coef_lo = on ? 1 : 0;
-    int th, tw;
-    default_tile(p, &th, &tw);
-    return tcs_problem_set_tile(p, th, tw);
+    return derive_tile(p);
 }
 
 extern "C" int64_t tcs_buffer_bytes(const tcs_problem *p) {

@@ -123,6 +123,10 @@ float bf16_bits_to_f32(uint16_t h);
 // kCoefLo for MFMA FP16/BF16 problems with coef_lo set.
 int kernel_prec(const tcs_problem *p);
 
+// Tuning cache entry applied to p (tcs_tune.cpp); want_frag != 0 keeps an
+// explicitly chosen fragment (entries for the other one are skipped).
+int apply_tuning(tcs_problem *p, const char *path, int batched, int want_frag);
+
 // MFMA operand windows of a problem's coefficients (tcs_host.cpp).
 void build_coef_windows(const tcs_problem *p, CoefWindows *cw);
 

@@ -23,7 +23,8 @@ FLAG_DICT script/param.py:1-6), run each under `ncu` and scrape "Duration"
       and the kernel's average duration comes from the profiler's stats file,
       otherwise from HIP events.  The CSV keeps the reference's columns
       `type,stencil,mesh size,tile size,time,unit` (process_data.py:14) and adds
-      GStencil/s and HBM GB/s.
+      GStencil/s (interior updates, (N-2r)^2 / t), the reference's figure rate
+      N^2 / t (draw.py:19) in G points/s, and HBM GB/s.
 """
 from __future__ import annotations
 
@@ -138,7 +139,8 @@ def run_edges(stencil, rows, cols, precision="fp16", sweeps=1, reps=20, seed=1, 
 def rocprof_kernel_avg_ns(cmd, kernel_prefix, workdir):
     """Run `cmd` under rocprofv3 --kernel-trace --stats; return (average ns, calls)
     of the kernels whose name contains kernel_prefix."""
-    full = ["rocprofv3", "--kernel-trace", "--stats", "-d", workdir, "-o", "run", "--"] + cmd
+    full = ["rocprofv3", "--kernel-trace", "--stats", "--output-format", "csv", "-d", workdir, "-o", "run",
+            "--"] + cmd
     subprocess.run(full, check=True, stdout=subprocess.DEVNULL, stderr=subprocess.DEVNULL)
     files = glob.glob(os.path.join(workdir, "**", "*kernel_stats.csv"), recursive=True)
     if not files:
@@ -222,6 +224,8 @@ def main(argv=None):
         row = {"type": "tensor" if method == "mfma" else "baseline", "stencil": stencil,
                "mesh size": size, "tile size": f"{th}x{tw}", "time": round(ms * 1e3, 3), "unit": "us",
                "gstencil_s": round(pts / (ms * 1e-3) / 1e9, 1),
+               # the reference's figure rate, N^2 / t (script/draw.py:19)
+               "gpts_n2_s": round(size * size / (ms * 1e-3) / 1e9, 1),
                "hbm_gbs": round(BYTES_PER_UPDATE[a.precision] * pts / (ms * 1e-3) / 1e9, 1)}
         rows.append(row)
         print(json.dumps(row), flush=True)

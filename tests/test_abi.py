@@ -343,3 +343,51 @@ def test_hi_lo_coefficient_option():
     p = T.Problem("box2d2r", 1024, 1024, B)
     p.set_coef_lo(True)
     assert p.coef_lo and ", 4, false>" in p.kernel_name
+
+
+def test_tuning_cache_lookup_and_apply(tmp_path, monkeypatch):
+    """The tuning cache file (no GPU needed with the 'device any' record): a
+    matching entry sets tile, wg_per_cu, fuse and fragment; tcs_problem_init
+    applies $TCS_TUNE_CACHE's single-grid entries; batched entries apply only
+    when asked for; an explicit fragment choice is kept; malformed files are
+    errors; a file for another device changes nothing."""
+    import tcstencil_amd as T
+    from tcstencil_amd import autotune as A
+    B = T.gen_coeffs("box2d2r", 1)
+    path = tmp_path / "tune.txt"
+    lines = [A.entry_line("box2d2r", "mfma", "fp16", 0, 0, 3, 16, (8, 8), 0, 8),
+             A.entry_line("box2d2r", "mfma", "fp16", 0, 0, 1, 16, (2, 4), 2, 1),
+             A.entry_line("box2d2r", "mfma", "fp16", 0, 1, 1, 32, (4, 4), 0, 4),
+             A.entry_line("box2d2r", "direct", "fp16", 0, 0, 3, 16, (2, 8), 0, 1),
+             A.entry_line("box2d2r", "mfma", "fp16", 1, 0, 3, 16, (4, 8), 0, 2)]
+    A.write_cache(str(path), lines, device="device any 0")
+    monkeypatch.delenv("TCS_TUNE_CACHE", raising=False)
+    assert T.Problem("box2d2r", 16384, 16384, B).tile == (4, 8)  # no cache named: defaults
+    p = T.Problem("box2d2r", 16384, 16384, B, tuned=(str(path), False))
+    assert (p.tile, p.wg_per_cu, p.fuse, p.frag) == ((8, 8), 0, 8, 16)
+    monkeypatch.setenv("TCS_TUNE_CACHE", str(path))
+    p = T.Problem("box2d2r", 500, 700, B)  # bucket 1 (smaller side 32 bricks)
+    assert (p.tile, p.wg_per_cu, p.fuse) == ((2, 4), 2, 1)
+    q = T.Problem("box2d2r", 500, 700, B, tuned=("", True))  # the batched entry
+    assert (q.tile, q.fuse, q.frag) == ((4, 4), 4, 32)
+    d = T.Problem("box2d2r", 16384, 16384, B, method="direct")
+    assert d.tile == (2, 8) and d.fuse == 0
+    lo = T.Problem("box2d2r", 16384, 16384, B, coef_lo=True)
+    assert (lo.tile, lo.fuse) == ((4, 8), 2)
+    f = T.Problem("box2d2r", 16384, 16384, B, frag=32)  # entry is for 16x16 fragments: not applied
+    assert f.frag == 32 and f.tile == (8, 8) and f.fuse == 0
+    t = T._lib.Tuning()
+    assert T.lib().tcs_tune_lookup(str(path).encode(), T.Problem("box2d2r", 64, 64, B).ref, 0,
+                                   ctypes.byref(t)) == T._lib.ERR_UNSUPPORTED  # bucket 0: no entry
+    monkeypatch.delenv("TCS_TUNE_CACHE")
+    other = tmp_path / "other.txt"
+    A.write_cache(str(other), lines, device="device gfx000 1")
+    assert T.Problem("box2d2r", 16384, 16384, B, tuned=(str(other), False)).tile == (4, 8)
+    bad = tmp_path / "bad.txt"
+    bad.write_text("device any 0\nentry box2d2r mfma fp16 0 0 3 16 8\n")
+    with pytest.raises(T.TcsError, match="bad entry"):
+        T.Problem("box2d2r", 16384, 16384, B, tuned=(str(bad), False))
+    bad.write_text("entry box2d2r mfma fp16 0 0 3 16 8 8 0 1\n")
+    with pytest.raises(T.TcsError, match="before the device"):
+        T.Problem("box2d2r", 16384, 16384, B, tuned=(str(bad), False))
+    assert [T.lib().tcs_tune_bucket(n, n) for n in (64, 240, 256, 1008, 1024, 4080, 4096)] == [0, 0, 1, 1, 2, 2, 3]

@@ -48,6 +48,22 @@ def test_update_rate_uses_interior_points_and_units(tmp_path):
     assert R.updates_per_s(rows[5]) == pytest.approx((1024 - 2) ** 2 / 1e-3)  # "ms" unit
 
 
+def test_reference_rate_is_n_squared_over_t(tmp_path):
+    """script/draw.py:19 plots mesh_size^2 / time * 1e6 (time in us); both rates
+    are columns of every output CSV."""
+    p = tmp_path / "sweep.csv"
+    _write_sweep(p, ROWS)
+    rows = R.load_sweep_csv(str(p))
+    assert R.reference_updates_per_s(rows[0]) == pytest.approx(1024 * 1024 / 2.0 * 1e6)
+    assert R.reference_updates_per_s(rows[5]) == pytest.approx(1024 * 1024 / 1e-3)
+    out = tmp_path / "out"
+    assert R.main([str(p), "--out-dir", str(out)]) == 0
+    with open(out / "sweep_best.csv") as f:
+        rec = next(csv.DictReader(f))
+    assert int(rec["ref_updates_per_s"]) == round(1024 * 1024 / 1.5e-6)
+    assert int(rec["updates_per_s"]) == round(1020 * 1020 / 1.5e-6)
+
+
 def test_cli_writes_reference_files_and_figures(tmp_path):
     p = tmp_path / "sweep.csv"
     _write_sweep(p, ROWS)
