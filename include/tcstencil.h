@@ -26,7 +26,12 @@ extern "C" {
 
 /* ---------------------------------------------------------------- constants */
 enum { TCS_STAR2D1R = 0, TCS_STAR2D2R = 1, TCS_BOX2D1R = 2, TCS_BOX2D2R = 3 };
-enum { TCS_METHOD_MFMA = 0, TCS_METHOD_DIRECT = 1 };
+/* MFMA: the matrix-core kernel.  DIRECT: one FP32 FMA per tap and point with
+ * FP32 taps (the drop-in for baseline/kernel_<stencil>.cu).  DIRECT_DOT2: the
+ * direct kernel on packed halves, v_dot2_f32_f16 per pair of adjacent taps
+ * (taps rounded to FP16, FP32 accumulation): the fastest plain-arithmetic form,
+ * measured beside the MFMA kernel. */
+enum { TCS_METHOD_MFMA = 0, TCS_METHOD_DIRECT = 1, TCS_METHOD_DIRECT_DOT2 = 2 };
 /* FP16: FP16 storage, FP16 coefficients, MFMA f16 with FP32 accumulation.
  * BF16: the same in BF16.
  * SPLIT: storage is a hi+lo FP16 pair (two planes) with a per-grid power-of-two

@@ -14,13 +14,13 @@ LIB_PATH = os.path.join(PKG_DIR, "libtcstencil.so")
 
 # Constants of include/tcstencil.h
 STAR2D1R, STAR2D2R, BOX2D1R, BOX2D2R = 0, 1, 2, 3
-METHOD_MFMA, METHOD_DIRECT = 0, 1
+METHOD_MFMA, METHOD_DIRECT, METHOD_DIRECT_DOT2 = 0, 1, 2
 PREC_FP16, PREC_BF16, PREC_SPLIT = 0, 1, 2
 OK, ERR_INVALID, ERR_UNSUPPORTED, ERR_HIP, ERR_ALLOC = 0, 1, 2, 3, 4
 
 STENCIL_NAMES = ("star2d1r", "star2d2r", "box2d1r", "box2d2r")
 STENCIL_ID = {name: i for i, name in enumerate(STENCIL_NAMES)}
-METHOD_ID = {"mfma": METHOD_MFMA, "direct": METHOD_DIRECT}
+METHOD_ID = {"mfma": METHOD_MFMA, "direct": METHOD_DIRECT, "direct_dot2": METHOD_DIRECT_DOT2}
 PREC_ID = {"fp16": PREC_FP16, "bf16": PREC_BF16, "split": PREC_SPLIT}
 
 

@@ -209,7 +209,7 @@ static void default_tile(const tcs_problem *p, int *th, int *tw) {
         // box2d2r 16384^2 FP16 193.8 us against 222.7 us on 4x8), else 4x8
         first = frag32(p) || has_coef_lo(kernel_prec(p)) ? 0 : 1;
     else
-        first = p->stencil == TCS_BOX2D2R ? 1 : 4;
+        first = 0;  // direct kernels: 8x8 (16384^2, all stencils, both forms; 4x8 within 1-2%)
     const long br = (p->layout.rows + 15) / 16, bc = (p->layout.cols + 15) / 16;
     int pick = last;
     for (int i = first; i <= last; ++i) {
@@ -275,12 +275,12 @@ extern "C" int tcs_problem_init(tcs_problem *p, int stencil, int method, int pre
     if (!p) return set_error(TCS_ERR_INVALID, "problem: null pointer");
     const int r = tcs_radius(stencil);
     if (r < 0) return set_error(TCS_ERR_INVALID, "problem: unknown stencil %d", stencil);
-    if (method != TCS_METHOD_MFMA && method != TCS_METHOD_DIRECT)
+    if (method != TCS_METHOD_MFMA && method != TCS_METHOD_DIRECT && method != TCS_METHOD_DIRECT_DOT2)
         return set_error(TCS_ERR_INVALID, "problem: unknown method %d", method);
     if (precision != TCS_PREC_FP16 && precision != TCS_PREC_BF16 && precision != TCS_PREC_SPLIT)
         return set_error(TCS_ERR_INVALID, "problem: unknown precision %d", precision);
-    if (method == TCS_METHOD_DIRECT && precision != TCS_PREC_FP16)
-        return set_error(TCS_ERR_UNSUPPORTED, "problem: the direct-FMA kernel runs FP16 storage only");
+    if (method != TCS_METHOD_MFMA && precision != TCS_PREC_FP16)
+        return set_error(TCS_ERR_UNSUPPORTED, "problem: the direct-FMA kernels run FP16 storage only");
     if (!B) return set_error(TCS_ERR_INVALID, "problem: null coefficients");
     if (n != 2 * r + 1) return set_error(TCS_ERR_INVALID, "problem: n = %d, stencil needs n = %d", n, 2 * r + 1);
     if (rows < 1 || cols < 1) return set_error(TCS_ERR_INVALID, "problem: grid %dx%d is empty", rows, cols);

@@ -1327,7 +1327,44 @@ __host__ __device__ constexpr bool tap(int dz, int dj) {
     return true;
 }
 
-template <int TH, int TW, int STENCIL, bool PERSIST>
+// Contiguous tap run [first, last] of stencil row dz (every row of the four
+// stencils is one run): star rows off the centre hold the middle tap only.
+template <int STENCIL>
+__host__ __device__ constexpr int tap_first(int dz) {
+    for (int dj = 0; dj < 5; ++dj)
+        if (tap<STENCIL>(dz, dj)) return dj;
+    return 5;
+}
+template <int STENCIL>
+__host__ __device__ constexpr int tap_last(int dz) {
+    for (int dj = 4; dj >= 0; --dj)
+        if (tap<STENCIL>(dz, dj)) return dj;
+    return -1;
+}
+
+typedef _Float16 half2_t __attribute__((ext_vector_type(2)));
+
+// Direct kernels store their results through an LDS image of the tile as whole
+// bricks (16-byte pieces, two bricks per wave instruction) rather than as
+// 8-byte pieces of 8 bricks per instruction, except box2d2r, which is bound by
+// its arithmetic and runs faster without the two extra barriers.  Measured on
+// 16384^2, best tile each, staged vs not (us): star2d1r 179 vs 228, box2d1r
+// 198 vs 237, star2d2r 198 vs 237, box2d2r 295 vs 284.
+template <int STENCIL>
+constexpr bool kDirectStaged = STENCIL != TCS_BOX2D2R;
+
+__device__ __forceinline__ float dot2(uint32_t x, uint32_t c, float acc) {
+    return __builtin_amdgcn_fdot2(__builtin_bit_cast(half2_t, x), __builtin_bit_cast(half2_t, c), acc, false);
+}
+
+// The direct kernel (FP32 taps, one FMA per tap and point) and, with DOT2, its
+// packed-half form: taps rounded to FP16 (as the MFMA kernel's), each pair of
+// adjacent taps of a row one v_dot2_f32_f16 (two FP16 products added into the
+// FP32 accumulator), the odd tap of a run paired with a zero half.  Per point of
+// box2d2r that is 15 dot products instead of 25 FMAs and 2 conversions.  A
+// non-finite input reaches the outputs whose pairs hold it (times a zero tap),
+// a superset of gold's set as for the MFMA kernel.
+template <int TH, int TW, int STENCIL, bool PERSIST, bool DOT2 = false>
 __global__ __launch_bounds__(kWG) void tcs_direct_sweep(SweepArgs a) {
     using T = Tile<TH, TW, 1>;
     constexpr int CG = 4 * TW;       // 4-column groups across the tile
@@ -1347,6 +1384,20 @@ __global__ __launch_bounds__(kWG) void tcs_direct_sweep(SweepArgs a) {
 
     const int cg = tid % CG, rg = tid / CG;
     const int r0 = rg * MR;  // first output row of this strip (tile coordinates)
+
+    // DOT2: FP16 tap pairs (c[dz][j], c[dz][j+1]) and single taps (c[dz][j], 0),
+    // rounded to nearest like the MFMA kernel's taps.
+    uint32_t cpair[DOT2 ? 25 : 1], csing[DOT2 ? 25 : 1];
+    if constexpr (DOT2) {
+#pragma unroll
+        for (int k = 0; k < 25; ++k) {
+            const uint16_t c0 = f16_bits(a.cf[k]);
+            const uint16_t c1 = (k % 5) < 4 ? f16_bits(a.cf[k + 1]) : (uint16_t)0;
+            // uniform across the wave: held in scalar registers
+            cpair[k] = __builtin_amdgcn_readfirstlane(pack2(c0, c1));
+            csing[k] = __builtin_amdgcn_readfirstlane((uint32_t)c0);
+        }
+    }
 
     for (;;) {
     write_tile<T>(R, lds, tid);
@@ -1388,17 +1439,42 @@ __global__ __launch_bounds__(kWG) void tcs_direct_sweep(SweepArgs a) {
         for (int o = 0; o < MR; ++o) {
             const int dz = ir - o;
             if (dz < 0 || dz > 4) continue;
+            if constexpr (DOT2) {
+                // Output column v takes x[v + dj], dj in the row's run: pairs
+                // (x[k], x[k+1]) are whole dwords for even k and an alignbit of
+                // two dwords for odd k; a last single x[k] is a dword with its
+                // other half zeroed.
+                const int ta = tap_first<STENCIL>(dz), tb = tap_last<STENCIL>(dz);
 #pragma unroll
-            for (int dj = 0; dj < 5; ++dj) {
-                if (!tap<STENCIL>(dz, dj)) continue;
-                const float c = a.cf[dz * 5 + dj];
+                for (int v = 0; v < 4; ++v) {
+                    float sacc = acc[o][v];
 #pragma unroll
-                for (int v = 0; v < 4; ++v) acc[o][v] = __builtin_fmaf(x[v + dj], c, acc[o][v]);
+                    for (int k = v + ta; k <= v + tb; k += 2) {
+                        if (k + 1 <= v + tb) {
+                            const uint32_t xp = (k & 1) ? __builtin_amdgcn_alignbit(w4[(k + 1) >> 1], w4[k >> 1], 16)
+                                                        : w4[k >> 1];
+                            sacc = dot2(xp, cpair[dz * 5 + k - v], sacc);
+                        } else {
+                            const uint32_t xs = (k & 1) ? (w4[k >> 1] >> 16) : (w4[k >> 1] & 0xffffu);
+                            sacc = dot2(xs, csing[dz * 5 + k - v], sacc);
+                        }
+                    }
+                    acc[o][v] = sacc;
+                }
+            } else {
+#pragma unroll
+                for (int dj = 0; dj < 5; ++dj) {
+                    if (!tap<STENCIL>(dz, dj)) continue;
+                    const float c = a.cf[dz * 5 + dj];
+#pragma unroll
+                    for (int v = 0; v < 4; ++v) acc[o][v] = __builtin_fmaf(x[v + dj], c, acc[o][v]);
+                }
             }
         }
     }
 
     const bool edge = !tile_is_interior(a, cgr0, cgc0, 16 * TH, 16 * TW);
+    u32x2 res[MR];
 #pragma unroll
     for (int o = 0; o < MR; ++o) {
         const int tr = r0 + o;                  // tile row
@@ -1413,8 +1489,31 @@ __global__ __launch_bounds__(kWG) void tcs_direct_sweep(SweepArgs a) {
             for (int v = 0; v < 4; ++v)
                 if (!point_is_interior(a, gr, gcb + v)) h[v] = lds[cidx + v];
         }
-        u32x2 w = {pack2(h[0], h[1]), pack2(h[2], h[3])};
-        store8<false>(a.out + stor_index(gr, gcb, a.sbc), w);
+        res[o] = u32x2{pack2(h[0], h[1]), pack2(h[2], h[3])};
+        if constexpr (!kDirectStaged<STENCIL>) store8<false>(a.out + stor_index(gr, gcb, a.sbc), res[o]);
+    }
+    if constexpr (kDirectStaged<STENCIL>) {
+    // The results go out as whole bricks, as the MFMA kernel's do: a row-major
+    // image of the tile in LDS (over the input image, which every thread has
+    // finished reading), then 16-byte pieces with consecutive lanes along a
+    // brick, so each store instruction of a wave writes two whole bricks.
+    constexpr int OP = 16 * TW + 8;  // output image pitch (elements)
+    static_assert(16 * TH * OP <= T::PLANE, "output image must fit the input image");
+    __syncthreads();
+#pragma unroll
+    for (int o = 0; o < MR; ++o) *reinterpret_cast<u32x2 *>(lds + (r0 + o) * OP + 4 * cg) = res[o];
+    __syncthreads();
+    constexpr int NCH = TH * TW * 32;  // 16-byte pieces of the tile's bricks
+    static_assert(NCH % kWG == 0, "whole passes of the workgroup");
+#pragma unroll
+    for (int i = 0; i < NCH / kWG; ++i) {
+        const int c = tid + i * kWG;
+        const int b = c >> 5, w = c & 31;
+        const int br = b / TW, bc = b - br * TW;
+        const int tr = 16 * br + (w >> 1), tc = 16 * bc + 8 * (w & 1);
+        const u32x4 v = *reinterpret_cast<const u32x4 *>(lds + tr * OP + tc);
+        store16_nt(a.out + stor_index(cgr0 + tr, cgc0 + tc, a.sbc), v);
+    }
     }
     // One-tile-per-workgroup launches cover every tile with the grid itself.
     if (!PERSIST || next >= sc.end) break;
@@ -1430,8 +1529,9 @@ __global__ __launch_bounds__(kWG) void tcs_direct_sweep(SweepArgs a) {
 //   1 FP16, 2 BF16, 3 SPLIT (16x16 fragments: single sweep, fused, edge, batch)
 //   4 FP16 and 5 BF16 with hi+lo coefficients (the same kernel kinds)
 //   6 32x32 fragments (single sweep and fused) and the direct-FMA kernel
+//   7 the packed-half direct kernel
 #ifndef TCS_PART
-#error "compile with -DTCS_PART=0..6"
+#error "compile with -DTCS_PART=0..7"
 #endif
 
 typedef void (*sweep_fn)(SweepArgs);
@@ -1498,6 +1598,11 @@ struct PartTables {
 #define TCS_DIRECT_TILES(S, PS) \
     TCS_DIRECT(2, 4, S, PS), TCS_DIRECT(4, 4, S, PS), TCS_DIRECT(2, 8, S, PS), TCS_DIRECT(4, 8, S, PS), \
     TCS_DIRECT(8, 8, S, PS)
+#define TCS_DOT2(TH, TW, S, PS) \
+    {TCS_METHOD_DIRECT_DOT2, TCS_PREC_FP16, -1, S, TH, TW, PS, 0, tcs_direct_sweep<TH, TW, S, PS, true>, \
+     "tcs_direct_sweep<" #TH ", " #TW ", " #S ", " #PS ", true>"}
+#define TCS_DOT2_TILES(S, PS) \
+    TCS_DOT2(2, 4, S, PS), TCS_DOT2(4, 4, S, PS), TCS_DOT2(2, 8, S, PS), TCS_DOT2(4, 8, S, PS), TCS_DOT2(8, 8, S, PS)
 #define TCS_MFMA32(TH, TW, ZLO, ZHI, P, R) \
     {TCS_METHOD_MFMA, P, R, -1, TH, TW, false, 32, tcs_mfma32_sweep<TH, TW, ZLO, ZHI, P>, \
      "tcs_mfma32_sweep<" #TH ", " #TW ", " #ZLO ", " #ZHI ", " #P ">"}
@@ -1580,10 +1685,17 @@ static const FusedEntry kFusedOther[] = {
 };
 extern const PartTables kPartOther = {kSweepOther, TCS_N(kSweepOther), kFusedOther, TCS_N(kFusedOther),
                                       nullptr, 0, nullptr, 0};
+#elif TCS_PART == 7
+// The packed-half direct kernels.
+static const KernelEntry kSweepDot2[] = {
+    TCS_DOT2_TILES(0, false), TCS_DOT2_TILES(1, false), TCS_DOT2_TILES(2, false), TCS_DOT2_TILES(3, false),
+    TCS_DOT2_TILES(0, true), TCS_DOT2_TILES(1, true), TCS_DOT2_TILES(2, true), TCS_DOT2_TILES(3, true),
+};
+extern const PartTables kPartDot2 = {kSweepDot2, TCS_N(kSweepDot2), nullptr, 0, nullptr, 0, nullptr, 0};
 #elif TCS_PART == 0
-extern const PartTables kPartFp16, kPartBf16, kPartSplit, kPartFp16Lo, kPartBf16Lo, kPartOther;
+extern const PartTables kPartFp16, kPartBf16, kPartSplit, kPartFp16Lo, kPartBf16Lo, kPartOther, kPartDot2;
 static const PartTables *const kParts[] = {&kPartFp16, &kPartBf16, &kPartSplit, &kPartFp16Lo, &kPartBf16Lo,
-                                           &kPartOther};
+                                           &kPartOther, &kPartDot2};
 
 static int frag_of(const tcs_problem *p) { return p->frag == 32 ? 32 : 16; }
 

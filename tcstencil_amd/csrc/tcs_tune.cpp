@@ -26,7 +26,7 @@ using namespace tcs;
 namespace {
 
 const char *const kStencilNames[] = {"star2d1r", "star2d2r", "box2d1r", "box2d2r"};
-const char *const kMethodNames[] = {"mfma", "direct"};
+const char *const kMethodNames[] = {"mfma", "direct", "direct_dot2"};
 const char *const kPrecNames[] = {"fp16", "bf16", "split"};
 
 int index_of(const char *s, const char *const *names, int n) {
@@ -112,7 +112,7 @@ extern "C" int tcs_tune_lookup(const char *path, const tcs_problem *p, int batch
             return set_error(TCS_ERR_INVALID, "tune cache %s:%d: entry before the device record", path, lineno);
         }
         if (!device_ok || found) continue;
-        if (index_of(st, kStencilNames, 4) == p->stencil && index_of(me, kMethodNames, 2) == p->method &&
+        if (index_of(st, kStencilNames, 4) == p->stencil && index_of(me, kMethodNames, 3) == p->method &&
             index_of(pr, kPrecNames, 3) == p->precision && clo == lo && ba == (batched ? 1 : 0) && bu == bucket) {
             *out = t;
             found = true;  // the first matching entry wins; keep checking the rest of the file's syntax

@@ -714,6 +714,12 @@ def direct_sweep(inp, L: Layout, TH, TW, coeff5, stencil, rs, per_xcd=0):
                     if taps[dz, dj]:
                         acc[:, o, :] += x[:, dj:dj + 4] * np.float32(coeff5[dz, dj])
         edge = not (gr0 >= rs and gr0 + 16 * TH <= L.rows - rs and gc0 >= rs and gc0 + 16 * TW <= L.cols - rs)
+        # results -> row-major output image over the input image (pitch OP);
+        # box2d2r stores 8-byte pieces straight from registers
+        staged = stencil != "box2d2r"
+        OP = 16 * TW + 8
+        assert 16 * TH * OP <= G.PLANE
+        img = np.full(G.PLANE, np.nan)
         for o in range(MR):
             tr = r0 + o
             gr = gr0 + tr
@@ -724,10 +730,36 @@ def direct_sweep(inp, L: Layout, TH, TW, coeff5, stencil, rs, per_xcd=0):
                 for v in range(4):
                     keep = ~_interior(gr, gcb + v, L, rs)
                     h[keep, v] = lds[cidx[keep] + v]
-            s = stor_index(gr, gcb, L.sbc)
-            _check(s, L.plane - 3, 4, "direct store (8 B)")
+            if not staged:
+                s = stor_index(gr, gcb, L.sbc)
+                _check(s, L.plane - 3, 4, "direct store (8 B)")
+                for v in range(4):
+                    assert not written[s + v].any()
+                    out[s + v] = h[:, v]
+                    written[s + v] = True
+                continue
+            d = tr * OP + 4 * cg
+            _check(d, G.PLANE - 3, 4, "direct output image write (8 B)")
             for v in range(4):
-                assert not written[s + v].any()
-                out[s + v] = h[:, v]
-                written[s + v] = True
+                img[d + v] = h[:, v]
+        if not staged:
+            continue
+        # whole bricks out: 16-byte pieces, consecutive threads along a brick
+        nch = TH * TW * 32
+        assert nch % WG == 0
+        for i in range(nch // WG):
+            c = tid + i * WG
+            b, w = c >> 5, c & 31
+            br, bc = b // TW, b % TW
+            tr, tc = 16 * br + (w >> 1), 16 * bc + 8 * (w & 1)
+            src = tr * OP + tc
+            _check(src, G.PLANE - 7, 8, "direct output image read (16 B)")
+            s = stor_index(gr0 + tr, gc0 + tc, L.sbc)
+            _check(s, L.plane - 7, 8, "direct store (16 B)")
+            for e in range(8):
+                v = img[src + e]
+                assert not np.isnan(v).any(), "output image read before written"
+                assert not written[s + e].any()
+                out[s + e] = v
+                written[s + e] = True
     return out, written

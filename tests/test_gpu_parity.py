@@ -81,7 +81,7 @@ def round_bf16(torch, x):
 # ------------------------------------------------------------ bitwise pins
 @pytest.mark.parametrize("stencil", STENCILS)
 @pytest.mark.parametrize("tile", MFMA_TILES, ids=lambda t: "t%dx%d" % t)
-@pytest.mark.parametrize("method", ["mfma", "direct"])
+@pytest.mark.parametrize("method", ["mfma", "direct", "direct_dot2"])
 def test_dyadic_exact_fp16(T, torch, oracle, stencil, tile, method):
     for rows, cols in ((70, 150), (33, 17), (130, 260)):
         say("case", stencil, method, tile, rows, cols)
@@ -156,7 +156,7 @@ def check_against_models(oracle, stencil, A, B32, got, precision, sweeps, coef_l
 
 
 @pytest.mark.parametrize("stencil", STENCILS)
-@pytest.mark.parametrize("method", ["mfma", "direct"])
+@pytest.mark.parametrize("method", ["mfma", "direct", "direct_dot2"])
 def test_random_fp16_within_bound(T, torch, oracle, stencil, method):
     say("case random fp16", stencil, method)
     A, B = random_problem(T, torch, stencil, 300, 517, seed=1)
@@ -167,7 +167,7 @@ def test_random_fp16_within_bound(T, torch, oracle, stencil, method):
     say("  ", res)
     # |err| <= u16*(sum|c|*max|x|) [coefficient rounding, MFMA only]
     #        + u16*|out| [output rounding] + FP32 accumulation (< 1e-6); u16 = 2^-11
-    bound = (2.0 ** -11) * (2.0 if method == "mfma" else 1.0) * np.abs(A).max() + 1e-6
+    bound = (2.0 ** -11) * (1.0 if method == "direct" else 2.0) * np.abs(A).max() + 1e-6
     assert res["max_abs"] <= bound
     assert res["rel_l2"] <= 2e-3
     check_against_models(oracle, stencil, A, B32, got, "fp16", 1, method=method, what=method)
@@ -239,7 +239,7 @@ def test_tiny_and_degenerate_grids(T, torch, oracle, stencil):
     for rows, cols in ((1, 1), (2, 3), (3, 3), (4, 5), (5, 5), (6, 40), (17, 1), (16, 16)):
         say("case tiny", stencil, rows, cols)
         A, B = dyadic(stencil, rows, cols, seed=rows * 31 + cols)
-        for method in ("mfma", "direct"):
+        for method in ("mfma", "direct", "direct_dot2"):
             got = gpu_sweeps(T, torch, stencil, A, B, method=method)
             want = oracle.sweep_f64(stencil, A, B).astype(np.float16).astype(np.float64)
             assert np.array_equal(got, want)
@@ -259,11 +259,12 @@ def test_reference_driver_inputs(T, torch, oracle):
 
 def test_nan_propagation(T, torch, oracle):
     """A NaN input poisons at least every output gold poisons (the direct kernel
-    exactly those; the MFMA form also the brick rows whose window holds it)."""
+    exactly those; the MFMA form also the brick rows whose window holds it, the
+    packed-half form the outputs whose tap pairs hold it)."""
     A, B = dyadic("box2d2r", 64, 64, seed=5)
     A[30, 33] = np.nan
     ref = oracle.sweep_f64("box2d2r", A, B)
-    for method in ("direct", "mfma"):
+    for method in ("direct", "mfma", "direct_dot2"):
         got = gpu_sweeps(T, torch, "box2d2r", A, B, method=method)
         assert np.all(np.isnan(got[np.isnan(ref)]))
         if method == "direct":
@@ -364,7 +365,7 @@ def test_host_sweep_vs_gold(T, torch, oracle, stencil):
     B = T.gen_coeffs(stencil, 12)
     gold = oracle.sweep_f32(stencil, A, B)
     r = RADIUS[stencil]
-    for method in ("mfma", "direct"):
+    for method in ("mfma", "direct", "direct_dot2"):
         C, ms = T.host_sweep(stencil, A, B, method=method)
         # input rounding to FP16 + coefficient rounding (MFMA) + output rounding
         assert np.abs(C - gold)[r:-r, r:-r].max() <= 3 * 2.0 ** -11 + 1e-6
