@@ -4,10 +4,12 @@
 One step = one Jacobi sweep of the whole grid (ping-pong between two brick
 buffers resident in HBM).  `value` is the MFMA kernel's GStencil/s =
 (N-4)^2 * steps / t / 1e9 with t measured by HIP events on the launch stream
-around exactly `steps` sweeps.  The same line reports the direct-FMA kernel on
-the same input, the accuracy of both against the FP64 oracle (after 1 and 10
-sweeps), the HBM roofline of the MFMA kernel and the reference's CPU gold timed
-on this host.
+around exactly `steps` sweeps.  The same line reports, on the same input and
+timed the same way: the MFMA kernel with hi+lo coefficients, the direct-FMA
+kernel with FP32 taps (the counterpart of the reference's baseline kernel), its
+packed-half form (v_dot2_f32_f16, FP16 taps), and 8 sweeps fused per launch;
+the accuracy of each against the FP64 oracle (after 1 and 10 sweeps), the HBM
+roofline of the MFMA kernel and the reference's CPU gold timed on this host.
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu]
 """
@@ -62,13 +64,22 @@ def accuracy(torch, T, p, x64, refs):
     return out
 
 
-def read_traffic():
-    """HBM bytes per launch from the committed rocprofv3 PMC summary, if any."""
-    path = os.path.join(ROOT, "profiles", "box2d2r_16384_fp16_mfma_pmc.md")
+TRAFFIC_PROFILE = os.path.join("profiles", "box2d2r_16384_fp16_mfma_pmc.md")
+
+
+def read_traffic(kernel):
+    """HBM bytes per launch of `kernel` from the committed rocprofv3 PMC summary
+    (FETCH_SIZE + WRITE_SIZE passes, calibrated on a copy of known size): the
+    bytes, or None when the summary is missing or was taken on another kernel."""
+    path = os.path.join(ROOT, TRAFFIC_PROFILE)
     if not os.path.exists(path):
         return None
-    m = re.search(r"traffic_bytes_per_launch:\s*([0-9.eE+]+)", open(path).read())
-    return float(m.group(1)) if m else None
+    text = open(path).read()
+    k = re.search(r"traffic_kernel:\s*(.+)", text)
+    m = re.search(r"traffic_bytes_per_launch:\s*([0-9.eE+]+)", text)
+    if not (k and m) or k.group(1).strip() != kernel:
+        return None
+    return float(m.group(1))
 
 
 def cpu_baseline(np):
@@ -115,6 +126,14 @@ def cpu_baseline_threads(np, threads):
             "kind": "port",
             "sample": f"{sweeps} box2d2r sweeps of {N}x{N} FP32 on {threads} row bands, {dt:.1f} s "
                       f"(oracle/_build/liboracle.so), host {os.cpu_count()} logical CPUs, {cpu_model()}"}
+
+
+def form_record(r, what):
+    """One measured form of the sweep for the JSON line."""
+    return {"value": round(r["gstencil"], 2), "unit": "GStencil/s", "ms_per_step": round(r["ms"], 5),
+            "kernel": r["kernel"], "tile_bricks": list(r["tile"]), "what": what,
+            "rel_l2_1sweep": r["acc"][1]["rel_l2"], "max_abs_1sweep": r["acc"][1]["max_abs"],
+            "rel_l2_10sweeps": r["acc"][10]["rel_l2"], "max_abs_10sweeps": r["acc"][10]["max_abs"]}
 
 
 def cpu_model():
@@ -169,17 +188,22 @@ def main():
     torch.cuda.synchronize()
 
     results = {}
-    for method in ("mfma", "direct"):
-        p = T.Problem(STENCIL, N, N, B, method=method, precision="fp16")
-        acc = accuracy(torch, T, p, x64, refs)
+    forms = (("mfma", "mfma", {}), ("mfma_hi_lo", "mfma", {"coef_lo": True}), ("direct", "direct", {}),
+             ("direct_dot2", "direct_dot2", {}), ("mfma_fused8", "mfma", {"fuse": 8}))
+    for name, method, kw in forms:
+        p = T.Problem(STENCIL, N, N, B, method=method, precision="fp16", **kw)
+        # fused sweeps are bitwise the unfused ones (tests/test_gpu_parity.py)
+        acc = results["mfma"]["acc"] if name == "mfma_fused8" else accuracy(torch, T, p, x64, refs)
         b0, b1 = p.alloc(), p.alloc()
         p.clear(b1)
         p.pack(x64, b0)
-        ms = time_sweeps(torch, p, b0, b1, args.steps, args.warmup)
+        steps = args.steps - args.steps % 8 if name == "mfma_fused8" else args.steps
+        ms = time_sweeps(torch, p, b0, b1, steps, args.warmup)
         gst = p.interior_points / (ms * 1e-3) / 1e9
-        results[method] = {"ms": ms, "gstencil": gst, "kernel": p.kernel_name, "tile": p.tile,
-                           "mfma_per_brick": p.mfma_per_brick, "acc": acc}
-        log(f"bench: {method:6s} {p.kernel_name}: {ms * 1e3:.1f} us/sweep, {gst:.1f} GStencil/s, "
+        kernel = p.fused_kernel_name if name == "mfma_fused8" else p.kernel_name
+        results[name] = {"ms": ms, "gstencil": gst, "kernel": kernel, "tile": p.tile,
+                         "mfma_per_brick": p.mfma_per_brick, "acc": acc}
+        log(f"bench: {name:11s} {kernel}: {ms * 1e3:.1f} us/sweep, {gst:.1f} GStencil/s, "
             f"rel-L2 1 sweep {acc[1]['rel_l2']:.3e}, 10 sweeps {acc[10]['rel_l2']:.3e}")
         del b0, b1
     del x64, refs
@@ -189,7 +213,7 @@ def main():
     achieved = BYTES_PER_UPDATE * pts / (m["ms"] * 1e-3) / 1e9
     bricks = (N // 16) ** 2
     mfma_tflops = m["mfma_per_brick"] * bricks * (16 * 16 * 32 * 2) / (m["ms"] * 1e-3) / 1e12
-    traffic = read_traffic()
+    traffic = read_traffic(m["kernel"])
 
     cpu = cpu_mt = None
     if not args.no_cpu:
@@ -218,12 +242,18 @@ def main():
                    "stencil": STENCIL, "grid": [N, N], "precision": "fp16", "method": "mfma",
                    "kernel": m["kernel"], "tile_bricks": list(m["tile"])},
         "hbm_frac": round(achieved / HBM_PEAK_GBS, 4),
-        "direct_fma": {"value": round(results["direct"]["gstencil"], 2), "unit": "GStencil/s",
-                       "ms_per_step": round(results["direct"]["ms"], 5), "kernel": results["direct"]["kernel"],
-                       "rel_l2_1sweep": results["direct"]["acc"][1]["rel_l2"],
-                       "max_abs_1sweep": results["direct"]["acc"][1]["max_abs"],
-                       "rel_l2_10sweeps": results["direct"]["acc"][10]["rel_l2"]},
+        "direct_fma": form_record(results["direct"], "FP32 taps, one FMA per tap (baseline/kernel_* counterpart)"),
+        "direct_dot2": form_record(results["direct_dot2"], "FP16 taps, v_dot2_f32_f16 per tap pair, FP32 accumulate"),
+        "mfma_hi_lo": form_record(results["mfma_hi_lo"], "MFMA with hi+lo FP16 taps (FP32-tap accuracy)"),
+        "mfma_fused8": form_record(results["mfma_fused8"], "MFMA, 8 sweeps fused per launch in LDS "
+                                                            "(bitwise the unfused sweeps; 0.5 B/update of HBM)"),
         "mfma_vs_direct": round(m["gstencil"] / results["direct"]["gstencil"], 3),
+        # the same FP16 taps, and the fastest direct form measured
+        "mfma_vs_direct_dot2": round(m["gstencil"] / results["direct_dot2"]["gstencil"], 3),
+        "mfma_vs_best_direct": round(m["gstencil"] / max(results["direct"]["gstencil"],
+                                                          results["direct_dot2"]["gstencil"]), 3),
+        # equal accuracy class: FP32-exact taps on both sides
+        "mfma_hi_lo_vs_direct": round(results["mfma_hi_lo"]["gstencil"] / results["direct"]["gstencil"], 3),
         "accuracy_vs_fp64": {"rel_l2_1sweep": m["acc"][1]["rel_l2"], "max_abs_1sweep": m["acc"][1]["max_abs"],
                              "rel_l2_10sweeps": m["acc"][10]["rel_l2"],
                              "max_abs_10sweeps": m["acc"][10]["max_abs"]},
@@ -231,7 +261,8 @@ def main():
                       "frac": round(mfma_tflops / FP16_DENSE_PEAK_TFLOPS, 4),
                       "useful_tflops": round(USEFUL_FLOPS_PER_POINT * pts / (m["ms"] * 1e-3) / 1e12, 2)},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic},
+                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
+                     "traffic_source": TRAFFIC_PROFILE if traffic is not None else None},
         "cpu_baseline": cpu,
         "cpu_baseline_threads": cpu_mt,
         "gpu_clocks": "default power management, not locked (the reference locks clocks, README.md:7)",
