@@ -769,6 +769,109 @@ void tcs_mfma_edge(SweepArgs a) {
 template <int TH, int TW, int NPL, bool STRIP = false>
 using FusedGeom = Geom<TH + 2, TW + 2, NPL, STRIP>;
 
+// One sweep of the fused image but the last: every brick is computed into
+// registers, then, after a barrier, written back in place.  Work per wave: a
+// TH/2 x BW/2 block of the core band (image brick rows 1..TH, every column,
+// the left and right ring bricks included), then its share of the ring rows
+// (image brick rows 0 and TH+1):
+//   !PAIRED  whole bricks: waves 0, 1 the top row, waves 2, 3 the bottom row,
+//            BW/2 columns each;
+//   PAIRED   (the sweep's results are needed only within 8 points of the core)
+//            one product per ring column pair: lanes 0..7 compute rows 8..15
+//            of the top brick, lanes 8..15 rows 0..7 of the bottom brick (a
+//            lane's operand rows are its own address: lane row m + 8, plus
+//            16 TH rows from lane 8 on); the coefficient operand and the K
+//            slots are those of a whole brick, so every computed value has the
+//            bits of the unfused sweep.  The other halves keep the previous
+//            sweep's values, finite and farther than 8 points from the core.
+//            A column half's BW/2 pairs go ceil(BW/4) to the first wave of the
+//            half and the rest to the second.
+template <class G, int TH, int ZLO, int ZHI, int PREC, bool PAIRED, bool LOOKAHEAD>
+__device__ __forceinline__ void fused_sweep(const SweepArgs &a, uint16_t *lds, const Coef<ZLO, ZHI, PREC> &C, int mo,
+                                            int g, int wave, bool edge, int er0, int ec0) {
+    constexpr int NPL = G::NPL;
+    constexpr int HB = G::BW / 2;        // image columns per column half
+    constexpr int QB = (TH / 2) * HB;    // core-band bricks per wave
+    constexpr int HP = (HB + 1) / 2;     // pairs of the first wave of a column half
+    constexpr int QT = QB + (PAIRED ? HP : HB);
+    static_assert(TH % 2 == 0 && G::BW % 2 == 0, "the image must split 2 x 2 over the waves");
+    using O = Ops<G, ZLO, ZHI, PREC>;
+    const int mp = mo + 8 + (mo >= 8 ? 16 * TH : 0);  // a paired unit's lane row, from image brick row 0
+    const typename O::Offsets offs(mo, g);
+    // LDS offsets are linear in the row (G::at), so a paired unit's operands
+    // are the same lane offsets from a lane-dependent base.
+    const uint16_t *ldsp = lds + G::at(mp - mo, 0);
+    // Unit q of this wave: its image brick, whether it is a ring pair, and
+    // whether it exists (the second wave of a column half has one pair fewer
+    // when BW/2 is odd).  Wave-uniform.
+    auto unit = [&](int q, int &br, int &bc, bool &pair, bool &valid) {
+        pair = false;
+        valid = true;
+        if (q < QB) {
+            br = 1 + (wave >> 1) * (TH / 2) + q / HB;
+            bc = (wave & 1) * HB + q % HB;
+        } else if constexpr (!PAIRED) {
+            br = (wave >> 1) ? TH + 1 : 0;
+            bc = (wave & 1) * HB + (q - QB);
+        } else {
+            const int j = q - QB;
+            br = 0;
+            bc = (wave & 1) * HB + (wave >> 1) * HP + j;
+            pair = true;
+            valid = (wave >> 1) == 0 || j < HB - HP;
+        }
+    };
+    uint32_t *l32 = reinterpret_cast<uint32_t *>(lds);
+    u32x2 res[QT][NPL];
+    O cur, nxt;
+    if constexpr (LOOKAHEAD) {
+        int br, bc;
+        bool pair, valid;
+        unit(0, br, bc, pair, valid);
+        cur.load(lds, br, bc, offs);
+    }
+#pragma unroll
+    for (int q = 0; q < QT; ++q) {
+        int br, bc;
+        bool pair, valid;
+        unit(q, br, bc, pair, valid);
+        if (!valid) continue;
+        if constexpr (LOOKAHEAD) {
+            if (q + 1 < QT) {
+                int nbr, nbc;
+                bool npair, nvalid;
+                unit(q + 1, nbr, nbc, npair, nvalid);
+                if (nvalid) nxt.load(npair ? ldsp : lds, nbr, nbc, offs);
+            }
+        } else {
+            cur.load(pair ? ldsp : lds, br, bc, offs);
+        }
+        __builtin_amdgcn_sched_barrier(0);
+        const int row = pair ? mp : mo;
+        mfma_brick<G, ZLO, ZHI, PREC>(a, lds, C, cur, br, bc, row, g, edge, er0 + 16 * br + row, ec0 + 16 * bc + 4 * g,
+                                      res[q]);
+        if constexpr (LOOKAHEAD) cur = nxt;
+    }
+    __syncthreads();  // every wave has read this sweep's input
+#pragma unroll
+    for (int q = 0; q < QT; ++q) {
+        int br, bc;
+        bool pair, valid;
+        unit(q, br, bc, pair, valid);
+        if (!valid) continue;
+        const int row = pair ? mp : mo;
+        const int db = G::at(16 * br, 16 * bc) >> 1;
+        const int d0 = db + (G::at(row + 2, 4 * g + 2) >> 1);
+        const int d1 = db + (G::at(row + 2, 4 * g + 4) >> 1);
+#pragma unroll
+        for (int p = 0; p < NPL; ++p) {
+            l32[p * (G::PLANE / 2) + d0] = res[q][p][0];
+            l32[p * (G::PLANE / 2) + d1] = res[q][p][1];
+        }
+    }
+    __syncthreads();
+}
+
 template <int TH, int TW, int ZLO, int ZHI, int PREC>
 __device__ __forceinline__ void fused_tile(const SweepArgs &a, const CoefWindows &cw, uint16_t *lds, int gr0,
                                            int gc0, int nsweeps) {
@@ -779,7 +882,6 @@ __device__ __forceinline__ void fused_tile(const SweepArgs &a, const CoefWindows
     // slower on 4x4 tiles, and SPLIT has no registers left for it.
     constexpr bool LOOKAHEAD = base_prec(PREC) != TCS_PREC_SPLIT && TH * TW >= 32;
     using G = FusedGeom<TH, TW, NPL, true>;
-    constexpr int QA = G::BH * G::BW / 4;  // bricks per wave, whole image
     constexpr int QC = TH * TW / 4;        // bricks per wave, core only
     static_assert(G::BH * G::BW % 4 == 0 && TH * TW % 4 == 0, "bricks must split over 4 waves");
     const int tid = threadIdx.x;
@@ -802,52 +904,19 @@ __device__ __forceinline__ void fused_tile(const SweepArgs &a, const CoefWindows
         // The lane row, made opaque every sweep: all lane-dependent address
         // parts below derive from it, so the compiler cannot hoist per-brick
         // addresses or edge tests out of the sweep loop (registers per brick
-        // would cost occupancy or spill).  With brick_of's compile-time brick
-        // offsets, each address is then one register plus an immediate.
-        int mo = m;
+        // would cost occupancy or spill).  With compile-time brick offsets,
+        // each address is then one register plus an immediate.
+        int mo = m, wv = __builtin_amdgcn_readfirstlane(wave);
         asm volatile("" : "+v"(mo));
-        const typename Ops<G, ZLO, ZHI, PREC>::Offsets offs(mo, g);
-        uint32_t *l32 = reinterpret_cast<uint32_t *>(lds);
-        u32x2 res[QA][NPL];
-        Ops<G, ZLO, ZHI, PREC> cur, nxt;
-        if constexpr (LOOKAHEAD) {
-            int br, bc;
-            block_brick<G::BH, G::BW, 0>(wave, 0, br, bc);
-            cur.load(lds, br, bc, offs);
-        }
-#pragma unroll
-        for (int q = 0; q < QA; ++q) {
-            int br, bc;
-            block_brick<G::BH, G::BW, 0>(wave, q, br, bc);
-            if constexpr (LOOKAHEAD) {
-                if (q + 1 < QA) {
-                    int nbr, nbc;
-                    block_brick<G::BH, G::BW, 0>(wave, q + 1, nbr, nbc);
-                    nxt.load(lds, nbr, nbc, offs);
-                }
-            } else {
-                cur.load(lds, br, bc, offs);
-            }
-            __builtin_amdgcn_sched_barrier(0);
-            mfma_brick<G, ZLO, ZHI, PREC>(a, lds, C, cur, br, bc, mo, g, edge, er0 + 16 * br + mo,
-                                          ec0 + 16 * bc + 4 * g, res[q]);
-            if constexpr (LOOKAHEAD) cur = nxt;
-        }
-        __syncthreads();  // every wave has read this sweep's input
-#pragma unroll
-        for (int q = 0; q < QA; ++q) {
-            int br, bc;
-            block_brick<G::BH, G::BW, 0>(wave, q, br, bc);
-            const int db = G::at(16 * br, 16 * bc) >> 1;
-            const int d0 = db + (G::at(mo + 2, 4 * g + 2) >> 1);
-            const int d1 = db + (G::at(mo + 2, 4 * g + 4) >> 1);
-#pragma unroll
-            for (int p = 0; p < NPL; ++p) {
-                l32[p * (G::PLANE / 2) + d0] = res[q][p][0];
-                l32[p * (G::PLANE / 2) + d1] = res[q][p][1];
-            }
-        }
-        __syncthreads();
+        asm volatile("" : "+s"(wv));  // likewise the wave's brick bases (scalar registers)
+        // This sweep's results are read by later sweeps only within
+        // r * (nsweeps - 1 - sw) points of the core; within 8, the top and
+        // bottom ring bricks are computed half each, paired in one product.
+        constexpr int R = ZLO == 0 ? 2 : 1;
+        if (R * (nsweeps - 1 - sw) <= 8)
+            fused_sweep<G, TH, ZLO, ZHI, PREC, true, LOOKAHEAD>(a, lds, C, mo, g, wv, edge, er0, ec0);
+        else
+            fused_sweep<G, TH, ZLO, ZHI, PREC, false, LOOKAHEAD>(a, lds, C, mo, g, wv, edge, er0, ec0);
     }
     Ops<G, ZLO, ZHI, PREC> cur, nxt;
     if constexpr (LOOKAHEAD) {
@@ -884,7 +953,8 @@ __device__ __forceinline__ void fused_tile(const SweepArgs &a, const CoefWindows
 // which LDS allows for tiles up to 4x8 (box2d2r 16384^2, 4x8 tile, 8 sweeps
 // fused: 142 -> 134 us per sweep against 2 waves per SIMD).
 template <int TH, int TW, int ZLO, int ZHI, int PREC>
-__global__ __launch_bounds__(kWG) __attribute__((amdgpu_waves_per_eu(base_prec(PREC) == TCS_PREC_SPLIT ? 1 : 4, 8)))
+__global__ __launch_bounds__(kWG)
+__attribute__((amdgpu_waves_per_eu(base_prec(PREC) == TCS_PREC_SPLIT ? (TH * TW < 64 ? 2 : 1) : 4, 8)))
 void tcs_mfma_fused(SweepArgs a, int nsweeps) {
     constexpr int NPL = base_prec(PREC) == TCS_PREC_SPLIT ? 2 : 1;
     __shared__ __attribute__((aligned(16))) uint16_t lds[NPL * FusedGeom<TH, TW, NPL, true>::PLANE];

@@ -280,15 +280,18 @@ def _afrag(coeff5, rs, round_coeff=f16):
     return afrag, ZLO, ZHI
 
 
-def _mfma_brick(lds, G: TileGeom, afrag, ZLO, ZHI, br, bc, edge, gr, gcb, L: Layout, rs):
+def _mfma_brick(lds, G: TileGeom, afrag, ZLO, ZHI, br, bc, edge, gr, gcb, L: Layout, rs, row=None):
     """Ops::load + mfma_brick: returns o[lane][v] (FP16 values) for LDS brick
-    (br, bc).  Every lane reads 16 bytes per fragment (KMap)."""
+    (br, bc).  Every lane reads 16 bytes per fragment (KMap).  row[lane]: the
+    lane's output row counted from the brick's first row (default lane % 16;
+    the fused kernel's paired ring units pass other rows)."""
     lane = np.arange(64)
     m, g = lane & 15, lane >> 4
+    row = m if row is None else row
     D = np.zeros((16, 16))
     for f in range(_nfrag(rs)):
         zi, kg, _ = _kmap(rs, f, g)
-        off = G.at(16 * br + m + ZLO + zi, 16 * bc + 8 * kg)
+        off = G.at(16 * br + row + ZLO + zi, 16 * bc + 8 * kg)
         _check(off, G.PLANE - 7, 8, "B operand LDS read (16 B)")
         bfrag = lds[off[:, None] + np.arange(8)[None, :]]  # [lane][e]
         Am = np.zeros((16, 32))
@@ -301,7 +304,7 @@ def _mfma_brick(lds, G: TileGeom, afrag, ZLO, ZHI, br, bc, edge, gr, gcb, L: Lay
     o = np.stack([f16(D[4 * g + v, m]) for v in range(4)], axis=1)  # [lane][v]
     if edge:
         for v in range(4):
-            cidx = G.at(16 * br + m + 2, 16 * bc + 4 * g + 2 + v)
+            cidx = G.at(16 * br + row + 2, 16 * bc + 4 * g + 2 + v)
             _check(cidx, G.PLANE - 1, 1, "centre read")
             keep = ~_interior(gr, gcb + v, L, rs)
             o[keep, v] = lds[cidx[keep]]
@@ -489,18 +492,42 @@ def mfma_fused(inp, L: Layout, TH, TW, coeff5, rs, nsweeps, round_coeff=f16):
         lds = stage_tile(inp, L, G, er0, ec0, check=True)
         assert not np.isnan(lds[G.image()]).any(), "fused image has unwritten LDS words"
         edge = not (er0 >= rs and er0 + 16 * BH <= L.rows - rs and ec0 >= rs and ec0 + 16 * BW <= L.cols - rs)
+        HB = BW // 2
+        QB, HP = (TH // 2) * HB, (HB + 1) // 2
+        mp = m + 8 + np.where(m >= 8, 16 * TH, 0)  # paired ring unit lane rows from brick row 0
         for sw in range(nsweeps - 1):
-            res = {}
+            paired = rs * (nsweeps - 1 - sw) <= 8
+            qt = QB + (HP if paired else HB)
+            res, rows_done = [], {}
             for wave in range(4):
-                for q in range(BH * BW // 4):
-                    br, bc = _block_brick(BH, BW, 0, wave, q)
-                    assert (br, bc) not in res, "two waves compute one brick"
-                    res[(br, bc)] = _mfma_brick(lds, G, afrag, ZLO, ZHI, br, bc, edge,
-                                                er0 + 16 * br + m, ec0 + 16 * bc + 4 * g, L, rs)
-            assert len(res) == BH * BW
-            for (br, bc), o in res.items():
+                for q in range(qt):
+                    if q < QB:
+                        br = 1 + (wave >> 1) * (TH // 2) + q // HB
+                        bc, row, pair = (wave & 1) * HB + q % HB, m, False
+                    elif not paired:
+                        br = TH + 1 if wave >> 1 else 0
+                        bc, row, pair = (wave & 1) * HB + (q - QB), m, False
+                    else:
+                        j = q - QB
+                        if (wave >> 1) and j >= HB - HP:
+                            continue
+                        br, bc, row, pair = 0, (wave & 1) * HB + (wave >> 1) * HP + j, mp, True
+                    # every output row of every computed brick, once
+                    for rr in np.unique(16 * br + row):
+                        key = (int(rr), bc)
+                        assert key not in rows_done, "two units compute one brick row"
+                        rows_done[key] = True
+                    o = _mfma_brick(lds, G, afrag, ZLO, ZHI, br, bc, edge, er0 + 16 * br + row,
+                                    ec0 + 16 * bc + 4 * g, L, rs, row=row)
+                    res.append((br, bc, row, o))
+            # rows the sweep must produce: within r * (nsweeps - 1 - sw) of the core
+            h = rs * (nsweeps - 1 - sw)
+            for rr in range(16 - h, 16 * (TH + 1) + h):
+                for bc in range(BW):
+                    assert (rr, bc) in rows_done, f"sweep {sw}: image row {rr} brick column {bc} not computed"
+            for br, bc, row, o in res:
                 for k in range(2):  # two dword writes per lane
-                    d = G.at(16 * br + m + 2, 16 * bc + 4 * g + 2 + 2 * k)
+                    d = G.at(16 * br + row + 2, 16 * bc + 4 * g + 2 + 2 * k)
                     _check(d, G.PLANE - 1, 2, "fused write-back (dword)")
                     lds[d] = o[:, 2 * k]
                     lds[d + 1] = o[:, 2 * k + 1]

@@ -96,7 +96,8 @@ def test_persistent_schedule_covers_every_tile(ntile_shape):
 
 
 @pytest.mark.parametrize("stencil", STENCILS)
-@pytest.mark.parametrize("case", [(70, 150, 2, 4, 3), (40, 70, 4, 4, 2), (50, 140, 2, 8, 5), (33, 140, 4, 8, 4)],
+@pytest.mark.parametrize("case", [(70, 150, 2, 4, 3), (40, 70, 4, 4, 2), (50, 140, 2, 8, 5), (33, 140, 4, 8, 4),
+                                  (40, 70, 4, 4, 9), (35, 140, 2, 8, 9)],
                          ids=lambda c: "%dx%d_t%dx%d_k%d" % c)
 def test_emulated_fused_equals_repeated_sweeps(oracle, stencil, case):
     """k fused sweeps == k single sweeps, bitwise, and both equal the FP64 oracle
