@@ -112,9 +112,11 @@ def _timed_graph(p, b0, b1, sweeps):
     return e0.elapsed_time(e1)
 
 
-def _measure(stencil, rows, cols, B, x, method, precision, sweeps, fuse=1, scale_exp=0, tile=None, graph=False):
+def _measure(stencil, rows, cols, B, x, method, precision, sweeps, fuse=1, scale_exp=0, tile=None, graph=False,
+             coef_lo=False):
     from . import api as T
-    p = T.Problem(stencil, rows, cols, B, method=method, precision=precision, fuse=fuse, tile=tile)
+    p = T.Problem(stencil, rows, cols, B, method=method, precision=precision, fuse=fuse, tile=tile,
+                  coef_lo=coef_lo)
     b0, b1 = p.alloc(), p.alloc()
     p.clear(b1)
     p.pack(x, b0, scale_exp)
@@ -122,7 +124,8 @@ def _measure(stencil, rows, cols, B, x, method, precision, sweeps, fuse=1, scale
     pts = p.interior_points * sweeps
     gst = pts / (ms * 1e-3) / 1e9
     return p, {"stencil": stencil, "grid": [rows, cols], "method": method, "precision": precision,
-               "fuse": fuse, "tile": list(p.tile), "sweeps": sweeps, "graph": graph, "ms": round(ms, 3),
+               "coef_lo": bool(coef_lo), "fuse": fuse, "tile": list(p.tile), "sweeps": sweeps, "graph": graph,
+               "ms": round(ms, 3),
                "gstencil_s": round(gst, 1),
                "hbm_gbs": round(gst * BYTES[precision] / fuse, 1),
                "hbm_frac": round(gst * BYTES[precision] / fuse / HBM_PEAK_GBS, 4),
@@ -159,10 +162,10 @@ def c2():
     _zero_ring(x, 1)
     refs = _fp64_refs(stencil, x, B, [1, 10, 100])
     res = {}
-    for method in ("mfma", "direct"):
-        p, rec = _measure(stencil, N, N, B, x, method, "fp16", 100)
+    for method, lo in (("mfma", False), ("mfma", True), ("direct", False), ("direct_dot2", False)):
+        p, rec = _measure(stencil, N, N, B, x, method, "fp16", 100, coef_lo=lo)
         rec["errors_vs_fp64"] = _errors(p, x, refs)
-        res[method] = rec["gstencil_s"]
+        res[method + ("_hi_lo" if lo else "")] = rec["gstencil_s"]
         emit(config="c2", **rec)
     for fuse in (8, 16):
         p, rec = _measure(stencil, N, N, B, x, "mfma", "fp16", 100, fuse=fuse)
@@ -171,7 +174,9 @@ def c2():
     for method, fuse in (("mfma", 1), ("direct", 1), ("mfma", 16)):
         p, rec = _measure(stencil, N, N, B, x, method, "fp16", 100, fuse=fuse, graph=True)
         emit(config="c2", **rec)
-    emit(config="c2", mfma_vs_direct=round(res["mfma"] / res["direct"], 3))
+    emit(config="c2", mfma_vs_direct=round(res["mfma"] / res["direct"], 3),
+         mfma_vs_direct_dot2=round(res["mfma"] / res["direct_dot2"], 3),
+         mfma_hi_lo_vs_direct=round(res["mfma_hi_lo"] / res["direct"], 3))
 
 
 def c3():
@@ -182,7 +187,7 @@ def c3():
         B = T.gen_coeffs(stencil, SEED)
         x64 = T.gen_sinusoid(N, N, seed=SEED, noise=1e-3)
         _zero_ring(x64, 2)
-        for precision in ("fp16", "bf16", "split"):
+        for precision, lo in (("fp16", False), ("fp16", True), ("bf16", False), ("bf16", True), ("split", False)):
             x = x64.clone()
             e = 0
             if precision == "split":
@@ -190,13 +195,14 @@ def c3():
             else:
                 T.round_to(x, precision)
             refs = _fp64_refs(stencil, x, B, [1, 10])
-            p, rec = _measure(stencil, N, N, B, x, "mfma", precision, 10, scale_exp=e)
+            p, rec = _measure(stencil, N, N, B, x, "mfma", precision, 10, scale_exp=e, coef_lo=lo)
             rec["errors_vs_fp64"] = _errors(p, x, refs, e)
             rec["input"] = "4 sinusoids + U[-1e-3,1e-3)" + ("" if precision == "split" else f", rounded to {precision}")
             emit(config="c3", **rec)
             del x, refs
-        p, rec = _measure(stencil, N, N, B, x64, "direct", "fp16", 10)
-        emit(config="c3", **rec)
+        for method in ("direct", "direct_dot2"):
+            p, rec = _measure(stencil, N, N, B, x64, method, "fp16", 10)
+            emit(config="c3", **rec)
         del x64
 
 
@@ -265,14 +271,19 @@ def c5(sweeps=256):
     ref = _fp64_refs(stencil, x, B, [sweeps])[sweeps]
     recs = []
     for fuse in (1, 2, 4, 8):
-        p, rec = _measure(stencil, R, C, B, x, "mfma", "split", sweeps, fuse=fuse, scale_exp=e)
-        if fuse == 8:
+        best = None
+        for tile in ((4, 4), (2, 8), (4, 8)):  # the tile is tuned per sweeps-per-launch count
+            p, rec = _measure(stencil, R, C, B, x, "mfma", "split", sweeps, fuse=fuse, scale_exp=e, tile=tile)
+            if best is None or rec["ms"] < best[1]["ms"]:
+                best = (p, rec)
+        p, rec = best
+        if fuse in (1, 8):
             rec["errors_vs_fp64"] = {sweeps: _errors(p, x, {sweeps: ref}, e)[sweeps]}
         recs.append(rec)
         emit(config="c5", **rec)
     x16 = x.clone()
     T.round_to(x16, "fp16")
-    for method in ("mfma", "direct"):
+    for method in ("mfma", "direct", "direct_dot2"):
         p, rec = _measure(stencil, R, C, B, x16, method, "fp16", 32)
         emit(config="c5", reference_point=True, **rec)
 
