@@ -97,7 +97,7 @@ def test_persistent_schedule_covers_every_tile(ntile_shape):
 
 @pytest.mark.parametrize("stencil", STENCILS)
 @pytest.mark.parametrize("case", [(70, 150, 2, 4, 3), (40, 70, 4, 4, 2), (50, 140, 2, 8, 5), (33, 140, 4, 8, 4),
-                                  (40, 70, 4, 4, 9), (35, 140, 2, 8, 9)],
+                                  (40, 70, 4, 4, 9), (35, 140, 2, 8, 9), (140, 150, 8, 8, 6)],
                          ids=lambda c: "%dx%d_t%dx%d_k%d" % c)
 def test_emulated_fused_equals_repeated_sweeps(oracle, stencil, case):
     """k fused sweeps == k single sweeps, bitwise, and both equal the FP64 oracle
@@ -131,6 +131,13 @@ def test_emulated_fused_equals_repeated_sweeps(oracle, stencil, case):
     for _ in range(k):
         ref = E.f16(oracle.sweep_f64(stencil, ref, B))
     assert np.array_equal(E.unpack(fused, L), ref)
+
+
+@pytest.mark.parametrize("stencil", ["star2d1r", "box2d1r"])
+def test_emulated_fused_every_ring_depth_radius1(oracle, stencil):
+    """Radius 1 at the most sweeps per launch (17): the ring rows of sweep s are
+    packed at every depth 16, 15, ..., 1, odd ones included."""
+    test_emulated_fused_equals_repeated_sweeps(oracle, stencil, (36, 70, 2, 4, 17))
 
 
 CASES32 = [(70, 150, 4, 4), (70, 150, 4, 8), (33, 17, 4, 4), (130, 140, 8, 8), (5, 3, 2, 8), (64, 128, 2, 8)]
