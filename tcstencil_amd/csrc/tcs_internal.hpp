@@ -25,6 +25,7 @@ constexpr int kCoefLo = 4;
 __host__ __device__ constexpr int base_prec(int kp) { return kp & 3; }
 __host__ __device__ constexpr bool has_coef_lo(int kp) { return (kp & kCoefLo) != 0; }
 constexpr int kXCD = 8;   // MI355X: 8 XCDs, workgroups dispatched round-robin
+constexpr int kLdsBytesPerCU = 160 * 1024;  // gfx950 LDS per CU
 
 // Storage element of global grid point (gr, gc), gr, gc >= -16: bricks are
 // 16x16 row-major, brick-row-major, behind a one-brick ghost ring, so stored

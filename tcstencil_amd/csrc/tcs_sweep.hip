@@ -951,10 +951,25 @@ __device__ __forceinline__ void fused_tile(const SweepArgs &a, const CoefWindows
 // Fused sweeps re-read every brick's operands from LDS each sweep (ring
 // included); FP16/BF16 kernels are held to 128 VGPRs for 4 waves per SIMD,
 // which LDS allows for tiles up to 4x8 (box2d2r 16384^2, 4x8 tile, 8 sweeps
-// fused: 142 -> 134 us per sweep against 2 waves per SIMD).
+// fused: 142 -> 134 us per sweep against 2 waves per SIMD).  SPLIT kernels
+// (two image planes) are held to as many waves per SIMD as LDS allows
+// workgroups per CU, at most 3 (168 VGPRs): 2x4, 4x4 and 2x8 tiles 3, 4x8 2,
+// 8x8 1.  Without the bound a 2x8 kernel that grew past 168 VGPRs fell to 2
+// waves per SIMD and ran 18% slower (box2d2r 16384^2, 8 fused sweeps: 391
+// against 331 us per sweep).
+template <int TH, int TW, int PREC>
+constexpr int kFusedMinWaves() {
+    if constexpr (base_prec(PREC) != TCS_PREC_SPLIT) {
+        return 4;
+    } else {
+        constexpr int wgs = kLdsBytesPerCU / (2 * 2 * FusedGeom<TH, TW, 2, true>::PLANE);
+        return wgs < 1 ? 1 : (wgs > 3 ? 3 : wgs);
+    }
+}
+
 template <int TH, int TW, int ZLO, int ZHI, int PREC>
 __global__ __launch_bounds__(kWG)
-__attribute__((amdgpu_waves_per_eu(base_prec(PREC) == TCS_PREC_SPLIT ? (TH * TW < 64 ? 2 : 1) : 4, 8)))
+__attribute__((amdgpu_waves_per_eu(kFusedMinWaves<TH, TW, PREC>(), 8)))
 void tcs_mfma_fused(SweepArgs a, int nsweeps) {
     constexpr int NPL = base_prec(PREC) == TCS_PREC_SPLIT ? 2 : 1;
     __shared__ __attribute__((aligned(16))) uint16_t lds[NPL * FusedGeom<TH, TW, NPL, true>::PLANE];
