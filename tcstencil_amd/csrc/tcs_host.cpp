@@ -480,13 +480,16 @@ extern "C" int tcs_graph_destroy(void *graph_exec) {
 // ----------------------------------------------------------- host drivers
 namespace {
 
-// Device allocations of one host call, released in every exit path.
+// Device allocations of one host call, released in every exit path (after
+// the call's stream has drained, so that no copy or kernel an error path left
+// in flight still uses them).
 struct DeviceScope {
     void *ptr[8] = {};
     int n = 0;
     hipStream_t stream = nullptr;
     hipEvent_t ev[2] = {nullptr, nullptr};
     ~DeviceScope() {
+        if (stream) (void)hipStreamSynchronize(stream);
         for (int i = 0; i < n; ++i) (void)hipFree(ptr[i]);
         if (ev[0]) (void)hipEventDestroy(ev[0]);
         if (ev[1]) (void)hipEventDestroy(ev[1]);
@@ -600,6 +603,7 @@ static int host_sweep_impl(int stencil, int method, int precision, const float *
     TCS_TRY(tcs_problem_init(&p, stencil, method, precision, rows, cols, h_B, n));
     TCS_TRY(tcs_problem_set_tile_size(&p, prm.TILE_SIZE));
 
+    PinnedStage stage;  // declared first: released after ds has drained the stream
     DeviceScope ds;
     TCS_HIP_CHECK(hipStreamCreateWithFlags(&ds.stream, hipStreamNonBlocking));
     TCS_HIP_CHECK(hipEventCreate(&ds.ev[0]));
@@ -613,7 +617,6 @@ static int host_sweep_impl(int stencil, int method, int precision, const float *
     TCS_TRY(ds.alloc(&buf1, buf_bytes));
     TCS_TRY(ds.alloc(&d_max, sizeof(double)));
     hipStream_t s = ds.stream;
-    PinnedStage stage;
     TCS_TRY(stage.init(grid_bytes));
     TCS_TRY(staged_h2d(d_src, h_A, grid_bytes, s, stage));
 
