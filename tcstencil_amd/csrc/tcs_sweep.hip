@@ -531,17 +531,24 @@ __device__ __forceinline__ void mfma_brick(const SweepArgs &a, const uint16_t *l
     }
     if (edge) {
         // Columns 4g+2, 4g+3 and 4g+4, 4g+5 of the image row: two pairs, each
-        // within one 8-column strip.
+        // one dword within an 8-column strip.  Both are read whatever the
+        // points are, then selected: no LDS read waits inside a branch.
         const int cb = G::at(16 * br, 16 * bc);
         const int c01 = cb + G::at(m + 2, 4 * g + 2);
         const int c23 = cb + G::at(m + 2, 4 * g + 4);
+        const uint32_t *l32 = reinterpret_cast<const uint32_t *>(lds);
+        const bool row_in = gr >= a.rs && gr < a.rows - a.rs;
 #pragma unroll
-        for (int v = 0; v < 4; ++v)
-            if (!point_is_interior(a, gr, gcb + v)) {
-                const int cidx = (v < 2 ? c01 : c23) + (v & 1);
+        for (int p = 0; p < NPL; ++p) {
+            const uint32_t k01 = l32[(p * G::PLANE + c01) >> 1], k23 = l32[(p * G::PLANE + c23) >> 1];
 #pragma unroll
-                for (int p = 0; p < NPL; ++p) o[p][v] = lds[p * G::PLANE + cidx];
+            for (int v = 0; v < 4; ++v) {
+                const int gc = gcb + v;
+                const bool interior = row_in && gc >= a.rs && gc < a.cols - a.rs;
+                const uint16_t keep = (uint16_t)((v < 2 ? k01 : k23) >> (16 * (v & 1)));
+                o[p][v] = interior ? o[p][v] : keep;
             }
+        }
     }
 #pragma unroll
     for (int p = 0; p < NPL; ++p) w[p] = u32x2{pack2(o[p][0], o[p][1]), pack2(o[p][2], o[p][3])};
@@ -1569,10 +1576,17 @@ __global__ __launch_bounds__(kWG) void tcs_direct_sweep(SweepArgs a) {
 #pragma unroll
         for (int v = 0; v < 4; ++v) h[v] = f16_bits(acc[o][v]);
         if (edge) {
+            // the input points, two dwords read whatever the points are, then selected
             const int cidx = (tr + 2) * T::LC + 4 * cg + 2;
+            const uint32_t *l32 = reinterpret_cast<const uint32_t *>(lds);
+            const uint32_t k01 = l32[cidx >> 1], k23 = l32[(cidx >> 1) + 1];
+            const bool row_in = gr >= a.rs && gr < a.rows - a.rs;
 #pragma unroll
-            for (int v = 0; v < 4; ++v)
-                if (!point_is_interior(a, gr, gcb + v)) h[v] = lds[cidx + v];
+            for (int v = 0; v < 4; ++v) {
+                const int gc = gcb + v;
+                const bool interior = row_in && gc >= a.rs && gc < a.cols - a.rs;
+                h[v] = interior ? h[v] : (uint16_t)((v < 2 ? k01 : k23) >> (16 * (v & 1)));
+            }
         }
         res[o] = u32x2{pack2(h[0], h[1]), pack2(h[2], h[3])};
         if constexpr (!kDirectStaged<STENCIL>) store8<false>(a.out + stor_index(gr, gcb, a.sbc), res[o]);
