@@ -8,12 +8,12 @@
 // of its next tile (16-byte loads for the data bricks, 4-byte loads for the
 // halo frame) so they are in flight meanwhile, then
 //   MFMA:   each wave computes whole 16x16 bricks.  For a brick, with X the
-//           20x20 haloed input (X[0][0] = brick origin - (2,2)),
-//               out^T = sum_z  C_z * S_z,   C_z[j][k] = c[z][k-j] (0 <= k-j <= 4),
+//           20x24 haloed input window (X[0][0] = brick origin - (2,4)),
+//               out^T = sum_z  C_z * S_z,   C_z[j][k] = c[z][k-j-2] (0 <= k-j-2 <= 4),
 //                                           S_z[k][i] = X[i+z][k]
 //           is one v_mfma_f32_16x16x32_{f16,bf16} per row shift z: M = output
-//           column j, N = output row i, K = haloed column k (20 of 32 slots
-//           used, the rest multiply zero coefficients).  The banded Toeplitz
+//           column j, N = output row i, K = window column k (slots 2..21 carry
+//           taps, the rest multiply zero coefficients).  The banded Toeplitz
 //           C_z is the reference's init_coe matrix (layout16/host_box2d2r.cu:29-50)
 //           transposed and widened to cover the halo columns, so no scalar
 //           halo fix-ups are needed.  C_z lives in registers for the whole
@@ -53,22 +53,31 @@ __device__ __forceinline__ uint32_t pack2(uint16_t lo, uint16_t hi) {
 
 // ------------------------------------------------------------ tile staging
 // LDS image of BH x BW data bricks plus a 2-point frame on every side: element
-// (r, c), r < LR, c < LC, where columns [16*BW+4, LC) hold zeros.  Two layouts:
-//   row-major (32x32 MFMA and direct kernels): pitch LC elements, so a row is
-//     16 (mod 32) bytes and rows step through the banks;
-//   STRIP (16x16 MFMA kernels): the image is cut into 8-column strips stored
-//     one after another, NR = 16*BH + 8 rows of 16 bytes each.  A 16x16 MFMA
-//     operand fragment (16 lanes = 16 consecutive rows of one 8-column group)
-//     then reads 256 contiguous bytes, and since NR = 8 (mod 16), strips two
-//     apart start at the same bank: lane groups (0,1) and (2,3) of a
+// (r, c), r < LR, c < LC.  Image row r holds global row r0 - 2 + r; image
+// column c holds global column c0 - FX + c, so the frame columns are FX - 2,
+// FX - 1 and 16*BW + FX, +1, and the other columns hold zeros.  Two layouts:
+//   row-major (32x32 MFMA and direct kernels, FX = 2): pitch LC elements, so a
+//     row is 16 (mod 32) bytes and rows step through the banks;
+//   STRIP (16x16 MFMA kernels, FX = 4): the image is cut into 8-column strips
+//     stored one after another, NR = 16*BH + 8 rows of 16 bytes each.  A 16x16
+//     MFMA operand fragment (16 lanes = 16 consecutive rows of one 8-column
+//     group) then reads 256 contiguous bytes, and since NR = 8 (mod 16), strips
+//     two apart start at the same bank: lane groups (0,1) and (2,3) of a
 //     ds_read_b128 reading the same rows of strips s and s+2 (or the same
 //     address) complete in the minimum 4 LDS cycles.  Measured on gfx950:
 //     ds_read_b128 of the 16x16 operand pattern takes 4 cycles per wave in
 //     this layout against 6 (unpacked) to 8 (packed) in the row-major one.
+//     With the data 4 columns in, a lane's 4 results (brick columns 4g..4g+3)
+//     are 8 aligned bytes of one strip row: the fused sweeps write them back
+//     with one ds_write_b64 that touches every bank exactly twice (16 lanes
+//     of consecutive rows per group, groups 0, 2 on the upper and 1, 3 on the
+//     lower half of the 16-byte rows), where two ds_write_b32 of the 2-column
+//     offset hit half the banks twice each.
 template <int BH_, int BW_, int NPL_, bool STRIP_ = false>
 struct Geom {
     static constexpr int BH = BH_, BW = BW_, NPL = NPL_;
     static constexpr bool STRIP = STRIP_;
+    static constexpr int FX = STRIP ? 4 : 2;    // image column of the first data column
     static constexpr int LR = 16 * BH + 4;      // image rows (2 frame rows each side)
     static constexpr int LC = 16 * BW + 8;      // image columns (row-major pitch)
     static constexpr int NR = 16 * BH + 8;      // rows per strip (STRIP)
@@ -121,9 +130,9 @@ __device__ __forceinline__ void halo_coords(int h, int &lr, int &ld) {
     }
 }
 
-// LDS element (lr, lc) of plane p holds global point (r0 - 2 + lr, c0 - 2 + lc)
-// for lr < LR, lc < 16*BW + 4; columns [16*BW+4, LP) hold zeros.  (r0, c0) is
-// the first element of the first data brick.  With CHECK, frame points outside
+// LDS element (lr, lc) of plane p holds global point (r0 - 2 + lr, c0 - FX + lc)
+// for lr < LR, FX - 2 <= lc < 16*BW + FX + 2; the other columns hold zeros.
+// (r0, c0) is the first element of the first data brick.  With CHECK, frame points outside
 // the stored bricks (possible when the data bricks include the ghost ring) read
 // as zero.  (Nontemporal data-brick loads measured slower.)
 template <class G, bool CHECK>
@@ -236,21 +245,30 @@ __device__ __forceinline__ void write_tile(const TileRegs<G> &R, uint16_t *lds, 
     for (int i = 0; i < G::CPT; ++i) {
         const int c = tid + i * kWG;
         if (G::NCH % kWG == 0 || c < G::NCH) {
-            int lr, lc;  // lc even: dword aligned, not 16-byte aligned
+            int lr, lc;  // lc = FX (mod 8)
             if constexpr (ROWMAJOR) {
                 const int lrow = c / (2 * G::BW);
                 lr = lrow + 2;
-                lc = 8 * (c - lrow * (2 * G::BW)) + 2;
+                lc = 8 * (c - lrow * (2 * G::BW)) + G::FX;
             } else {
                 const int b = c >> 5, w = c & 31;
                 const int br = b / G::BW, bc = b - br * G::BW;
                 lr = 16 * br + (w >> 1) + 2;
-                lc = 16 * bc + 8 * (w & 1) + 2;
+                lc = 16 * bc + 8 * (w & 1) + G::FX;
             }
 #pragma unroll
             for (int p = 0; p < G::NPL; ++p) {
+                if constexpr (G::FX == 4) {
+                    // the chunk's halves: elements 4..7 of one strip row and
+                    // 0..3 of the next strip's, 8 aligned bytes each
+                    u32x2 *l64 = reinterpret_cast<u32x2 *>(lds + p * G::PLANE);
+                    l64[G::at(lr, lc) >> 2] = u32x2{R.cv[p][i][0], R.cv[p][i][1]};
+                    l64[G::at(lr, lc + 4) >> 2] = u32x2{R.cv[p][i][2], R.cv[p][i][3]};
+                } else {
 #pragma unroll
-                for (int q = 0; q < 4; ++q) l32[p * (G::PLANE / 2) + (G::at(lr, lc + 2 * q) >> 1)] = R.cv[p][i][q];
+                    for (int q = 0; q < 4; ++q)
+                        l32[p * (G::PLANE / 2) + (G::at(lr, lc + 2 * q) >> 1)] = R.cv[p][i][q];
+                }
             }
         }
     }
@@ -260,20 +278,25 @@ __device__ __forceinline__ void write_tile(const TileRegs<G> &R, uint16_t *lds, 
         if (h < G::NH) {
             int lr, ld;
             halo_coords<G>(h, lr, ld);
-            const int d = G::at(lr, 2 * ld) >> 1;
+            const int d = G::at(lr, G::FX - 2 + 2 * ld) >> 1;
 #pragma unroll
             for (int p = 0; p < G::NPL; ++p) l32[p * (G::PLANE / 2) + d] = R.hv[p][i];
         }
     }
 }
 
-// Columns [16*BW+4, 16*BW+8) of every image row: read by the MFMA operand of
-// the last brick column (against zero coefficients), never written by write_tile.
+// The image columns that hold no grid point: [0, FX-2) and [16*BW+FX+2, LC),
+// one dword of each side per row for STRIP (FX = 4), columns [16*BW+4,
+// 16*BW+8) for the row-major image (FX = 2).  The MFMA operands of the first
+// and last brick column read them against zero coefficients, so they must be
+// finite; write_tile never writes them.
 template <class G>
 __device__ __forceinline__ void zero_pad(uint16_t *lds, int tid) {
+    static_assert(G::FX == 2 || G::FX == 4, "two pad dwords per image row");
     uint32_t *l32 = reinterpret_cast<uint32_t *>(lds);
     for (int zc = tid; zc < 2 * G::LR; zc += kWG) {
-        const int d = G::at(zc >> 1, 16 * G::BW + 4 + 2 * (zc & 1)) >> 1;
+        const int col = G::FX == 4 ? ((zc & 1) ? 16 * G::BW + 6 : 0) : 16 * G::BW + 4 + 2 * (zc & 1);
+        const int d = G::at(zc >> 1, col) >> 1;
 #pragma unroll
         for (int p = 0; p < G::NPL; ++p) l32[p * (G::PLANE / 2) + d] = 0u;
     }
@@ -372,53 +395,58 @@ template <>
 struct Frag<TCS_PREC_BF16 | kCoefLo> : Frag<TCS_PREC_BF16> {};
 
 // K slots of the 16x16x32 MFMAs of one brick.  K slot group g (k = 8g..8g+7)
-// of MFMA f multiplies haloed row m + ZLO + zi, columns 8kg..8kg+7 (operand
-// B, lane (m, g)) by the taps c[ZLO + zi][8kg + e - j] (operand A, lane
-// (j, g)).  Radius 1 (3 row shifts): one MFMA per shift zi = f, groups
-// g = 0..3 -> kg = 0, 2, 1, 1, the last against zero taps.  Radius 2 (5 row
-// shifts, 20 haloed columns): the 15 (shift, column group) pairs are packed
-// into 4 MFMAs instead of 5 (20% fewer MFMAs and LDS operand reads):
-//     f = 0: (0,0) (0,2) (1,0) (1,2)     f = 2: (4,0) (4,2) (0,1) (0,1)*
-//     f = 1: (2,0) (2,2) (3,0) (3,2)     f = 3: (1,1) (2,1) (3,1) (4,1)
-// (* zero taps).  The order serves the STRIP LDS layout: groups 0,1 and
-// groups 2,3 read the same rows of strips two apart, or the same address,
-// which reads without bank conflicts; only the f = 3 pairs differ by a row.
+// of MFMA f multiplies window row m + ZLO + zi, columns 8kg..8kg+7 (operand
+// B, lane (m, g)) by the taps c[ZLO + zi][8kg + e - j - 2] (operand A, lane
+// (j, g)).  Fragments 0..2 each cover one 8-column group, kg = 0, 2, 1, with
+// lane group g on row shift zi = g; radius 2 adds fragment 3 for the fifth
+// shift (zi = 4) with groups kg = 0, 2, 1, 1:
+//     f = 0: (0,0) (1,0) (2,0) (3,0)     f = 2: (0,1) (1,1) (2,1) (3,1)
+//     f = 1: (0,2) (1,2) (2,2) (3,2)     f = 3: (4,0) (4,2) (4,1) (4,1)*
+// (* zero taps).  Radius 1 has fragments 0..2 only, lane group 3 on shift 3
+// against zero taps.  4 MFMAs per brick for radius 2 (15 of 16 slot groups
+// carry taps), 3 for radius 1.
+//   Reuse: window columns 16..23 of a brick are columns 0..7 of the brick to
+// its right, so fragment 0 of a brick is fragment 1 of its left neighbour,
+// lane for lane: a wave that walks a brick row left to right reads 3
+// fragments per brick instead of 4 (radius 1: 2 instead of 3).
+//   LDS banks: a fragment 0..2 read is one strip, each lane group 16
+// consecutive rows; fragment 3 reads the same rows of strips two apart in
+// lane groups 0, 1 (the same banks, which gfx950 serves without conflict)
+// and one address twice in groups 2, 3.  Each takes the minimum 4 LDS cycles
+// per ds_read_b128 (measured); lane groups 0, 1 on strips one apart take 8.
 // Every 16x16 kernel (one sweep, fused, batched) uses this one map, so the
 // MFMAs add the same 8-column partial sums into the FP32 accumulator in the
 // same order everywhere, and k fused sweeps stay bitwise equal to k single
-// sweeps.
+// sweeps.  (The MFMA's FP32 sum depends on the slot order: the same products
+// with two slot groups swapped round differently in about half the outputs.)
 template <int ZLO, int ZHI>
 struct KMap {
     static constexpr int NZ = ZHI - ZLO + 1;
-    static constexpr bool PACKED = NZ == 5;
-    static constexpr int NF = PACKED ? 4 : NZ;  // MFMAs (operand fragments) per brick and plane
+    static_assert(NZ == 3 || NZ == 5, "radius 1 or 2");
+    static constexpr int NF = NZ == 5 ? 4 : 3;  // MFMAs (operand fragments) per brick and plane
+    static constexpr int REUSE_FROM = 1;          // fragment 0 = the left neighbour's fragment 1
+    static __host__ __device__ constexpr int kg_of(int f) { return f == 0 ? 0 : (f == 1 ? 2 : 1); }
     static __device__ __forceinline__ void group(int f, int g, int &zi, int &kg, bool &real) {
-        real = true;
-        if constexpr (PACKED) {
-            if (f < 2 || (f == 2 && g < 2)) {
-                zi = 2 * f + (g >> 1);
-                kg = 2 * (g & 1);
-            } else if (f == 2) {
-                zi = 0;
-                kg = 1;
-                real = g == 2;
-            } else {
-                zi = g + 1;
-                kg = 1;
-            }
+        if (f < 3) {
+            zi = g;
+            kg = kg_of(f);
+            real = g < NZ;
         } else {
-            zi = f;
-            kg = g == 0 ? 0 : (g == 1 ? 2 : 1);
+            zi = 4;
+            kg = kg_of(g < 3 ? g : 2);
             real = g < 3;
         }
     }
 };
 
-// Coefficient fragments: lane (j, g) of MFMA f holds c[ZLO + zi][8kg + e - j],
-// e = 0..7 (zero outside the taps) -- the window at d0 = 8kg - j of the
+// Coefficient fragments: lane (j, g) of MFMA f holds c[ZLO + zi][8kg + e - j - 2],
+// e = 0..7 (zero outside the taps) -- the window at d0 = 8kg - j - 2 of the
 // zero-padded tap row, loaded whole from the host-built window table
-// (CoefWindows; window 12 is all zeros).  lo: the second plane of the table
-// (SPLIT, and FP16/BF16 with hi+lo coefficients).
+// (CoefWindows; window 12 is all zeros).  K slot k = 8kg + e is image column
+// 16bc + k of the brick's window, and output column j sits at image column
+// 16bc + 4 + j (the STRIP image's data start 4 columns in), so tap d of
+// output j is slot j + 2 + d.  lo: the second plane of the table (SPLIT, and
+// FP16/BF16 with hi+lo coefficients).
 template <int ZLO, int ZHI, int PREC>
 struct Coef {
     typedef typename Frag<PREC>::type frag_t;
@@ -431,7 +459,7 @@ struct Coef {
             int zi, kg;
             bool real;
             K::group(f, g, zi, kg, real);
-            const int d0 = 8 * kg - m;
+            const int d0 = 8 * kg - m - 2;
             const int j = real && d0 >= -7 && d0 <= 4 ? d0 + 7 : 12;
             const uint32_t *wh = cw.w[0][ZLO + zi][j];
             hi[f] = __builtin_bit_cast(frag_t, u32x4{wh[0], wh[1], wh[2], wh[3]});
@@ -444,40 +472,71 @@ struct Coef {
 };
 
 // Operands of one 16x16 output brick: brick (br, bc) of the LDS image of
-// geometry G, whose haloed 20x20 window starts at LDS row 16*br, column 16*bc.
+// geometry G, whose 20x24 window starts at LDS row 16*br, column 16*bc.
 // Fragment f: lane (m, g) holds haloed row m + ZLO + zi, columns 8kg..8kg+7
 // (KMap), one 16-byte LDS read; all reads of a brick issue back to back.
 template <class G, int ZLO, int ZHI, int PREC>
 struct Ops {
     static_assert(G::STRIP, "the K slot order is chosen for the STRIP LDS layout");
     using K = KMap<ZLO, ZHI>;
+    static constexpr int NPL = base_prec(PREC) == TCS_PREC_SPLIT ? 2 : 1;
     u32x4 h[K::NF];
-    u32x4 l[base_prec(PREC) == TCS_PREC_SPLIT ? K::NF : 1];
+    u32x4 l[NPL == 2 ? K::NF : 1];
 
-    // Per-lane element offsets of the fragments within a brick's window.
+    // Per-lane element offsets of the fragments within a brick's window:
+    // fragments 1 and 2 are whole strips from fragment 0 (compile-time), so
+    // only fragment 0 and, for radius 2, fragment 3 need a register.
     struct Offsets {
-        int o[K::NF];
+        int o0, o3;
         __device__ __forceinline__ Offsets(int m, int g) {
-#pragma unroll
-            for (int f = 0; f < K::NF; ++f) {
-                int zi, kg;
-                bool real;
-                K::group(f, g, zi, kg, real);
-                o[f] = G::at(m + ZLO + zi, 8 * kg);
+            int zi, kg;
+            bool real;
+            K::group(0, g, zi, kg, real);
+            o0 = G::at(m + ZLO + zi, 8 * kg);
+            o3 = o0;
+            if constexpr (K::NF == 4) {
+                K::group(3, g, zi, kg, real);
+                o3 = G::at(m + ZLO + zi, 8 * kg);
             }
+        }
+        __device__ __forceinline__ int of(int f) const {
+            return f == 3 ? o3 : o0 + G::at(0, 8 * K::kg_of(f));
         }
     };
 
-    // Unpacked fragments differ by whole LDS rows (compile-time offsets from
-    // fragment 0's address); packed ones need the per-lane offsets.
+    __device__ __forceinline__ void read(const uint16_t *base, const Offsets &off, int f) {
+        h[f] = *reinterpret_cast<const u32x4 *>(base + off.of(f));
+        if constexpr (NPL == 2) l[f] = *reinterpret_cast<const u32x4 *>(base + G::PLANE + off.of(f));
+    }
+
+    // All fragments of brick (br, bc).
     __device__ __forceinline__ void load(const uint16_t *lds, int br, int bc, const Offsets &off) {
         const uint16_t *base = lds + G::at(16 * br, 16 * bc);
 #pragma unroll
-        for (int f = 0; f < K::NF; ++f) {
-            const int o = K::PACKED ? off.o[f] : off.o[0] + G::at(f, 0);
-            h[f] = *reinterpret_cast<const u32x4 *>(base + o);
-            if constexpr (base_prec(PREC) == TCS_PREC_SPLIT) l[f] = *reinterpret_cast<const u32x4 *>(base + G::PLANE + o);
-        }
+        for (int f = 0; f < K::NF; ++f) read(base, off, f);
+    }
+
+    // Brick (br, bc) whose left neighbour's operands are `left` (same lane
+    // rows): fragment 0 is taken from there, the others are read.  `left`
+    // may be this object.
+    __device__ __forceinline__ void load_after(const uint16_t *lds, int br, int bc, const Offsets &off,
+                                               const Ops &left) {
+        const u32x4 h0 = left.h[K::REUSE_FROM];
+        const u32x4 l0 = left.l[NPL == 2 ? K::REUSE_FROM : 0];
+        const uint16_t *base = lds + G::at(16 * br, 16 * bc);
+#pragma unroll
+        for (int f = 1; f < K::NF; ++f) read(base, off, f);
+        h[0] = h0;
+        if constexpr (NPL == 2) l[0] = l0;
+    }
+
+    // load_after when `reuse` (a compile-time constant after unrolling), else load.
+    __device__ __forceinline__ void load(const uint16_t *lds, int br, int bc, const Offsets &off, const Ops &left,
+                                         bool reuse) {
+        if (reuse)
+            load_after(lds, br, bc, off, left);
+        else
+            load(lds, br, bc, off);
     }
 };
 
@@ -530,22 +589,20 @@ __device__ __forceinline__ void mfma_brick(const SweepArgs &a, const uint16_t *l
         }
     }
     if (edge) {
-        // Columns 4g+2, 4g+3 and 4g+4, 4g+5 of the image row: two pairs, each
-        // one dword within an 8-column strip.  Both are read whatever the
-        // points are, then selected: no LDS read waits inside a branch.
-        const int cb = G::at(16 * br, 16 * bc);
-        const int c01 = cb + G::at(m + 2, 4 * g + 2);
-        const int c23 = cb + G::at(m + 2, 4 * g + 4);
-        const uint32_t *l32 = reinterpret_cast<const uint32_t *>(lds);
+        // The input at the 4 output points: columns 4g+4..4g+7 of the image
+        // row, 8 aligned bytes of one strip row, read whatever the points are,
+        // then selected: no LDS read waits inside a branch.
+        static_assert(G::FX == 4, "a lane's 4 points are 8 aligned bytes of the STRIP image");
+        const int ck = G::at(16 * br, 16 * bc) + G::at(m + 2, 4 * g + G::FX);
         const bool row_in = gr >= a.rs && gr < a.rows - a.rs;
 #pragma unroll
         for (int p = 0; p < NPL; ++p) {
-            const uint32_t k01 = l32[(p * G::PLANE + c01) >> 1], k23 = l32[(p * G::PLANE + c23) >> 1];
+            const u32x2 k = *reinterpret_cast<const u32x2 *>(lds + p * G::PLANE + ck);
 #pragma unroll
             for (int v = 0; v < 4; ++v) {
                 const int gc = gcb + v;
                 const bool interior = row_in && gc >= a.rs && gc < a.cols - a.rs;
-                const uint16_t keep = (uint16_t)((v < 2 ? k01 : k23) >> (16 * (v & 1)));
+                const uint16_t keep = (uint16_t)(k[v >> 1] >> (16 * (v & 1)));
                 o[p][v] = interior ? o[p][v] : keep;
             }
         }
@@ -828,7 +885,10 @@ __device__ __forceinline__ void fused_sweep(const SweepArgs &a, uint16_t *lds, c
             valid = (wave >> 1) == 0 || j < HB - HP;
         }
     };
-    uint32_t *l32 = reinterpret_cast<uint32_t *>(lds);
+    // Unit q follows its left neighbour (same lane rows) in this wave's order,
+    // so its fragment 0 is the previous unit's fragment 1.
+    auto reuse = [&](int q) { return q < QB ? q % HB != 0 : q > QB; };
+    u32x2 *l64 = reinterpret_cast<u32x2 *>(lds);
     u32x2 res[QT][NPL];
     O cur, nxt;
     if constexpr (LOOKAHEAD) {
@@ -848,10 +908,10 @@ __device__ __forceinline__ void fused_sweep(const SweepArgs &a, uint16_t *lds, c
                 int nbr, nbc;
                 bool npair, nvalid;
                 unit(q + 1, nbr, nbc, npair, nvalid);
-                if (nvalid) nxt.load(npair ? ldsp : lds, nbr, nbc, offs);
+                if (nvalid) nxt.load(npair ? ldsp : lds, nbr, nbc, offs, cur, reuse(q + 1));
             }
         } else {
-            cur.load(pair ? ldsp : lds, br, bc, offs);
+            cur.load(pair ? ldsp : lds, br, bc, offs, cur, reuse(q));
         }
         __builtin_amdgcn_sched_barrier(0);
         const int row = pair ? mp : mo;
@@ -867,14 +927,10 @@ __device__ __forceinline__ void fused_sweep(const SweepArgs &a, uint16_t *lds, c
         unit(q, br, bc, pair, valid);
         if (!valid) continue;
         const int row = pair ? mp : mo;
-        const int db = G::at(16 * br, 16 * bc) >> 1;
-        const int d0 = db + (G::at(row + 2, 4 * g + 2) >> 1);
-        const int d1 = db + (G::at(row + 2, 4 * g + 4) >> 1);
+        // the lane's 4 results: 8 aligned bytes of one strip row (one ds_write_b64)
+        const int e = G::at(16 * br, 16 * bc) + G::at(row + 2, 4 * g + G::FX);
 #pragma unroll
-        for (int p = 0; p < NPL; ++p) {
-            l32[p * (G::PLANE / 2) + d0] = res[q][p][0];
-            l32[p * (G::PLANE / 2) + d1] = res[q][p][1];
-        }
+        for (int p = 0; p < NPL; ++p) l64[(p * G::PLANE + e) >> 2] = res[q][p];
     }
     __syncthreads();
 }
@@ -931,6 +987,9 @@ __device__ __forceinline__ void fused_tile(const SweepArgs &a, const CoefWindows
         block_brick<TH, TW, 1>(wave, 0, br, bc);
         cur.load(lds, br, bc, off);
     }
+    // block_brick walks the wave's block row by row: brick q follows its left
+    // neighbour unless it starts a row.
+    auto reuse = [](int q) { return q % (TW / 2) != 0; };
 #pragma unroll
     for (int q = 0; q < QC; ++q) {
         int br, bc;
@@ -939,10 +998,10 @@ __device__ __forceinline__ void fused_tile(const SweepArgs &a, const CoefWindows
             if (q + 1 < QC) {
                 int nbr, nbc;
                 block_brick<TH, TW, 1>(wave, q + 1, nbr, nbc);
-                nxt.load(lds, nbr, nbc, off);
+                nxt.load(lds, nbr, nbc, off, cur, reuse(q + 1));
             }
         } else {
-            cur.load(lds, br, bc, off);
+            cur.load(lds, br, bc, off, cur, reuse(q));
         }
         __builtin_amdgcn_sched_barrier(0);
         const int gr = er0 + 16 * br + m, gcb = ec0 + 16 * bc + 4 * g;

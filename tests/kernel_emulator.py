@@ -102,12 +102,14 @@ def unpack(st: np.ndarray, L: Layout, ring=None, rs=0) -> np.ndarray:
 # ------------------------------------------------------------ tile staging
 class TileGeom:
     """LDS image of TH x TW bricks plus the 2-point frame: image point (r, c),
-    r < LR, c < LP.  Row-major (pitch LP) for the 32x32 and direct kernels;
-    strip=True (16x16 MFMA kernels): 8-column strips of NR = 16*TH + 8 rows,
-    stored one after another."""
+    r < LR, c < LP, holds grid point (r0 - 2 + r, c0 - FX + c).  Row-major
+    (pitch LP, FX = 2) for the 32x32 and direct kernels; strip=True (16x16 MFMA
+    kernels, FX = 4): 8-column strips of NR = 16*TH + 8 rows, stored one after
+    another."""
 
     def __init__(self, TH, TW, strip=False):
         self.TH, self.TW, self.strip = TH, TW, strip
+        self.FX = 4 if strip else 2
         self.LR = 16 * TH + 4
         self.LP = 16 * TW + 8
         self.NR = 16 * TH + 8
@@ -131,6 +133,14 @@ class TileGeom:
         return r * self.LP + c
 
 
+def _pad_col(G: TileGeom, zc: int) -> int:
+    """zero_pad: the image columns holding no grid point, one dword each side
+    per row (FX = 4: columns 0,1 and 16TW+6,+7; FX = 2: 16TW+4..+7)."""
+    if G.FX == 4:
+        return 16 * G.TW + 6 if zc & 1 else 0
+    return 16 * G.TW + 4 + 2 * (zc & 1)
+
+
 def stage_tile(inp: np.ndarray, L: Layout, G: TileGeom, gr0: int, gc0: int, check=False,
                zero_pad=True) -> np.ndarray:
     """load_tile + write_tile (+ zero_pad).  check: frame points outside the
@@ -147,12 +157,19 @@ def stage_tile(inp: np.ndarray, L: Layout, G: TileGeom, gr0: int, gc0: int, chec
         s = stor_index(gr0 + 16 * br + (w >> 1), gc0 + 16 * bc + 8 * (w & 1), L.sbc)
         _check(s, L.plane - 7, 8, "centre load (16 B)")
         lr = 16 * br + (w >> 1) + 2
-        lc = 16 * bc + 8 * (w & 1) + 2
-        for q in range(4):  # four dword writes
-            e = G.at(lr, lc + 2 * q)
-            _check(e, G.PLANE - 1, 2, "centre LDS write (dword)")
-            lds[e] = inp[s + 2 * q]
-            lds[e + 1] = inp[s + 2 * q + 1]
+        lc = 16 * bc + 8 * (w & 1) + G.FX
+        if G.FX == 4:  # two 8-byte writes: the chunk straddles two strips
+            for q in range(2):
+                e = G.at(lr, lc + 4 * q)
+                _check(e, G.PLANE - 3, 4, "centre LDS write (8 B)")
+                for t in range(4):
+                    lds[e + t] = inp[s + 4 * q + t]
+        else:
+            for q in range(4):  # four dword writes
+                e = G.at(lr, lc + 2 * q)
+                _check(e, G.PLANE - 1, 2, "centre LDS write (dword)")
+                lds[e] = inp[s + 2 * q]
+                lds[e + 1] = inp[s + 2 * q + 1]
     for i in range(G.HPT):
         h = tid + i * WG
         lr = np.empty_like(h)
@@ -173,14 +190,14 @@ def stage_tile(inp: np.ndarray, L: Layout, G: TileGeom, gr0: int, gc0: int, chec
             inside = (((gr + 16) >> 4) >= 0) & (((gr + 16) >> 4) < L.sbr) & (((gc + 16) >> 4) >= 0) & (((gc + 16) >> 4) < L.sbc)
         s = stor_index(gr, gc, L.sbc)
         _check(s[inside], L.plane - 1, 2, "halo load (4 B)")
-        d = G.at(lr[act], 2 * ld[act]) >> 1
+        d = G.at(lr[act], G.FX - 2 + 2 * ld[act]) >> 1
         _check(2 * d, G.PLANE - 1, 2, "halo LDS write")
         v0, v1 = np.zeros(gr.shape), np.zeros(gr.shape)
         v0[inside], v1[inside] = inp[s[inside]], inp[s[inside] + 1]
         lds[2 * d] = v0
         lds[2 * d + 1] = v1
     for zc in range(2 * G.LR if zero_pad else 0):
-        d = int(G.at(zc >> 1, 16 * TW + 4 + 2 * (zc & 1))) >> 1
+        d = int(G.at(zc >> 1, _pad_col(G, zc))) >> 1
         assert 0 <= 2 * d < G.PLANE
         lds[2 * d] = 0.0
         lds[2 * d + 1] = 0.0
@@ -237,29 +254,22 @@ def _interior(gr, gc, L: Layout, rs: int):
 
 
 # ------------------------------------------------------------- MFMA sweep
-# (row shift index, column group) of the K slot groups g = 0..3 of MFMA f; the
-# radius-2 packing of the 15 pairs into 4 MFMAs, with None for the group that
-# meets zero taps (it reads the pair to its left).
-_PACKED_R2 = (((0, 0), (0, 2), (1, 0), (1, 2)),
-              ((2, 0), (2, 2), (3, 0), (3, 2)),
-              ((4, 0), (4, 2), (0, 1), None),
-              ((1, 1), (2, 1), (3, 1), (4, 1)))
+# (row shift index, column group) of the K slot groups g = 0..3 of MFMA f:
+# fragments 0..2 are the column groups 0, 2, 1 with lane group g on shift g;
+# radius 2 adds fragment 3, shift 4 with column groups 0, 2, 1, 1 (the last
+# against zero taps).  Radius 1 (shifts 0..2) reads shift 3 in lane group 3
+# against zero taps.  Fragment 0 of a brick is fragment 1 of its left neighbour.
+_KG = (0, 2, 1)
 
 
 def _kmap(rs, f, g):
     """KMap of tcs_sweep.hip: (row shift index, column group, real) of K slot
-    group g (array of lane groups) of MFMA f.  Radius 2: the packed table
-    above.  Radius 1: one MFMA per shift, column groups 0, 2, 1 and a repeat
-    of 1 against zero taps."""
+    group g (array of lane groups) of MFMA f."""
     g = np.asarray(g)
-    if rs == 2:
-        row = _PACKED_R2[f]
-        ent = [row[k] if row[k] is not None else row[k - 1] for k in range(4)]
-        zi = np.array([ent[k][0] for k in range(4)])[g]
-        kg = np.array([ent[k][1] for k in range(4)])[g]
-        real = np.array([row[k] is not None for k in range(4)])[g]
-        return zi, kg, real
-    return np.full_like(g, f), np.array([0, 2, 1, 1])[g], g < 3
+    nz = 5 if rs == 2 else 3
+    if f < 3:
+        return g.copy(), np.full_like(g, _KG[f]), g < nz
+    return np.full_like(g, 4), np.array([0, 2, 1, 1])[g], g < 3
 
 
 def _nfrag(rs):
@@ -274,13 +284,13 @@ def _afrag(coeff5, rs, round_coeff=f16):
     for f in range(_nfrag(rs)):
         zi, kg, real = _kmap(rs, f, g)
         for e in range(8):
-            d = 8 * kg + e - m
+            d = 8 * kg + e - m - 2  # output j is window column j + 2 (the strip image's FX = 4)
             ok = real & (d >= 0) & (d <= 4)
             afrag[f, ok, e] = round_coeff(coeff5[ZLO + zi[ok], d[ok]])
     return afrag, ZLO, ZHI
 
 
-def _mfma_brick(lds, G: TileGeom, afrag, ZLO, ZHI, br, bc, edge, gr, gcb, L: Layout, rs, row=None):
+def _mfma_brick(lds, G: TileGeom, afrag, ZLO, ZHI, br, bc, edge, gr, gcb, L: Layout, rs, row=None, frags_out=None):
     """Ops::load + mfma_brick: returns o[lane][v] (FP16 values) for LDS brick
     (br, bc).  Every lane reads 16 bytes per fragment (KMap).  row[lane]: the
     lane's output row counted from the brick's first row (default lane % 16;
@@ -289,11 +299,13 @@ def _mfma_brick(lds, G: TileGeom, afrag, ZLO, ZHI, br, bc, edge, gr, gcb, L: Lay
     m, g = lane & 15, lane >> 4
     row = m if row is None else row
     D = np.zeros((16, 16))
+    frags = []
     for f in range(_nfrag(rs)):
         zi, kg, _ = _kmap(rs, f, g)
         off = G.at(16 * br + row + ZLO + zi, 16 * bc + 8 * kg)
         _check(off, G.PLANE - 7, 8, "B operand LDS read (16 B)")
         bfrag = lds[off[:, None] + np.arange(8)[None, :]]  # [lane][e]
+        frags.append(bfrag)
         Am = np.zeros((16, 32))
         Bm = np.zeros((32, 16))
         for l in range(64):
@@ -304,11 +316,22 @@ def _mfma_brick(lds, G: TileGeom, afrag, ZLO, ZHI, br, bc, edge, gr, gcb, L: Lay
     o = np.stack([f16(D[4 * g + v, m]) for v in range(4)], axis=1)  # [lane][v]
     if edge:
         for v in range(4):
-            cidx = G.at(16 * br + row + 2, 16 * bc + 4 * g + 2 + v)
+            cidx = G.at(16 * br + row + 2, 16 * bc + 4 * g + G.FX + v)
             _check(cidx, G.PLANE - 1, 1, "centre read")
             keep = ~_interior(gr, gcb + v, L, rs)
             o[keep, v] = lds[cidx[keep]]
+    if frags_out is not None:
+        frags_out[:] = frags
     return o
+
+
+def _check_reuse(frags, prev):
+    """The kernel takes fragment 0 of a brick from its left neighbour's
+    fragment 1 instead of reading it: both must hold the same LDS words."""
+    assert prev is not None, "reuse without a previous unit"
+    a, b = frags[0], prev[1]
+    assert np.array_equal(np.isnan(a), np.isnan(b)) and np.array_equal(a[~np.isnan(a)], b[~np.isnan(b)]), \
+        "fragment 0 differs from the left neighbour's fragment 1"
 
 
 def _store(out, written, L, gr, gcb, o):
@@ -368,12 +391,14 @@ def stage_tile_rm(src, rows, cols, ld, G: TileGeom, gr0: int, gc0: int, vec: boo
             ie = gr[ok] * ld + gc[ok] + e
             _check(ie, n, 1, "row-major load")
             x[ok, e] = src[ie]
-        lr, lc = lrow + 2, 8 * cc + 2
-        for q in range(4):
-            e0 = G.at(lr, lc + 2 * q)
-            assert np.all(np.isnan(lds[e0])), "two chunks write one LDS word"
-            lds[e0] = f16(x[:, 2 * q])
-            lds[e0 + 1] = f16(x[:, 2 * q + 1])
+        lr, lc = lrow + 2, 8 * cc + G.FX
+        step = 4 if G.FX == 4 else 2  # STRIP: two 8-byte writes per chunk, else four dwords
+        for q in range(8 // step):
+            e0 = G.at(lr, lc + step * q)
+            _check(e0, G.PLANE - step + 1, step, "row-major chunk LDS write")
+            for t in range(step):
+                assert np.all(np.isnan(lds[e0 + t])), "two chunks write one LDS word"
+                lds[e0 + t] = f16(x[:, step * q + t])
     for i in range(G.HPT):
         h = tid + i * WG
         h = h[h < G.NH]
@@ -389,7 +414,7 @@ def stage_tile_rm(src, rows, cols, ld, G: TileGeom, gr0: int, gc0: int, vec: boo
         m4 = ~(m1 | m2 | m3)
         lr[m4], ldw[m4] = 2 + (h[m4] - 4 * G.HD - 16 * TH), G.HD - 1
         gr, gc = gr0 - 2 + lr, gc0 - 2 + 2 * ldw
-        d = G.at(lr, 2 * ldw)
+        d = G.at(lr, G.FX - 2 + 2 * ldw)
         for e in range(2):
             ok = (gr >= 0) & (gr < rows) & (gc + e >= 0) & (gc + e < cols)
             ie = gr[ok] * ld + gc[ok] + e
@@ -398,7 +423,7 @@ def stage_tile_rm(src, rows, cols, ld, G: TileGeom, gr0: int, gc0: int, vec: boo
             v[ok] = src[ie]
             lds[d + e] = f16(v)
     for zc in range(2 * G.LR):
-        d = int(G.at(zc >> 1, 16 * TW + 4 + 2 * (zc & 1)))
+        d = int(G.at(zc >> 1, _pad_col(G, zc)))
         lds[d] = lds[d + 1] = 0.0
     return lds
 
@@ -500,6 +525,7 @@ def mfma_fused(inp, L: Layout, TH, TW, coeff5, rs, nsweeps, round_coeff=f16):
             qt = QB + (HP if paired else HB)
             res, rows_done = [], {}
             for wave in range(4):
+                prev = None
                 for q in range(qt):
                     if q < QB:
                         br = 1 + (wave >> 1) * (TH // 2) + q // HB
@@ -517,25 +543,33 @@ def mfma_fused(inp, L: Layout, TH, TW, coeff5, rs, nsweeps, round_coeff=f16):
                         key = (int(rr), bc)
                         assert key not in rows_done, "two units compute one brick row"
                         rows_done[key] = True
+                    frags = []
                     o = _mfma_brick(lds, G, afrag, ZLO, ZHI, br, bc, edge, er0 + 16 * br + row,
-                                    ec0 + 16 * bc + 4 * g, L, rs, row=row)
+                                    ec0 + 16 * bc + 4 * g, L, rs, row=row, frags_out=frags)
+                    if (q % HB != 0) if q < QB else (q > QB):  # tcs_sweep.hip fused_sweep's reuse(q)
+                        _check_reuse(frags, prev)
+                    prev = frags
                     res.append((br, bc, row, o))
             # rows the sweep must produce: within r * (nsweeps - 1 - sw) of the core
             h = rs * (nsweeps - 1 - sw)
             for rr in range(16 - h, 16 * (TH + 1) + h):
                 for bc in range(BW):
                     assert (rr, bc) in rows_done, f"sweep {sw}: image row {rr} brick column {bc} not computed"
-            for br, bc, row, o in res:
-                for k in range(2):  # two dword writes per lane
-                    d = G.at(16 * br + row + 2, 16 * bc + 4 * g + 2 + 2 * k)
-                    _check(d, G.PLANE - 1, 2, "fused write-back (dword)")
-                    lds[d] = o[:, 2 * k]
-                    lds[d + 1] = o[:, 2 * k + 1]
+            for br, bc, row, o in res:  # one 8-byte write per lane
+                d = G.at(16 * br + row + 2, 16 * bc + 4 * g + G.FX)
+                _check(d, G.PLANE - 3, 4, "fused write-back (8 B)")
+                for v in range(4):
+                    lds[d + v] = o[:, v]
         for wave in range(4):
+            prev = None
             for q in range(TH * TW // 4):
                 br, bc = _block_brick(TH, TW, 1, wave, q)
                 gr, gcb = er0 + 16 * br + m, ec0 + 16 * bc + 4 * g
-                o = _mfma_brick(lds, G, afrag, ZLO, ZHI, br, bc, edge, gr, gcb, L, rs)
+                frags = []
+                o = _mfma_brick(lds, G, afrag, ZLO, ZHI, br, bc, edge, gr, gcb, L, rs, frags_out=frags)
+                if q % (TW // 2) != 0:  # fused_tile's reuse(q)
+                    _check_reuse(frags, prev)
+                prev = frags
                 _store(out, written, L, gr, gcb, o)
     return out, written
 
