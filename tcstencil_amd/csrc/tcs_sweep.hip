@@ -540,16 +540,10 @@ struct Ops {
     }
 };
 
-// The brick's MFMAs and epilogue: lane (m, g) gets output row m, columns
-// 4g..4g+3 in storage format (w[p]: four 16-bit values of plane p).  gr, gcb:
-// global row and first column of those outputs; with `edge`, points outside
-// the interior keep their input value (read from the LDS image).
+// The brick's MFMAs: lane (m, g) accumulates output row m, columns 4g..4g+3
+// in FP32.
 template <class G, int ZLO, int ZHI, int PREC>
-__device__ __forceinline__ void mfma_brick(const SweepArgs &a, const uint16_t *lds,
-                                           const Coef<ZLO, ZHI, PREC> &C,
-                                           const Ops<G, ZLO, ZHI, PREC> &op, int br, int bc, int m, int g,
-                                           bool edge, int gr, int gcb, u32x2 (&w)[G::NPL]) {
-    constexpr int NPL = G::NPL;
+__device__ __forceinline__ f32x4 mfma_acc(const Coef<ZLO, ZHI, PREC> &C, const Ops<G, ZLO, ZHI, PREC> &op) {
     constexpr int NF = KMap<ZLO, ZHI>::NF;
     using F = Frag<PREC>;
     typedef typename F::type frag_t;
@@ -574,6 +568,18 @@ __device__ __forceinline__ void mfma_brick(const SweepArgs &a, const uint16_t *l
             acc = F::mma(C.lo[f], bh, acc);
         }
     }
+    return acc;
+}
+
+// The brick's epilogue: the accumulator of lane (m, g) (output row m, columns
+// 4g..4g+3 of brick (br, bc)) in storage format (w[p]: four 16-bit values of
+// plane p).  gr, gcb: global row and first column of those outputs; with
+// `edge`, points outside the interior keep their input value (read from the
+// LDS image).
+template <class G, int PREC>
+__device__ __forceinline__ void brick_out(const SweepArgs &a, const uint16_t *lds, const f32x4 &acc, int br, int bc,
+                                          int m, int g, bool edge, int gr, int gcb, u32x2 (&w)[G::NPL]) {
+    constexpr int NPL = G::NPL;
     uint16_t o[NPL][4];
 #pragma unroll
     for (int v = 0; v < 4; ++v) {
@@ -609,6 +615,15 @@ __device__ __forceinline__ void mfma_brick(const SweepArgs &a, const uint16_t *l
     }
 #pragma unroll
     for (int p = 0; p < NPL; ++p) w[p] = u32x2{pack2(o[p][0], o[p][1]), pack2(o[p][2], o[p][3])};
+}
+
+// MFMAs and epilogue of one brick.
+template <class G, int ZLO, int ZHI, int PREC>
+__device__ __forceinline__ void mfma_brick(const SweepArgs &a, const uint16_t *lds,
+                                           const Coef<ZLO, ZHI, PREC> &C,
+                                           const Ops<G, ZLO, ZHI, PREC> &op, int br, int bc, int m, int g,
+                                           bool edge, int gr, int gcb, u32x2 (&w)[G::NPL]) {
+    brick_out<G, PREC>(a, lds, mfma_acc<G, ZLO, ZHI, PREC>(C, op), br, bc, m, g, edge, gr, gcb, w);
 }
 
 // Brick q of a wave's list -> (br, bc) in the LDS image: bricks dealt to the
@@ -897,28 +912,45 @@ __device__ __forceinline__ void fused_sweep(const SweepArgs &a, uint16_t *lds, c
         unit(0, br, bc, pair, valid);
         cur.load(lds, br, bc, offs);
     }
+    // A unit's epilogue (conversion to the storage format) runs after the next
+    // unit's MFMAs are issued, so the wave does not wait on its own MFMA chain.
+    f32x4 pacc = {0.0f, 0.0f, 0.0f, 0.0f};
+    int pbr = 0, pbc = 0, prow = 0;
+    bool pvalid = false;
 #pragma unroll
     for (int q = 0; q < QT; ++q) {
         int br, bc;
         bool pair, valid;
         unit(q, br, bc, pair, valid);
-        if (!valid) continue;
-        if constexpr (LOOKAHEAD) {
-            if (q + 1 < QT) {
-                int nbr, nbc;
-                bool npair, nvalid;
-                unit(q + 1, nbr, nbc, npair, nvalid);
-                if (nvalid) nxt.load(npair ? ldsp : lds, nbr, nbc, offs, cur, reuse(q + 1));
-            }
-        } else {
-            cur.load(pair ? ldsp : lds, br, bc, offs, cur, reuse(q));
-        }
-        __builtin_amdgcn_sched_barrier(0);
         const int row = pair ? mp : mo;
-        mfma_brick<G, ZLO, ZHI, PREC>(a, lds, C, cur, br, bc, row, g, edge, er0 + 16 * br + row, ec0 + 16 * bc + 4 * g,
-                                      res[q]);
+        f32x4 acc = pacc;
+        if (valid) {
+            if constexpr (LOOKAHEAD) {
+                if (q + 1 < QT) {
+                    int nbr, nbc;
+                    bool npair, nvalid;
+                    unit(q + 1, nbr, nbc, npair, nvalid);
+                    if (nvalid) nxt.load(npair ? ldsp : lds, nbr, nbc, offs, cur, reuse(q + 1));
+                }
+            } else {
+                cur.load(pair ? ldsp : lds, br, bc, offs, cur, reuse(q));
+            }
+            __builtin_amdgcn_sched_barrier(0);
+            acc = mfma_acc<G, ZLO, ZHI, PREC>(C, cur);
+        }
+        if (q > 0 && pvalid)
+            brick_out<G, PREC>(a, lds, pacc, pbr, pbc, prow, g, edge, er0 + 16 * pbr + prow, ec0 + 16 * pbc + 4 * g,
+                               res[q - 1]);
+        pacc = acc;
+        pbr = br;
+        pbc = bc;
+        prow = row;
+        pvalid = valid;
         if constexpr (LOOKAHEAD) cur = nxt;
     }
+    if (pvalid)
+        brick_out<G, PREC>(a, lds, pacc, pbr, pbc, prow, g, edge, er0 + 16 * pbr + prow, ec0 + 16 * pbc + 4 * g,
+                           res[QT - 1]);
     __syncthreads();  // every wave has read this sweep's input
 #pragma unroll
     for (int q = 0; q < QT; ++q) {
