@@ -635,16 +635,31 @@ __device__ __forceinline__ void brick_of(int b, int &br, int &bc) {
 }
 
 // Brick q of a wave's share of a BH x BW brick region (both even) whose first
-// brick is (BASE, BASE), for the sweep loop of the fused kernels: the 4 waves
-// split the region 2 x 2, wave w taking the (BH/2) x (BW/2) block at block
-// position (w / 2, w % 2), q row-major within it.  A brick's offset from its
-// block's first brick is then a compile-time constant, so a per-brick LDS
-// address is one register plus an immediate.
+// brick is (BASE, BASE), for the sweep loops of the fused kernels, q row-major
+// within the wave's block: with BH a multiple of 4 wave w takes whole rows
+// (BH/4 of them, from row w * BH/4), else the 4 waves split the region 2 x 2,
+// wave w taking the (BH/2) x (BW/2) block at block position (w / 2, w % 2).
+// A brick's offset from its block's first brick is a compile-time constant, so
+// a per-brick LDS address is one register plus an immediate, and a wave walks
+// each row of its block left to right (BlockRows<BH, BW>::RW bricks per row:
+// every brick but a row's first reuses its left neighbour's fragment).
+template <int BH, int BW>
+struct BlockRows {
+    static_assert(BH % 2 == 0 && BW % 2 == 0, "brick region must split over the 4 waves");
+    static constexpr bool WHOLE = BH % 4 == 0;
+    static constexpr int RW = WHOLE ? BW : BW / 2;  // bricks per row of a wave's block
+};
+
 template <int BH, int BW, int BASE>
 __device__ __forceinline__ void block_brick(int wave, int q, int &br, int &bc) {
-    static_assert(BH % 2 == 0 && BW % 2 == 0, "brick region must split 2 x 2 over the waves");
-    br = BASE + (wave >> 1) * (BH / 2) + q / (BW / 2);
-    bc = BASE + (wave & 1) * (BW / 2) + q % (BW / 2);
+    using B = BlockRows<BH, BW>;
+    if constexpr (B::WHOLE) {
+        br = BASE + wave * (BH / 4) + q / BW;
+        bc = BASE + q % BW;
+    } else {
+        br = BASE + (wave >> 1) * (BH / 2) + q / (BW / 2);
+        bc = BASE + (wave & 1) * (BW / 2) + q % (BW / 2);
+    }
 }
 
 // Operand look-ahead across bricks costs 4*NZ registers (8*NZ for SPLIT); it
@@ -849,10 +864,10 @@ template <int TH, int TW, int NPL, bool STRIP = false>
 using FusedGeom = Geom<TH + 2, TW + 2, NPL, STRIP>;
 
 // One sweep of the fused image but the last: every brick is computed into
-// registers, then, after a barrier, written back in place.  Work per wave: a
-// TH/2 x BW/2 block of the core band (image brick rows 1..TH, every column,
-// the left and right ring bricks included), then its share of the ring rows
-// (image brick rows 0 and TH+1):
+// registers, then, after a barrier, written back in place.  Work per wave: its
+// block of the core band (image brick rows 1..TH, every column, the left and
+// right ring bricks included; block_brick: TH/4 whole rows, or for TH = 2 one
+// half row), then its share of the ring rows (image brick rows 0 and TH+1):
 //   !PAIRED  whole bricks: waves 0, 1 the top row, waves 2, 3 the bottom row,
 //            BW/2 columns each;
 //   PAIRED   (the sweep's results are needed only within 8 points of the core)
@@ -887,8 +902,8 @@ __device__ __forceinline__ void fused_sweep(const SweepArgs &a, uint16_t *lds, c
         pair = false;
         valid = true;
         if (q < QB) {
-            br = 1 + (wave >> 1) * (TH / 2) + q / HB;
-            bc = (wave & 1) * HB + q % HB;
+            block_brick<TH, G::BW, 0>(wave, q, br, bc);  // the core band: image brick rows 1..TH
+            br += 1;
         } else if constexpr (!PAIRED) {
             br = (wave >> 1) ? TH + 1 : 0;
             bc = (wave & 1) * HB + (q - QB);
@@ -902,7 +917,7 @@ __device__ __forceinline__ void fused_sweep(const SweepArgs &a, uint16_t *lds, c
     };
     // Unit q follows its left neighbour (same lane rows) in this wave's order,
     // so its fragment 0 is the previous unit's fragment 1.
-    auto reuse = [&](int q) { return q < QB ? q % HB != 0 : q > QB; };
+    auto reuse = [&](int q) { return q < QB ? q % BlockRows<TH, G::BW>::RW != 0 : q > QB; };
     u32x2 *l64 = reinterpret_cast<u32x2 *>(lds);
     u32x2 res[QT][NPL];
     O cur, nxt;
@@ -1021,7 +1036,7 @@ __device__ __forceinline__ void fused_tile(const SweepArgs &a, const CoefWindows
     }
     // block_brick walks the wave's block row by row: brick q follows its left
     // neighbour unless it starts a row.
-    auto reuse = [](int q) { return q % (TW / 2) != 0; };
+    auto reuse = [](int q) { return q % BlockRows<TH, TW>::RW != 0; };
 #pragma unroll
     for (int q = 0; q < QC; ++q) {
         int br, bc;

@@ -494,10 +494,18 @@ def mfma_edge(mode, src, rows, cols, ld, L: Layout, TH, TW, coeff5, rs, vec, inp
     return out, written
 
 
+def _block_rw(BH, BW):
+    """BlockRows<BH, BW>::RW: bricks per row of a wave's block."""
+    return BW if BH % 4 == 0 else BW // 2
+
+
 def _block_brick(BH, BW, base, wave, q):
-    """block_brick of tcs_sweep.hip: the 4 waves split a BH x BW brick region
-    2 x 2; wave w takes the (BH/2) x (BW/2) block at (w // 2, w % 2)."""
+    """block_brick of tcs_sweep.hip: with BH a multiple of 4 wave w takes BH/4
+    whole rows from row w * BH/4; else the 4 waves split a BH x BW brick region
+    2 x 2, wave w taking the (BH/2) x (BW/2) block at (w // 2, w % 2)."""
     assert BH % 2 == 0 and BW % 2 == 0
+    if BH % 4 == 0:
+        return base + wave * (BH // 4) + q // BW, base + q % BW
     hb, wb = BH // 2, BW // 2
     return base + (wave >> 1) * hb + q // wb, base + (wave & 1) * wb + q % wb
 
@@ -528,8 +536,8 @@ def mfma_fused(inp, L: Layout, TH, TW, coeff5, rs, nsweeps, round_coeff=f16):
                 prev = None
                 for q in range(qt):
                     if q < QB:
-                        br = 1 + (wave >> 1) * (TH // 2) + q // HB
-                        bc, row, pair = (wave & 1) * HB + q % HB, m, False
+                        br, bc = _block_brick(TH, BW, 0, wave, q)
+                        br, row, pair = br + 1, m, False
                     elif not paired:
                         br = TH + 1 if wave >> 1 else 0
                         bc, row, pair = (wave & 1) * HB + (q - QB), m, False
@@ -546,7 +554,7 @@ def mfma_fused(inp, L: Layout, TH, TW, coeff5, rs, nsweeps, round_coeff=f16):
                     frags = []
                     o = _mfma_brick(lds, G, afrag, ZLO, ZHI, br, bc, edge, er0 + 16 * br + row,
                                     ec0 + 16 * bc + 4 * g, L, rs, row=row, frags_out=frags)
-                    if (q % HB != 0) if q < QB else (q > QB):  # tcs_sweep.hip fused_sweep's reuse(q)
+                    if (q % _block_rw(TH, BW) != 0) if q < QB else (q > QB):  # fused_sweep's reuse(q)
                         _check_reuse(frags, prev)
                     prev = frags
                     res.append((br, bc, row, o))
@@ -567,7 +575,7 @@ def mfma_fused(inp, L: Layout, TH, TW, coeff5, rs, nsweeps, round_coeff=f16):
                 gr, gcb = er0 + 16 * br + m, ec0 + 16 * bc + 4 * g
                 frags = []
                 o = _mfma_brick(lds, G, afrag, ZLO, ZHI, br, bc, edge, gr, gcb, L, rs, frags_out=frags)
-                if q % (TW // 2) != 0:  # fused_tile's reuse(q)
+                if q % _block_rw(TH, TW) != 0:  # fused_tile's reuse(q)
                     _check_reuse(frags, prev)
                 prev = frags
                 _store(out, written, L, gr, gcb, o)
