@@ -232,6 +232,21 @@ int tcs_split_scale_exp(double maxabs);
  * max|x| * S^sweeps (growth capped at 2^60), keeping every stored value below
  * binary16's largest finite value.  With S <= 1 it equals tcs_split_scale_exp. */
 int tcs_split_scale_exp_for(const tcs_problem *p, double maxabs, int sweeps);
+/* Per-grid scale exponent e for a run of `sweeps` sweeps of problem p in its
+ * precision, given max|x| of the input (the pack and unpack calls multiply by
+ * 2^e and 2^-e, exact powers of two):
+ *   SPLIT  tcs_split_scale_exp_for;
+ *   FP16   max|x| * S^sweeps brought to [2^14, 2^15) (S = the taps' absolute sum
+ *          when above 1, growth capped at 2^60): values beyond binary16's
+ *          65504 stay finite and values below its normal range (6.1e-5) keep
+ *          full precision; results for values already in the normal range are
+ *          those of e = 0 (the rounding is scale-free there);
+ *   BF16   0 (the FP32 exponent range).
+ * The host drivers (tcs_host_sweep, host_<stencil>) apply it; device callers
+ * may pass it to tcs_pack_f32 / tcs_unpack_f32 / tcs_run_f32.  The reference
+ * converts to binary16 unscaled (common/common.hpp:146-160, array_float2half),
+ * so |x| > 65504 becomes Inf there.  No counterpart otherwise. */
+int tcs_scale_exp_for(const tcs_problem *p, double maxabs, int sweeps);
 
 /* ----------------------------------------------------------------- sweeping */
 /* One sweep d_in -> d_out (distinct storage buffers of this problem; d_out's

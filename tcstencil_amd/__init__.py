@@ -8,12 +8,12 @@ from ._lib import (BOX2D1R, BOX2D2R, METHOD_DIRECT, METHOD_MFMA, PREC_BF16, PREC
                    PREC_SPLIT, STAR2D1R, STAR2D2R, STENCIL_NAMES, TcsError, lib)
 from .api import (Batch, Graph, Problem, copy, error_norms, gen_coeffs, gen_sinusoid, gen_uniform,
                   host_sweep, maxabs, oracle_sweep_f64, oracle_sweeps_f64, params, radius,
-                  round_to, split_scale_exp)
+                  round_to, scale_exp, split_scale_exp)
 
 __all__ = [
     "BOX2D1R", "BOX2D2R", "STAR2D1R", "STAR2D2R", "METHOD_MFMA", "METHOD_DIRECT",
     "PREC_FP16", "PREC_BF16", "PREC_SPLIT", "STENCIL_NAMES", "TcsError", "lib",
     "Batch", "Graph", "Problem", "copy", "error_norms", "gen_coeffs", "gen_sinusoid", "gen_uniform",
     "host_sweep", "maxabs", "oracle_sweep_f64", "oracle_sweeps_f64", "params", "radius",
-    "round_to", "split_scale_exp",
+    "round_to", "scale_exp", "split_scale_exp",
 ]

@@ -101,6 +101,7 @@ SIGNATURES = {
     "tcs_maxabs_f64": (_i, [_vp, _i, _i, _i64, _vp, _vp]),
     "tcs_split_scale_exp": (_i, [ctypes.c_double]),
     "tcs_split_scale_exp_for": (_i, [_prob, ctypes.c_double, _i]),
+    "tcs_scale_exp_for": (_i, [_prob, ctypes.c_double, _i]),
     "tcs_sweep": (_i, [_prob, _vp, _vp, _vp]),
     "tcs_run": (_i, [_prob, _vp, _vp, _i, _vp]),
     "tcs_run_f32": (_i, [_prob, _vp, _vp, _i64, _vp, _vp, _i, _i, _vp]),

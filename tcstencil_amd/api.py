@@ -396,6 +396,14 @@ def split_scale_exp(mx: float, problem: "Problem | None" = None, sweeps: int = 1
     return int(lib().tcs_split_scale_exp_for(problem.ref, mx, sweeps))
 
 
+def scale_exp(problem: "Problem", mx: float, sweeps: int = 1) -> int:
+    """The per-grid scale exponent of a run of `sweeps` sweeps of `problem` in
+    its precision, given max|x| = mx (tcs_scale_exp_for): FP16 brings
+    mx * growth into [2^14, 2^15), SPLIT as split_scale_exp, BF16 0.  Pass it
+    to pack / unpack / run_f32."""
+    return int(lib().tcs_scale_exp_for(problem.ref, mx, sweeps))
+
+
 def gen_uniform(rows: int, cols: int, seed: int, stream_id: int = 0, lo=-1.0, hi=1.0,
                 dtype=None, device="cuda", stream=None):
     torch = _torch()

@@ -339,14 +339,28 @@ extern "C" int64_t tcs_buffer_bytes(const tcs_problem *p) {
 // must stay below binary16's 65504 after scaling: with taps of absolute sum
 // S > 1 a sweep can grow max|x| by S, so the exponent is taken for
 // max|x| * max(1, S)^sweeps (growth capped at 2^60).
-extern "C" int tcs_split_scale_exp_for(const tcs_problem *p, double maxabs, int sweeps) {
-    if (!p) return tcs_split_scale_exp(maxabs);
+// The factor by which `sweeps` sweeps of p can grow max|x|: S^sweeps for taps
+// of absolute sum S > 1 (capped at 2^60), else 1.
+static double run_growth(const tcs_problem *p, int sweeps) {
     double s = 0.0;
     for (int k = 0; k < 25; ++k) s += fabs((double)p->coeff[k]);
     double growth = 1.0;
     for (int i = 0; i < sweeps && s > 1.0 && growth < 0x1p60; ++i) growth *= s;
-    if (growth > 0x1p60) growth = 0x1p60;
-    return tcs_split_scale_exp(maxabs * growth);
+    return growth > 0x1p60 ? 0x1p60 : growth;
+}
+
+extern "C" int tcs_split_scale_exp_for(const tcs_problem *p, double maxabs, int sweeps) {
+    if (!p) return tcs_split_scale_exp(maxabs);
+    return tcs_split_scale_exp(maxabs * run_growth(p, sweeps));
+}
+
+extern "C" int tcs_scale_exp_for(const tcs_problem *p, double maxabs, int sweeps) {
+    if (!p) return 0;
+    if (p->precision == TCS_PREC_SPLIT) return tcs_split_scale_exp_for(p, maxabs, sweeps);
+    if (p->precision != TCS_PREC_FP16) return 0;
+    const double grown = maxabs * run_growth(p, sweeps);
+    if (!(grown > 0.0) || !isfinite(grown)) return 0;
+    return 14 - ilogb(grown);  // grown * 2^e in [2^14, 2^15)
 }
 
 extern "C" int tcs_split_scale_exp(double maxabs) {
@@ -620,13 +634,15 @@ static int host_sweep_impl(int stencil, int method, int precision, const float *
     TCS_TRY(stage.init(grid_bytes));
     TCS_TRY(staged_h2d(d_src, h_A, grid_bytes, s, stage));
 
+    // Per-grid scaling (FP16 and SPLIT storage): a power of two that keeps the
+    // run inside binary16's range.
     int scale_exp = 0;
-    if (precision == TCS_PREC_SPLIT) {
+    if (precision == TCS_PREC_SPLIT || precision == TCS_PREC_FP16) {
         TCS_TRY(tcs_maxabs_f32((const float *)d_src, rows, cols, cols, (double *)d_max, s));
         double mx = 0.0;
         TCS_HIP_CHECK(hipMemcpyAsync(&mx, d_max, sizeof mx, hipMemcpyDeviceToHost, s));
         TCS_HIP_CHECK(hipStreamSynchronize(s));
-        scale_exp = tcs_split_scale_exp_for(&p, mx, sweeps);
+        scale_exp = tcs_scale_exp_for(&p, mx, sweeps);
     }
     TCS_TRY(tcs_pack_f32(&p, (const float *)d_src, cols, buf0, scale_exp, s));
     TCS_TRY(tcs_clear(&p, buf1, s));

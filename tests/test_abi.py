@@ -147,6 +147,33 @@ def test_split_scale_exponent_covers_tap_growth(oracle):
     assert T.split_scale_exp(0.7, q, 10) == T.split_scale_exp(0.7)
 
 
+def test_scale_exponent_per_precision(oracle):
+    """tcs_scale_exp_for: FP16 brings max|x| times the run's tap growth into
+    [2^14, 2^15), so the largest stored value stays below binary16's 65504
+    and values far below its normal range (2^-14) are lifted into it; SPLIT is
+    split_scale_exp; BF16, and inputs that are zero or not finite, get 0."""
+    import tcstencil_amd as T
+    B = T.gen_coeffs("box2d2r", 3)
+    p16 = T.Problem("box2d2r", 64, 64, B, precision="fp16")
+    for mx in (1.0, 0.75, 3e-7, 1e-3, 7e4, 1e6, 3.4e30):
+        e = T.scale_exp(p16, mx)
+        assert 2 ** 14 <= mx * 2.0 ** e < 2 ** 15, (mx, e)
+        assert mx * 2.0 ** e < 65504.0
+    assert T.scale_exp(p16, 0.0) == 0 and T.scale_exp(p16, float("nan")) == 0 and T.scale_exp(p16, float("inf")) == 0
+    # taps of absolute sum S > 1 (the reference's init_B) grow the run's values
+    Bi = oracle.init_B("star2d2r")
+    q16 = T.Problem("star2d2r", 64, 64, Bi, precision="fp16")
+    S = float(np.abs(np.array(q16.s.coeff, dtype=np.float64)).sum())
+    for sweeps in (1, 2):
+        e = T.scale_exp(q16, 1.0, sweeps)
+        assert 2 ** 14 <= S ** sweeps * 2.0 ** e < 2 ** 15
+    ps = T.Problem("box2d2r", 64, 64, B, precision="split")
+    for mx in (1.0, 1e-3, 1e5):
+        assert T.scale_exp(ps, mx, 3) == T.split_scale_exp(mx, ps, 3)
+    pb = T.Problem("box2d2r", 64, 64, B, precision="bf16")
+    assert T.scale_exp(pb, 1e6) == 0 and T.scale_exp(pb, 1e-9) == 0
+
+
 def test_gen_coeffs_normalised():
     import tcstencil_amd as T
     for s in T.STENCIL_NAMES:

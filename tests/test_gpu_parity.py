@@ -377,6 +377,35 @@ def test_host_sweep_vs_gold(T, torch, oracle, stencil):
     assert np.abs(C - want)[r:-r, r:-r].max() <= 1e-5
 
 
+@pytest.mark.parametrize("amp", [3e5, 1e-6])
+def test_host_sweep_fp16_per_grid_scaling(T, oracle, amp):
+    """FP16 host driver on inputs beyond binary16's range (|x| up to 3e5 >
+    65504, Inf without scaling) and far below its normal range (|x| ~ 1e-6,
+    subnormal without scaling: 6% relative steps): the per-grid power-of-two
+    scale (tcs_scale_exp_for) keeps the relative error that of normal-range
+    FP16.  Bound as for the reference fixtures, in units of max|x|:
+    u (2 sum|B| max|A| + max|C|) + g sum|B| max|A|, u = 2^-11, g = 2n 2^-24."""
+    say("case host_sweep fp16 scaling", amp)
+    rng = np.random.default_rng(31)
+    stencil = "box2d2r"
+    A = (rng.uniform(-1, 1, (200, 180)) * amp).astype(np.float32)
+    B = T.gen_coeffs(stencil, 31)
+    gold = oracle.sweep_f32(stencil, A, B)
+    r = RADIUS[stencil]
+    u, g = 2.0 ** -11, 2 * 25 * 2.0 ** -24
+    sb, ma, mc = float(np.abs(B).sum()), float(np.abs(A).max()), float(np.abs(gold).max())
+    bound = u * (2 * sb * ma + mc) + g * sb * ma
+    for method in ("mfma", "direct"):
+        C, _ = T.host_sweep(stencil, A, B, method=method)
+        assert np.isfinite(C).all()
+        err = float(np.abs(C.astype(np.float64) - gold)[r:-r, r:-r].max())
+        say(f"  {method}: max |out - gold| {err:.3e} (bound {bound:.3e})")
+        assert err <= bound
+        ring = np.ones_like(A, bool)
+        ring[r:-r, r:-r] = False
+        assert np.array_equal(C[ring], A[ring])
+
+
 @pytest.mark.parametrize("method", ["mfma", "direct"])
 @pytest.mark.parametrize("stencil", STENCILS)
 def test_dropin_driver_binary(stencil, method):
