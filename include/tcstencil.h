@@ -355,6 +355,11 @@ int tcs_gen_coeffs(int stencil, uint64_t seed, float *B, int n);
 const char *tcs_kernel_name(const tcs_problem *p);
 /* Kernel name of the fused (temporally blocked) launch for this problem, or NULL. */
 const char *tcs_fused_kernel_name(const tcs_problem *p);
+/* Kernel name of tcs_run_f32's sweep with the layout transforms fused in (mode 1:
+ * row-major FP32 in, storage out; 2: storage in, row-major FP32 out; 3: both, a
+ * one-sweep run), or NULL when tcs_run_f32 runs the three steps instead.  Its
+ * tile may differ from the problem's (results do not depend on the tile). */
+const char *tcs_edge_kernel_name(const tcs_problem *p, int mode);
 /* MFMA instructions issued per 16x16 brick, or per 2x2-brick block with 32x32
  * fragments (0 for the direct method). */
 int tcs_mfma_per_brick(const tcs_problem *p);

@@ -125,6 +125,7 @@ SIGNATURES = {
     "tcs_gen_coeffs": (_i, [_i, _u64, _fp, _i]),
     "tcs_kernel_name": (ctypes.c_char_p, [_prob]),
     "tcs_fused_kernel_name": (ctypes.c_char_p, [_prob]),
+    "tcs_edge_kernel_name": (ctypes.c_char_p, [_prob, _i]),
     "tcs_mfma_per_brick": (_i, [_prob]),
     "tcs_mfma_flops_per_brick": (_i64, [_prob]),
 }

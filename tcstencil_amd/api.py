@@ -157,6 +157,12 @@ class Problem:
         n = lib().tcs_fused_kernel_name(self.ref)
         return n.decode() if n else ""
 
+    def edge_kernel_name(self, mode: int = 3) -> str:
+        """The kernel run_f32 uses for a sweep with the layout transforms fused
+        in (mode 1 first, 2 last, 3 the only sweep), "" when it runs three steps."""
+        n = lib().tcs_edge_kernel_name(self.ref, mode)
+        return n.decode() if n else ""
+
     @property
     def tile(self):
         return (self.s.tile_h, self.s.tile_w)

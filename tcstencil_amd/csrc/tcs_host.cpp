@@ -372,6 +372,9 @@ extern "C" int tcs_split_scale_exp(double maxabs) {
 
 extern "C" const char *tcs_kernel_name(const tcs_problem *p) { return p ? sweep_kernel_name(p) : nullptr; }
 extern "C" const char *tcs_fused_kernel_name(const tcs_problem *p) { return p ? fused_kernel_name(p) : nullptr; }
+extern "C" const char *tcs_edge_kernel_name(const tcs_problem *p, int mode) {
+    return p && mode >= 1 && mode <= 3 ? edge_kernel_name(p, mode) : nullptr;
+}
 extern "C" int tcs_mfma_per_brick(const tcs_problem *p) { return p ? mfma_per_brick(p) : 0; }
 extern "C" int64_t tcs_mfma_flops_per_brick(const tcs_problem *p) { return p ? mfma_flops_per_brick(p) : 0; }
 

@@ -100,6 +100,7 @@ int launch_sweep(const tcs_problem *p, const void *d_in, void *d_out, hipStream_
 // storage d_out, mode 2 reads storage d_in and writes row-major FP32 dst (ring
 // from src), mode 3 both.  edge_supported: a kernel exists for the problem.
 bool edge_supported(const tcs_problem *p);
+const char *edge_kernel_name(const tcs_problem *p, int mode);
 int launch_edge(const tcs_problem *p, int mode, const void *d_in, void *d_out, const float *src, float *dst,
                 int64_t ld, int scale_exp, hipStream_t stream);
 int launch_fused(const tcs_problem *p, const void *d_in, void *d_out, int nsweeps, hipStream_t stream);

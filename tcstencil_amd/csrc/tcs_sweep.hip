@@ -1771,6 +1771,7 @@ struct BatchEntry {
 struct EdgeEntry {
     int prec, radius, th, tw, mode;
     sweep_fn fn;
+    const char *name;
 };
 
 struct PartTables {
@@ -1837,12 +1838,14 @@ struct PartTables {
 #define TCS_BATCH32_PREC(P) \
     TCS_BATCH32(P, false, false), TCS_BATCH32(P, true, false), TCS_BATCH32(P, false, true), TCS_BATCH32(P, true, true)
 
-#define TCS_EDGE(TH, TW, ZLO, ZHI, P, R, M) {P, R, TH, TW, M, tcs_mfma_edge<TH, TW, ZLO, ZHI, P, M>}
+#define TCS_EDGE(TH, TW, ZLO, ZHI, P, R, M)                          \
+    {P, R, TH, TW, M, tcs_mfma_edge<TH, TW, ZLO, ZHI, P, M>,          \
+     "tcs_mfma_edge<" #TH ", " #TW ", " #ZLO ", " #ZHI ", " #P ", " #M ">"}
 #define TCS_EDGE_MODES(TH, TW, ZLO, ZHI, P, R) \
     TCS_EDGE(TH, TW, ZLO, ZHI, P, R, 1), TCS_EDGE(TH, TW, ZLO, ZHI, P, R, 2), TCS_EDGE(TH, TW, ZLO, ZHI, P, R, 3)
 #define TCS_EDGE_TILES(ZLO, ZHI, P, R)                                                                   \
     TCS_EDGE_MODES(2, 4, ZLO, ZHI, P, R), TCS_EDGE_MODES(4, 4, ZLO, ZHI, P, R), TCS_EDGE_MODES(2, 8, ZLO, ZHI, P, R), \
-        TCS_EDGE_MODES(4, 8, ZLO, ZHI, P, R)
+        TCS_EDGE_MODES(4, 8, ZLO, ZHI, P, R), TCS_EDGE_MODES(2, 16, ZLO, ZHI, P, R)
 #define TCS_EDGE_PREC(P) TCS_EDGE_TILES(0, 4, P, 2), TCS_EDGE_TILES(1, 3, P, 1)
 
 // The tables below spell kernel precision ids as literals (0 FP16, 1 BF16,
@@ -2050,7 +2053,10 @@ static const EdgeEntry *select_edge(const tcs_problem *p, int mode) {
 // divides the layout (brick results depend on neither).  SPLIT: 4x4 for tiles of 32 or more
 // bricks -- holding both planes' results for the staged output costs
 // registers (box2d2r 16384^2, one sweep: 4x8 0.62 ms, 2x8 0.61, 4x4 0.57, 2x4
-// 0.61).  FP16/BF16: 4x8 for 8x8 tiles (0.40 against 0.44 ms).
+// 0.61, 2x16 0.62, 1x16 0.62).  FP16/BF16: 2x16 where it fits (one sweep 0.38 ms
+// against 0.40 for 4x8, 0.40 for 1x16 and 0.44 for 8x8): the row-major FP32 side
+// streams 1 KB row segments instead of 512 bytes (the same access pattern as a
+// plain copy: 387 us for 4x8-brick tiles, 362 for 2x16, 322 linear).
 static tcs_problem edge_problem(const tcs_problem *p) {
     tcs_problem q = *p;
     q.wg_per_cu = 0;  // one tile per workgroup: persistence does not change results
@@ -2060,12 +2066,25 @@ static tcs_problem edge_problem(const tcs_problem *p) {
         q.tile_h = 4;
         q.tile_w = tw;
     }
+    // FP16/BF16 edges on 2x16 tiles (32 rows of 1 KB FP32 row segments) where the layout
+    // divides into them and they still make 4096 tiles or more.
+    const int64_t wide = (int64_t)(p->layout.nbr / 2) * (p->layout.nbc / 16);
+    if (!split && p->layout.nbr % 2 == 0 && p->layout.nbc % 16 == 0 && wide >= 4096) {
+        q.tile_h = 2;
+        q.tile_w = 16;
+    }
     return q;
 }
 
 bool edge_supported(const tcs_problem *p) {
     const tcs_problem q = edge_problem(p);
     return select_edge(&q, 3) != nullptr;
+}
+
+const char *edge_kernel_name(const tcs_problem *p, int mode) {
+    const tcs_problem q = edge_problem(p);
+    const EdgeEntry *k = select_edge(&q, mode);
+    return k ? k->name : nullptr;
 }
 
 int launch_edge(const tcs_problem *pp, int mode, const void *d_in, void *d_out, const float *src, float *dst,

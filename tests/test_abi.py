@@ -174,6 +174,24 @@ def test_scale_exponent_per_precision(oracle):
     assert T.scale_exp(pb, 1e6) == 0 and T.scale_exp(pb, 1e-9) == 0
 
 
+def test_edge_kernel_choice():
+    """tcs_run_f32's fused edge sweeps: FP16/BF16 on 2x16-brick tiles when the
+    layout divides into them and gives 4096 tiles or more, SPLIT on 4x4, small
+    grids on the problem's tile; modes outside 1..3 and the direct method have
+    no edge kernel (the three steps run)."""
+    import tcstencil_amd as T
+    B = T.gen_coeffs("box2d2r", 1)
+    big = T.Problem("box2d2r", 8192, 4096, B)
+    assert [big.edge_kernel_name(m) for m in (1, 2, 3)] == [
+        "tcs_mfma_edge<2, 16, 0, 4, 0, %d>" % m for m in (1, 2, 3)]
+    assert T.Problem("box2d2r", 8192, 4096, B, precision="bf16").edge_kernel_name(3) == "tcs_mfma_edge<2, 16, 0, 4, 1, 3>"
+    assert T.Problem("box2d2r", 8192, 4096, B, precision="split").edge_kernel_name(3) == "tcs_mfma_edge<4, 4, 0, 4, 2, 3>"
+    small = T.Problem("box2d2r", 1024, 1024, B)
+    assert small.edge_kernel_name(3) == "tcs_mfma_edge<%d, %d, 0, 4, 0, 3>" % small.tile
+    assert big.edge_kernel_name(0) == "" and big.edge_kernel_name(4) == ""
+    assert T.Problem("box2d2r", 8192, 4096, B, method="direct").edge_kernel_name(3) == ""
+
+
 def test_gen_coeffs_normalised():
     import tcstencil_amd as T
     for s in T.STENCIL_NAMES:

@@ -216,7 +216,7 @@ def test_emulated_frag32_fused_equals_repeated_sweeps(oracle, stencil, case):
 
 
 EDGE_CASES = [(70, 150, 150, 2, 4), (70, 152, 152, 4, 8), (33, 17, 20, 2, 4), (64, 128, 132, 2, 8), (5, 3, 4, 4, 4),
-              (40, 61, 64, 4, 4)]
+              (40, 61, 64, 4, 4), (37, 250, 252, 2, 16)]
 
 
 @pytest.mark.parametrize("stencil", STENCILS)

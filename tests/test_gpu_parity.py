@@ -802,6 +802,29 @@ def test_run_f32_fused_edges_bitwise(T, torch, precision):
                         assert torch.equal(got, want), (stencil, rows, cols, off, tile, sweeps, fuse)
 
 
+@pytest.mark.parametrize("precision", ["fp16", "bf16"])
+def test_run_f32_wide_edge_tiles(T, torch, precision):
+    """Large FP16/BF16 grids run tcs_run_f32's first, last and only sweeps on
+    2x16-brick tiles (rows of 1 KB FP32 segments), the middle sweeps on the
+    problem's own tile: bitwise the three steps, with ragged edges (rows and
+    columns not multiples of 16), 1-3 sweeps and fused middle sweeps."""
+    rows, cols = 8187, 4090
+    for stencil in ("box2d2r", "star2d1r"):
+        B = T.gen_coeffs(stencil, 9)
+        p = T.Problem(stencil, rows, cols, B, precision=precision)
+        for mode in (1, 2, 3):
+            assert p.edge_kernel_name(mode).startswith("tcs_mfma_edge<2, 16,"), p.edge_kernel_name(mode)
+        src = T.gen_uniform(rows, cols, seed=9, dtype=torch.float32)
+        b0, b1 = p.alloc(), p.alloc()
+        for sweeps, fuse in ((1, 1), (2, 1), (3, 1), (5, 3)):
+            say("case run_f32 wide edges", stencil, precision, rows, cols, sweeps, fuse)
+            p.fuse = fuse
+            got = p.run_f32(src, sweeps, buf0=b0, buf1=b1)
+            want = three_steps(T, torch, p, src, sweeps)
+            assert torch.equal(got, want), (stencil, sweeps, fuse)
+        del b0, b1, src
+
+
 def test_run_f32_tiny_grids(T, torch, oracle):
     """Grids smaller than one brick or one tile, in one or both directions:
     tcs_run_f32 equals pack -> run -> unpack, and one sweep equals the oracle
