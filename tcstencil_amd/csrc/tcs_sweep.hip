@@ -626,14 +626,6 @@ __device__ __forceinline__ void mfma_brick(const SweepArgs &a, const uint16_t *l
     brick_out<G, PREC>(a, lds, mfma_acc<G, ZLO, ZHI, PREC>(C, op), br, bc, m, g, edge, gr, gcb, w);
 }
 
-// Brick q of a wave's list -> (br, bc) in the LDS image: bricks dealt to the
-// 4 waves round robin, b = wave + 4q (one-pass kernels).
-template <int BW, int BASE>
-__device__ __forceinline__ void brick_of(int b, int &br, int &bc) {
-    br = BASE + b / BW;
-    bc = BASE + (b - (b / BW) * BW);
-}
-
 // Brick q of a wave's share of a BH x BW brick region (both even) whose first
 // brick is (BASE, BASE), for the sweep loops of the fused kernels, q row-major
 // within the wave's block: with BH a multiple of 4 wave w takes whole rows
@@ -757,23 +749,26 @@ __device__ __forceinline__ void sweep_tile16(const SweepArgs &a) {
         const bool edge = !tile_is_interior(a, cgr0, cgc0, 16 * TH, 16 * TW);
         // With LOOKAHEAD, operand reads run one brick ahead of the MFMAs.
         Ops<T, ZLO, ZHI, PREC> cur, nxt;
+        // Each wave walks its block of the tile row by row (block_brick), so a
+        // brick takes its first operand fragment from its left neighbour.
+        auto reuse = [](int q) { return q % BlockRows<TH, TW>::RW != 0; };
         if constexpr (LOOKAHEAD) {
             int br, bc;
-            brick_of<TW, 0>(wave, br, bc);
+            block_brick<TH, TW, 0>(wave, 0, br, bc);
             cur.load(lds, br, bc, off);
         }
 #pragma unroll
         for (int q = 0; q < NB; ++q) {
             int br, bc;
-            brick_of<TW, 0>(wave + 4 * q, br, bc);
+            block_brick<TH, TW, 0>(wave, q, br, bc);
             if constexpr (LOOKAHEAD) {
                 if (q + 1 < NB) {
                     int nbr, nbc;
-                    brick_of<TW, 0>(wave + 4 * (q + 1), nbr, nbc);
-                    nxt.load(lds, nbr, nbc, off);
+                    block_brick<TH, TW, 0>(wave, q + 1, nbr, nbc);
+                    nxt.load(lds, nbr, nbc, off, cur, reuse(q + 1));
                 }
             } else {
-                cur.load(lds, br, bc, off);
+                cur.load(lds, br, bc, off, cur, reuse(q));
             }
             __builtin_amdgcn_sched_barrier(0);
             const int gr = cgr0 + 16 * br + m, gcb = cgc0 + 16 * bc + 4 * g;
@@ -802,7 +797,7 @@ __device__ __forceinline__ void sweep_tile16(const SweepArgs &a) {
 #pragma unroll
             for (int q = 0; q < NB; ++q) {
                 int br, bc;
-                brick_of<TW, 0>(wave + 4 * q, br, bc);
+                block_brick<TH, TW, 0>(wave, q, br, bc);
 #pragma unroll
                 for (int p = 0; p < NPL; ++p)  // plane p's image over input plane p
                     *reinterpret_cast<u32x2 *>(lds + p * T::PLANE + (16 * br + m) * OP + 16 * bc + 4 * g) =
@@ -1408,23 +1403,24 @@ __device__ __forceinline__ void batch_tile(const SweepArgs &a, const CoefWindows
     const bool edge = !tile_is_interior(a, gr0, gc0, 16 * TH, 16 * TW);
     constexpr int NB = TH * TW / 4;
     Ops<T, ZLO, ZHI, PREC> cur, nxt;
+    auto reuse = [](int q) { return q % BlockRows<TH, TW>::RW != 0; };
     if constexpr (LOOKAHEAD) {
         int br, bc;
-        brick_of<TW, 0>(wave, br, bc);
+        block_brick<TH, TW, 0>(wave, 0, br, bc);
         cur.load(lds, br, bc, off);
     }
 #pragma unroll
     for (int q = 0; q < NB; ++q) {
         int br, bc;
-        brick_of<TW, 0>(wave + 4 * q, br, bc);
+        block_brick<TH, TW, 0>(wave, q, br, bc);
         if constexpr (LOOKAHEAD) {
             if (q + 1 < NB) {
                 int nbr, nbc;
-                brick_of<TW, 0>(wave + 4 * (q + 1), nbr, nbc);
-                nxt.load(lds, nbr, nbc, off);
+                block_brick<TH, TW, 0>(wave, q + 1, nbr, nbc);
+                nxt.load(lds, nbr, nbc, off, cur, reuse(q + 1));
             }
         } else {
-            cur.load(lds, br, bc, off);
+            cur.load(lds, br, bc, off, cur, reuse(q));
         }
         __builtin_amdgcn_sched_barrier(0);
         const int gr = gr0 + 16 * br + m, gcb = gc0 + 16 * bc + 4 * g;

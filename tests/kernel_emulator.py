@@ -355,11 +355,15 @@ def mfma_sweep(inp, L: Layout, TH, TW, coeff5, rs, round_coeff=f16, per_xcd=0):
         lds = stage_tile(inp, L, G, gr0, gc0)
         edge = not (gr0 >= rs and gr0 + 16 * TH <= L.rows - rs and gc0 >= rs and gc0 + 16 * TW <= L.cols - rs)
         for wave in range(4):
+            prev = None
             for q in range(TH * TW // 4):
-                b = wave + 4 * q
-                br, bc = b // TW, b % TW
+                br, bc = _block_brick(TH, TW, 0, wave, q)
                 gr, gcb = gr0 + 16 * br + m, gc0 + 16 * bc + 4 * g
-                o = _mfma_brick(lds, G, afrag, ZLO, ZHI, br, bc, edge, gr, gcb, L, rs)
+                frags = []
+                o = _mfma_brick(lds, G, afrag, ZLO, ZHI, br, bc, edge, gr, gcb, L, rs, frags_out=frags)
+                if q % _block_rw(TH, TW) != 0:  # sweep_tile16's reuse(q)
+                    _check_reuse(frags, prev)
+                prev = frags
                 _store(out, written, L, gr, gcb, o)
     return out, written
 
