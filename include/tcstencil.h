@@ -150,8 +150,9 @@ int tcs_problem_init(tcs_problem *p, int stencil, int method, int precision, int
                      const float *B, int n);
 int tcs_radius(int stencil);
 /* Choose the workgroup tile (bricks, rows x cols): 2x4, 4x4, 2x8, 4x8 or 8x8.
- * With 16x16 fragments single-sweep and fused kernels exist for all five; a
- * missing combination (32x32 fragments: see tcs_problem_set_frag) fails at
+ * With 16x16 fragments single-sweep and fused kernels exist for all five
+ * (FP16/BF16 with rounded taps fuse 8x8 problems on 4x8 tiles, which measure
+ * faster at every fused depth); a missing combination (32x32 fragments: see tcs_problem_set_frag) fails at
  * launch with TCS_ERR_UNSUPPORTED (tcs_kernel_name / tcs_fused_kernel_name
  * return NULL for it).  Re-derives the layout's alignment, so buffers packed
  * for another tile must be packed again. */
@@ -289,7 +290,8 @@ int tcs_graph_destroy(void *graph_exec);
  * on a grid of one workgroup per tile or, with wg_per_cu > 0, a persistent grid
  * of wg_per_cu workgroups per CU.  A sweep of the whole batch is a handful of
  * launches instead of one per grid.  All problems: MFMA, the same precision
- * (FP16 or BF16); tiles 2x4, 4x4 or 4x8 (16x16 fragments), 4x4 or 4x8 (32x32).
+ * (FP16 or BF16); tiles 2x4, 4x4 or 4x8 (16x16 fragments), 4x4 or 4x8 (32x32);
+ * 8x8 problems run on 4x8 tiles (same storage layout, same results).
  * d_buf0[i] / d_buf1[i] are problem i's two storage buffers (d_buf1 ghost ring
  * zero); after tcs_batch_run the results are in buf[sweeps % 2].  The batch owns
  * device copies of the descriptors (allocated here, freed by tcs_batch_destroy).

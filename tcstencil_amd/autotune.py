@@ -24,6 +24,8 @@ import itertools
 import json
 import sys
 
+from .timing import warm_up
+
 TILES = [(2, 4), (4, 4), (2, 8), (4, 8), (8, 8)]
 
 
@@ -48,8 +50,7 @@ def tune(stencil, rows, cols, method="mfma", precision="fp16", sweeps=20, warmup
         b0, b1 = p.alloc(), p.alloc()
         p.clear(b1)
         p.pack(x, b0, e)
-        p.run(b0, b1, warmup)
-        torch.cuda.synchronize()
+        warm_up(lambda: p.run(b0, b1, max(2, warmup)))
         ev = [torch.cuda.Event(enable_timing=True) for _ in range(2)]
         ev[0].record()
         p.run(b0, b1, sweeps)
@@ -163,8 +164,7 @@ def time_batch(batch, sweeps, reps=3):
     """ms for `sweeps` sweeps of the batch, replayed as one hipGraph."""
     import torch
     g = batch.graph(sweeps)
-    g.launch()
-    torch.cuda.synchronize()
+    warm_up(g.launch)
     ev = [torch.cuda.Event(enable_timing=True) for _ in range(2)]
     ev[0].record()
     for _ in range(reps):

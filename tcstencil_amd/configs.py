@@ -29,6 +29,8 @@ import sys
 
 import numpy as np
 
+from .timing import warm_up
+
 HBM_PEAK_GBS = 8000.0
 BYTES = {"fp16": 4.0, "bf16": 4.0, "split": 8.0}
 SEED = 1
@@ -73,8 +75,7 @@ def _fp64_refs(stencil, x64, B, counts):
 
 def _timed_run(p, b0, b1, sweeps, warmup=2):
     torch = _torch()
-    p.run(b0, b1, warmup + warmup % 2)
-    torch.cuda.synchronize()
+    warm_up(lambda: p.run(b0, b1, max(2, warmup + warmup % 2)))
     e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
     e0.record()
     res = p.run(b0, b1, sweeps)
@@ -97,12 +98,11 @@ def _errors(p, x, refs, scale_exp=0):
 
 
 def _timed_graph(p, b0, b1, sweeps):
-    """ms of one replay of the sweep loop captured as a hipGraph (after one
-    untimed replay)."""
+    """ms of one replay of the sweep loop captured as a hipGraph (after untimed
+    replays for the warm-up time)."""
     torch = _torch()
     g = p.graph(b0, b1, sweeps)
-    g.launch()
-    torch.cuda.synchronize()
+    warm_up(g.launch)
     e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
     e0.record()
     g.launch()
@@ -236,9 +236,7 @@ def c4(nprob=512, sweeps=20):
     (gst_f, ms_f, nl_f), probs, b0s, b1s = timed(lambda i: fixed[bucket[i]])
     # one launch per grid per sweep, the same work
     e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-    for p, b0, b1 in zip(probs, b0s, b1s):
-        p.run(b0, b1, 2)
-    torch.cuda.synchronize()
+    warm_up(lambda: [p.run(b0, b1, 2) for p, b0, b1 in zip(probs, b0s, b1s)])
     e0.record()
     for p, b0, b1 in zip(probs, b0s, b1s):
         p.run(b0, b1, sweeps)

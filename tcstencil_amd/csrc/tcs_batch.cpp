@@ -66,17 +66,20 @@ extern "C" int tcs_batch_create(tcs_batch **out, int nprob, const tcs_problem *p
         if (has_coef_lo(prec) && frag_of(p) != 16)
             return set_error(TCS_ERR_UNSUPPORTED, "batch: hi+lo coefficients need 16x16 fragments (problem %d)", i);
         const int frag = frag_of(p);
-        if (!batch_tile_offered(p.tile_h, p.tile_w, frag))
+        // 8x8 problems run on 4x8 tiles, which divide the same layout (brick
+        // results depend on neither); the batched kernels stop at 4x8.
+        const int th = p.tile_h == 8 && p.tile_w == 8 ? 4 : p.tile_h, tw = p.tile_w;
+        if (!batch_tile_offered(th, tw, frag))
             return set_error(TCS_ERR_UNSUPPORTED,
-                             "batch: problem %d tile %dx%d fragment %d (offered 2x4, 4x4, 4x8; 32x32: 4x4, 4x8)", i,
-                             p.tile_h, p.tile_w, frag);
+                             "batch: problem %d tile %dx%d fragment %d (offered 2x4, 4x4, 4x8, 8x8 as 4x8; "
+                             "32x32: 4x4, 4x8, 8x8 as 4x8)", i, p.tile_h, p.tile_w, frag);
         const int fuse = p.fuse > 1 ? p.fuse : 1;
         if (p.radius * (fuse - 1) > 16)
             return set_error(TCS_ERR_INVALID, "batch: problem %d fuses %d sweeps, radius %d allows 1..%d", i, fuse,
                              p.radius, 1 + 16 / p.radius);
         if (p.wg_per_cu < 0) return set_error(TCS_ERR_INVALID, "batch: problem %d has wg_per_cu < 0", i);
         const tcs_layout &L = p.layout;
-        if (L.nbr % p.tile_h || L.nbc % p.tile_w)
+        if (L.nbr % th || L.nbc % tw)
             return set_error(TCS_ERR_INVALID, "batch: problem %d layout is not a multiple of its tile", i);
         if (!d_buf0[i] || !d_buf1[i] || d_buf0[i] == d_buf1[i])
             return set_error(TCS_ERR_INVALID, "batch: problem %d needs two distinct buffers", i);
@@ -90,10 +93,10 @@ extern "C" int tcs_batch_create(tcs_batch **out, int nprob, const tcs_problem *p
         q.sbr = L.sbr;
         q.sbc = L.sbc;
         q.rs = p.radius;
-        q.tiles_x = L.nbc / p.tile_w;
+        q.tiles_x = L.nbc / tw;
         q.out_scale = ldexpf(1.0f, -p.coeff_exp);
         build_coef_windows(&p, &q.cw);
-        const Shape shape{p.tile_h, p.tile_w, frag, fuse, p.wg_per_cu};
+        const Shape shape{th, tw, frag, fuse, p.wg_per_cu};
         int gi = -1;
         for (size_t s = 0; s < shapes.size(); ++s)
             if (shapes[s] == shape) gi = (int)s;
@@ -102,7 +105,7 @@ extern "C" int tcs_batch_create(tcs_batch **out, int nprob, const tcs_problem *p
             tiles.emplace_back();
             gi = (int)shapes.size() - 1;
         }
-        const int nt = (L.nbr / p.tile_h) * q.tiles_x;
+        const int nt = (L.nbr / th) * q.tiles_x;
         for (int t = 0; t < nt; ++t) tiles[gi].push_back(BatchTile{i, t});
         const int r = p.radius;
         if (L.rows > 2 * r && L.cols > 2 * r) interior += (int64_t)(L.rows - 2 * r) * (L.cols - 2 * r);

@@ -207,6 +207,8 @@ static void default_tile(const tcs_problem *p, int *th, int *tw) {
     if (p->method == TCS_METHOD_MFMA)
         // 8x8 for 32x32 fragments and for hi+lo coefficients (4 waves per SIMD:
         // box2d2r 16384^2 FP16 193.8 us against 222.7 us on 4x8), else 4x8
+        // (warm GPU, box2d2r 16384^2: FP16 177.4 us against 180.0 on 8x8, BF16
+        // 177.5 against 180.0, SPLIT 358.6 against 383.5)
         first = frag32(p) || has_coef_lo(kernel_prec(p)) ? 0 : 1;
     else
         first = 0;  // direct kernels: 8x8 (16384^2, all stencils, both forms; 4x8 within 1-2%)

@@ -1816,8 +1816,11 @@ struct PartTables {
     {P, R, TH, TW, 16, tcs_mfma_fused<TH, TW, ZLO, ZHI, P>, "tcs_mfma_fused<" #TH ", " #TW ", " #ZLO ", " #ZHI ", " #P ">"}
 #define TCS_FUSED_TILES(ZLO, ZHI, P, R)                                                           \
     TCS_FUSED(2, 4, ZLO, ZHI, P, R), TCS_FUSED(4, 4, ZLO, ZHI, P, R), TCS_FUSED(2, 8, ZLO, ZHI, P, R), \
-        TCS_FUSED(4, 8, ZLO, ZHI, P, R), TCS_FUSED(8, 8, ZLO, ZHI, P, R)
+        TCS_FUSED(4, 8, ZLO, ZHI, P, R)
+#define TCS_FUSED_TILES88(ZLO, ZHI, P, R) TCS_FUSED_TILES(ZLO, ZHI, P, R), TCS_FUSED(8, 8, ZLO, ZHI, P, R)
+// FP16/BF16 with rounded taps fuse 8x8 problems on 4x8 tiles (fused_problem).
 #define TCS_FUSED_PREC(P) TCS_FUSED_TILES(0, 4, P, 2), TCS_FUSED_TILES(1, 3, P, 1)
+#define TCS_FUSED_PREC88(P) TCS_FUSED_TILES88(0, 4, P, 2), TCS_FUSED_TILES88(1, 3, P, 1)
 #define TCS_FUSED32(TH, TW, ZLO, ZHI, P, R) \
     {P, R, TH, TW, 32, tcs_mfma32_fused<TH, TW, ZLO, ZHI, P>, \
      "tcs_mfma32_fused<" #TH ", " #TW ", " #ZLO ", " #ZHI ", " #P ">"}
@@ -1852,9 +1855,9 @@ static_assert(TCS_STAR2D1R == 0 && TCS_STAR2D2R == 1 && TCS_BOX2D1R == 2 && TCS_
 
 // The 16x16-fragment kernels of one kernel precision: single sweep, fused,
 // edge, and (FP16/BF16, with or without hi+lo coefficients) batched.
-#define TCS_PART_16(NAME, P, BATCH...)                                                        \
+#define TCS_PART_16(NAME, P, FUSED, BATCH...)                                                 \
     static const KernelEntry kSweep##NAME[] = {TCS_MFMA_PREC(P)};                             \
-    static const FusedEntry kFused##NAME[] = {TCS_FUSED_PREC(P)};                             \
+    static const FusedEntry kFused##NAME[] = {FUSED};                                         \
     static const EdgeEntry kEdge##NAME[] = {TCS_EDGE_PREC(P)};                                \
     static const BatchEntry kBatch##NAME[] = {BATCH};                                         \
     extern const PartTables kPart##NAME = {kSweep##NAME, TCS_N(kSweep##NAME), kFused##NAME,   \
@@ -1862,20 +1865,20 @@ static_assert(TCS_STAR2D1R == 0 && TCS_STAR2D2R == 1 && TCS_BOX2D1R == 2 && TCS_
                                            kEdge##NAME, TCS_N(kEdge##NAME)};
 
 #if TCS_PART == 1
-TCS_PART_16(Fp16, 0, TCS_BATCH16_PREC(0), TCS_BATCH32_PREC(0))
+TCS_PART_16(Fp16, 0, TCS_FUSED_PREC(0), TCS_BATCH16_PREC(0), TCS_BATCH32_PREC(0))
 #elif TCS_PART == 2
-TCS_PART_16(Bf16, 1, TCS_BATCH16_PREC(1), TCS_BATCH32_PREC(1))
+TCS_PART_16(Bf16, 1, TCS_FUSED_PREC(1), TCS_BATCH16_PREC(1), TCS_BATCH32_PREC(1))
 #elif TCS_PART == 3
 // SPLIT (8x8: two workgroups per CU at 72 KB of LDS); no batched kernels.
 static const KernelEntry kSweepSplit[] = {TCS_MFMA_PREC(2)};
-static const FusedEntry kFusedSplit[] = {TCS_FUSED_PREC(2)};
+static const FusedEntry kFusedSplit[] = {TCS_FUSED_PREC88(2)};
 static const EdgeEntry kEdgeSplit[] = {TCS_EDGE_PREC(2)};
 extern const PartTables kPartSplit = {kSweepSplit, TCS_N(kSweepSplit), kFusedSplit, TCS_N(kFusedSplit),
                                       nullptr, 0, kEdgeSplit, TCS_N(kEdgeSplit)};
 #elif TCS_PART == 4
-TCS_PART_16(Fp16Lo, 4, TCS_BATCH16_PREC(4))
+TCS_PART_16(Fp16Lo, 4, TCS_FUSED_PREC88(4), TCS_BATCH16_PREC(4))
 #elif TCS_PART == 5
-TCS_PART_16(Bf16Lo, 5, TCS_BATCH16_PREC(5))
+TCS_PART_16(Bf16Lo, 5, TCS_FUSED_PREC88(5), TCS_BATCH16_PREC(5))
 #elif TCS_PART == 6
 // 32x32 fragments (FP16 and BF16, tiles whose 2x2-brick blocks split evenly over
 // the 4 waves, one tile per workgroup) and the direct-FMA kernel.
@@ -1931,6 +1934,20 @@ static const FusedEntry *select_fused(const tcs_problem *p) {
     return nullptr;
 }
 
+// The problem a fused launch runs: the caller's, except that FP16/BF16 with
+// rounded taps and 16x16 fragments fuses 8x8 problems on 4x8 tiles, which divide
+// the same layout (brick results depend on neither).  8x8 tiles hold 2 waves per
+// SIMD at the fused kernel's 56 KB image and measure slower than 4x8 at every
+// fused depth (box2d2r 16384^2).
+static tcs_problem fused_problem(const tcs_problem *p) {
+    tcs_problem q = *p;
+    const int kp = kernel_prec(p);
+    if (p->method == TCS_METHOD_MFMA && frag_of(p) == 16 && (kp == TCS_PREC_FP16 || kp == TCS_PREC_BF16) &&
+        p->tile_h == 8 && p->tile_w == 8)
+        q.tile_h = 4;
+    return q;
+}
+
 static const BatchEntry *select_batch(int prec, int th, int tw, int frag, bool fused, bool persist) {
     for (const PartTables *t : kParts)
         for (int i = 0; i < t->nbatch; ++i) {
@@ -1963,7 +1980,8 @@ int launch_batch(int precision, int tile_h, int tile_w, int frag, const BatchPro
 }
 
 const char *fused_kernel_name(const tcs_problem *p) {
-    const FusedEntry *k = select_fused(p);
+    const tcs_problem q = fused_problem(p);
+    const FusedEntry *k = select_fused(&q);
     return k ? k->name : nullptr;
 }
 
@@ -2016,7 +2034,9 @@ static int make_args(const tcs_problem *p, const void *d_in, void *d_out, SweepA
     return TCS_OK;
 }
 
-int launch_fused(const tcs_problem *p, const void *d_in, void *d_out, int nsweeps, hipStream_t stream) {
+int launch_fused(const tcs_problem *p_in, const void *d_in, void *d_out, int nsweeps, hipStream_t stream) {
+    const tcs_problem q = fused_problem(p_in);
+    const tcs_problem *p = &q;
     const FusedEntry *k = select_fused(p);
     if (!k)
         return set_error(TCS_ERR_UNSUPPORTED, "no fused kernel for method %d precision %d tile %dx%d", p->method,

@@ -38,6 +38,8 @@ import subprocess
 import sys
 import tempfile
 
+from .timing import warm_up
+
 HBM_PEAK_GBS = 8000.0
 BYTES_PER_UPDATE = {"fp16": 4.0, "bf16": 4.0, "split": 8.0}  # per sweep, unfused; fused: / fuse
 TILES = [(2, 4), (4, 4), (2, 8), (4, 8), (8, 8)]
@@ -59,8 +61,8 @@ def run_workload(stencil, rows, cols, method="mfma", precision="fp16", tile=None
     p.clear(b1)
     p.pack(x, b0, e)
     del x
-    p.run(b0, b1, warmup + warmup % 2)
-    torch.cuda.synchronize()
+    chunk = max(2, warmup + warmup % 2)
+    warm_up(lambda: p.run(b0, b1, chunk))
     ev = [torch.cuda.Event(enable_timing=True) for _ in range(2)]
     ev[0].record()
     p.run(b0, b1, sweeps)
@@ -120,9 +122,7 @@ def run_edges(stencil, rows, cols, precision="fp16", sweeps=1, reps=20, seed=1, 
     res = {"stencil": stencil, "rows": rows, "cols": cols, "precision": precision, "sweeps": sweeps,
            "tile": list(p.tile)}
     for name, fn in (("fused_edges", fused), ("three_steps", three)):
-        for _ in range(3):
-            fn()
-        torch.cuda.synchronize()
+        warm_up(fn)
         ev[0].record()
         for _ in range(reps):
             fn()

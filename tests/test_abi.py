@@ -70,6 +70,15 @@ def test_layout_and_problem_setup():
     assert (L.nbr, L.nbc, L.sbr, L.sbc, L.align) == (1024, 1024, 1026, 1026, 8)
     assert p.buffer_bytes == 1026 * 1026 * 256 * 2
     assert p.tile == (4, 8) and p.mfma_per_brick == 4
+    assert p.kernel_name == "tcs_mfma_sweep<4, 8, 0, 4, 0, false>"
+    # fused FP16/BF16 launches of 8x8 problems run 4x8 tiles; SPLIT keeps 8x8
+    p.set_tile(8, 8)
+    assert p.kernel_name == "tcs_mfma_sweep<8, 8, 0, 4, 0, false>"
+    assert p.fused_kernel_name == "tcs_mfma_fused<4, 8, 0, 4, 0>"
+    bf = T.Problem("box2d1r", 16384, 16384, np.ones((3, 3), np.float32), precision="bf16", tile=(8, 8))
+    assert bf.fused_kernel_name == "tcs_mfma_fused<4, 8, 1, 3, 1>"
+    sp = T.Problem("box2d2r", 16384, 16384, B, precision="split", tile=(8, 8))
+    assert sp.fused_kernel_name == "tcs_mfma_fused<8, 8, 0, 4, 2>"
     assert T.Problem("box2d2r", 64, 64, B, precision="split").mfma_per_brick == 12  # hi*hi, hi*lo, lo*hi
     assert list(p.s.coeff) == list(range(25))
     p = T.Problem("star2d1r", 100, 37, np.arange(9, dtype=np.float32).reshape(3, 3))
@@ -214,7 +223,7 @@ def test_batch_validation_needs_no_gpu():
     ok = T.Problem("box2d1r", 40, 40, B, tile=(2, 4))
     bad_tile = T.Problem("box2d1r", 40, 40, B, tile=(2, 8))
     direct = T.Problem("box2d1r", 40, 40, B, method="direct", tile=(2, 4))
-    bad32 = T.Problem("box2d1r", 40, 40, B, tile=(8, 8), frag=32)
+    bad32 = T.Problem("box2d1r", 40, 40, B, tile=(2, 8), frag=32)
     too_fused = T.Problem("box2d2r", 40, 40, np.ones((5, 5), np.float32), tile=(2, 4), fuse=10)
     fake = (ctypes.c_void_p * 1)(0x1000)
     fake2 = (ctypes.c_void_p * 1)(0x2000)

@@ -545,13 +545,14 @@ def test_fused_limits(T, torch):
 # ------------------------------------------------------- batched uneven grids
 def test_batch_bitwise_equals_per_grid(T, torch):
     """One launch per sweep (per tile shape) over many uneven grids gives the
-    same bits as sweeping each grid on its own."""
+    same bits as sweeping each grid on its own; 8x8 problems join the 4x8
+    group."""
     rng = np.random.default_rng(31)
     probs, b0s, b1s, refs, xs = [], [], [], [], []
     for i in range(12):
         stencil = ("box2d1r", "star2d2r")[i % 2]
         rows, cols = (int(v) for v in rng.integers(5, 300, 2))
-        tile = [(2, 4), (4, 4), (4, 8)][i % 3]
+        tile = [(2, 4), (4, 4), (4, 8), (8, 8)][i % 4]
         B = T.gen_coeffs(stencil, 100 + i)
         x = T.gen_uniform(rows, cols, seed=100 + i, dtype=torch.float32)
         p = T.Problem(stencil, rows, cols, B, tile=tile)
