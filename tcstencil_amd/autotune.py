@@ -24,7 +24,7 @@ import itertools
 import json
 import sys
 
-from .timing import warm_up
+from .timing import warm_chunk, warm_up
 
 TILES = [(2, 4), (4, 4), (2, 8), (4, 8), (8, 8)]
 
@@ -50,7 +50,8 @@ def tune(stencil, rows, cols, method="mfma", precision="fp16", sweeps=20, warmup
         b0, b1 = p.alloc(), p.alloc()
         p.clear(b1)
         p.pack(x, b0, e)
-        warm_up(lambda: p.run(b0, b1, max(2, warmup)))
+        chunk = warm_chunk(warmup, fuse)
+        warm_up(lambda: p.run(b0, b1, chunk))
         ev = [torch.cuda.Event(enable_timing=True) for _ in range(2)]
         ev[0].record()
         p.run(b0, b1, sweeps)

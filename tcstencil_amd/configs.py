@@ -29,7 +29,7 @@ import sys
 
 import numpy as np
 
-from .timing import warm_up
+from .timing import warm_chunk, warm_up
 
 HBM_PEAK_GBS = 8000.0
 BYTES = {"fp16": 4.0, "bf16": 4.0, "split": 8.0}
@@ -75,7 +75,8 @@ def _fp64_refs(stencil, x64, B, counts):
 
 def _timed_run(p, b0, b1, sweeps, warmup=2):
     torch = _torch()
-    warm_up(lambda: p.run(b0, b1, max(2, warmup + warmup % 2)))
+    chunk = warm_chunk(warmup, p.fuse)
+    warm_up(lambda: p.run(b0, b1, chunk))
     e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
     e0.record()
     res = p.run(b0, b1, sweeps)

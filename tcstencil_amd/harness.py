@@ -38,7 +38,7 @@ import subprocess
 import sys
 import tempfile
 
-from .timing import warm_up
+from .timing import warm_chunk, warm_up
 
 HBM_PEAK_GBS = 8000.0
 BYTES_PER_UPDATE = {"fp16": 4.0, "bf16": 4.0, "split": 8.0}  # per sweep, unfused; fused: / fuse
@@ -61,7 +61,7 @@ def run_workload(stencil, rows, cols, method="mfma", precision="fp16", tile=None
     p.clear(b1)
     p.pack(x, b0, e)
     del x
-    chunk = max(2, warmup + warmup % 2)
+    chunk = warm_chunk(warmup, fuse)
     warm_up(lambda: p.run(b0, b1, chunk))
     ev = [torch.cuda.Event(enable_timing=True) for _ in range(2)]
     ev[0].record()

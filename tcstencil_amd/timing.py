@@ -13,6 +13,14 @@ import time
 WARM_MS = 100.0
 
 
+def warm_chunk(sweeps, fuse=1):
+    """Sweeps per warm-up call: an even count of at least `sweeps` that is a
+    multiple of 2 * fuse, so that fused runs warm up on full launches (a run
+    keeps its result in buffer sweeps % 2, so it splits an odd launch count)."""
+    unit = 2 * max(int(fuse or 1), 1)
+    return max(unit, (sweeps + unit - 1) // unit * unit)
+
+
 def warm_up(launch, min_ms=WARM_MS, max_calls=100000):
     """Call launch() (GPU work on the current stream) and wait for it, again and
     again, until min_ms of wall time has passed; returns the number of calls (at
