@@ -196,8 +196,8 @@ static bool frag32(const tcs_problem *p) { return p->frag == 32; }
 // Default tile: the fastest shape measured on 16384^2 grids for this method,
 // precision and radius, stepped down until the grid has at least 4096 tiles
 // (16 per CU), so that smaller grids still fill the chip.  Over the mesh sweep
-// 160..7200 x all stencils this picks within 2.1% (MFMA) of the best tile on
-// average; a 1024-tile threshold was 4.9% off.
+// 160..7200 x all stencils this picks within 2.1% (MFMA) and 2.5% (direct) of
+// the best tile on average; a 1024-tile threshold was 4.0% and 6.5% off.
 static void default_tile(const tcs_problem *p, int *th, int *tw) {
     static const int kPref[5][2] = {{8, 8}, {4, 8}, {4, 4}, {2, 8}, {2, 4}};
     // 32x32 fragments need 2x2-brick blocks, 4 per tile; 4x4 is the smallest
