@@ -151,8 +151,10 @@ int tcs_problem_init(tcs_problem *p, int stencil, int method, int precision, int
 int tcs_radius(int stencil);
 /* Choose the workgroup tile (bricks, rows x cols): 2x4, 4x4, 2x8, 4x8 or 8x8.
  * With 16x16 fragments single-sweep and fused kernels exist for all five
- * (FP16/BF16 with rounded taps fuse 8x8 problems on 4x8 tiles, which measure
- * faster at every fused depth); a missing combination (32x32 fragments: see tcs_problem_set_frag) fails at
+ * (FP16/BF16 with rounded taps fuse large 4x8 problems -- at least 2048 8x8
+ * tiles -- on 8x8 tiles of 8-wave workgroups, which measure faster at every
+ * fused depth; SPLIT fuses 6 or more sweeps per launch of 4x8 problems on
+ * 8-wave workgroups); a missing combination (32x32 fragments: see tcs_problem_set_frag) fails at
  * launch with TCS_ERR_UNSUPPORTED (tcs_kernel_name / tcs_fused_kernel_name
  * return NULL for it).  Re-derives the layout's alignment, so buffers packed
  * for another tile must be packed again. */
@@ -355,7 +357,8 @@ int tcs_gen_coeffs(int stencil, uint64_t seed, float *B, int n);
 /* ------------------------------------------------------------ introspection */
 /* Kernel name (as rocprofv3 reports it) used for this problem, or NULL. */
 const char *tcs_kernel_name(const tcs_problem *p);
-/* Kernel name of the fused (temporally blocked) launch for this problem, or NULL. */
+/* Kernel name of the fused (temporally blocked) launch of p->fuse sweeps for
+ * this problem, or NULL. */
 const char *tcs_fused_kernel_name(const tcs_problem *p);
 /* Kernel name of tcs_run_f32's sweep with the layout transforms fused in (mode 1:
  * row-major FP32 in, storage out; 2: storage in, row-major FP32 out; 3: both, a

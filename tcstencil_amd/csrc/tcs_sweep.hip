@@ -73,19 +73,20 @@ __device__ __forceinline__ uint32_t pack2(uint16_t lo, uint16_t hi) {
 //     of consecutive rows per group, groups 0, 2 on the upper and 1, 3 on the
 //     lower half of the 16-byte rows), where two ds_write_b32 of the 2-column
 //     offset hit half the banks twice each.
-template <int BH_, int BW_, int NPL_, bool STRIP_ = false>
+template <int BH_, int BW_, int NPL_, bool STRIP_ = false, int NT_ = kWG>
 struct Geom {
     static constexpr int BH = BH_, BW = BW_, NPL = NPL_;
     static constexpr bool STRIP = STRIP_;
+    static constexpr int NT = NT_;              // threads of the workgroup that stages the image
     static constexpr int FX = STRIP ? 4 : 2;    // image column of the first data column
     static constexpr int LR = 16 * BH + 4;      // image rows (2 frame rows each side)
     static constexpr int LC = 16 * BW + 8;      // image columns (row-major pitch)
     static constexpr int NR = 16 * BH + 8;      // rows per strip (STRIP)
     static constexpr int NCH = BH * BW * 32;    // 16-byte chunks of the data bricks
-    static constexpr int CPT = (NCH + kWG - 1) / kWG;
+    static constexpr int CPT = (NCH + NT - 1) / NT;
     static constexpr int HD = 8 * BW + 2;       // dwords in one top/bottom frame row
     static constexpr int NH = 4 * HD + 32 * BH; // frame dwords: 2 top + 2 bottom rows, left, right
-    static constexpr int HPT = (NH + kWG - 1) / kWG;
+    static constexpr int HPT = (NH + NT - 1) / NT;
     static constexpr int PLANE = STRIP ? LC * NR : LR * LC;  // LDS elements per plane
     // Element offset of image point (r, c).  Offsets add: at(r1 + r2, c1 + c2)
     // = at(r1, c1) + at(r2, c2) whenever c1 is a multiple of 8.
@@ -139,8 +140,8 @@ template <class G, bool CHECK>
 __device__ __forceinline__ void load_tile(const SweepArgs &a, TileRegs<G> &R, int r0, int c0, int tid) {
 #pragma unroll
     for (int i = 0; i < G::CPT; ++i) {
-        const int c = tid + i * kWG;
-        if (G::NCH % kWG == 0 || c < G::NCH) {
+        const int c = tid + i * G::NT;
+        if (G::NCH % G::NT == 0 || c < G::NCH) {
             const int b = c >> 5, w = c & 31;
             const int br = b / G::BW, bc = b - br * G::BW;
             const int64_t s = stor_index(r0 + 16 * br + (w >> 1), c0 + 16 * bc + 8 * (w & 1), a.sbc);
@@ -151,7 +152,7 @@ __device__ __forceinline__ void load_tile(const SweepArgs &a, TileRegs<G> &R, in
     }
 #pragma unroll
     for (int i = 0; i < G::HPT; ++i) {
-        const int h = tid + i * kWG;
+        const int h = tid + i * G::NT;
         if (h < G::NH) {
             int lr, ld;
             halo_coords<G>(h, lr, ld);
@@ -193,8 +194,8 @@ __device__ __forceinline__ void load_tile_rm(const SweepArgs &a, TileRegs<G> &R,
     static_assert(G::NPL == 1 || base_prec(PREC) == TCS_PREC_SPLIT, "one storage plane, or the SPLIT pair");
 #pragma unroll
     for (int i = 0; i < G::CPT; ++i) {
-        const int c = tid + i * kWG;
-        if (G::NCH % kWG == 0 || c < G::NCH) {
+        const int c = tid + i * G::NT;
+        if (G::NCH % G::NT == 0 || c < G::NCH) {
             const int lrow = c / (2 * G::BW), cc = c - lrow * (2 * G::BW);
             const int gr = r0 + lrow, gc = c0 + 8 * cc;  // both >= 0
             float x[8];
@@ -221,7 +222,7 @@ __device__ __forceinline__ void load_tile_rm(const SweepArgs &a, TileRegs<G> &R,
     }
 #pragma unroll
     for (int i = 0; i < G::HPT; ++i) {
-        const int h = tid + i * kWG;
+        const int h = tid + i * G::NT;
         if (h < G::NH) {
             int lr, ld;
             halo_coords<G>(h, lr, ld);
@@ -243,8 +244,8 @@ __device__ __forceinline__ void write_tile(const TileRegs<G> &R, uint16_t *lds, 
     uint32_t *l32 = reinterpret_cast<uint32_t *>(lds);
 #pragma unroll
     for (int i = 0; i < G::CPT; ++i) {
-        const int c = tid + i * kWG;
-        if (G::NCH % kWG == 0 || c < G::NCH) {
+        const int c = tid + i * G::NT;
+        if (G::NCH % G::NT == 0 || c < G::NCH) {
             int lr, lc;  // lc = FX (mod 8)
             if constexpr (ROWMAJOR) {
                 const int lrow = c / (2 * G::BW);
@@ -274,7 +275,7 @@ __device__ __forceinline__ void write_tile(const TileRegs<G> &R, uint16_t *lds, 
     }
 #pragma unroll
     for (int i = 0; i < G::HPT; ++i) {
-        const int h = tid + i * kWG;
+        const int h = tid + i * G::NT;
         if (h < G::NH) {
             int lr, ld;
             halo_coords<G>(h, lr, ld);
@@ -294,7 +295,7 @@ template <class G>
 __device__ __forceinline__ void zero_pad(uint16_t *lds, int tid) {
     static_assert(G::FX == 2 || G::FX == 4, "two pad dwords per image row");
     uint32_t *l32 = reinterpret_cast<uint32_t *>(lds);
-    for (int zc = tid; zc < 2 * G::LR; zc += kWG) {
+    for (int zc = tid; zc < 2 * G::LR; zc += G::NT) {
         const int col = G::FX == 4 ? ((zc & 1) ? 16 * G::BW + 6 : 0) : 16 * G::BW + 4 + 2 * (zc & 1);
         const int d = G::at(zc >> 1, col) >> 1;
 #pragma unroll
@@ -626,30 +627,32 @@ __device__ __forceinline__ void mfma_brick(const SweepArgs &a, const uint16_t *l
     brick_out<G, PREC>(a, lds, mfma_acc<G, ZLO, ZHI, PREC>(C, op), br, bc, m, g, edge, gr, gcb, w);
 }
 
-// Brick q of a wave's share of a BH x BW brick region (both even) whose first
-// brick is (BASE, BASE), for the sweep loops of the fused kernels, q row-major
-// within the wave's block: with BH a multiple of 4 wave w takes whole rows
-// (BH/4 of them, from row w * BH/4), else the 4 waves split the region 2 x 2,
-// wave w taking the (BH/2) x (BW/2) block at block position (w / 2, w % 2).
-// A brick's offset from its block's first brick is a compile-time constant, so
-// a per-brick LDS address is one register plus an immediate, and a wave walks
-// each row of its block left to right (BlockRows<BH, BW>::RW bricks per row:
-// every brick but a row's first reuses its left neighbour's fragment).
-template <int BH, int BW>
+// Brick q of a wave's share of a BH x BW brick region whose first brick is
+// (BASE, BASE), split over the NW waves of a workgroup, q row-major within the
+// wave's block: with BH a multiple of NW wave w takes whole rows (BH/NW of
+// them, from row w * BH/NW), else the waves split the region into NW/2 row
+// bands by 2 column halves, wave w taking the (2BH/NW) x (BW/2) block at
+// (w / 2, w % 2) (4 waves: the 2 x 2 split).  A brick's offset from its
+// block's first brick is a compile-time constant, so a per-brick LDS address
+// is one register plus an immediate, and a wave walks each row of its block
+// left to right (BlockRows::RW bricks per row: every brick but a row's first
+// reuses its left neighbour's fragment).
+template <int BH, int BW, int NW = 4>
 struct BlockRows {
-    static_assert(BH % 2 == 0 && BW % 2 == 0, "brick region must split over the 4 waves");
-    static constexpr bool WHOLE = BH % 4 == 0;
+    static constexpr bool WHOLE = BH % NW == 0;
+    static_assert(WHOLE || (BH % (NW / 2) == 0 && BW % 2 == 0), "brick region must split over the waves");
     static constexpr int RW = WHOLE ? BW : BW / 2;  // bricks per row of a wave's block
+    static constexpr int Q = BH * BW / NW;          // bricks per wave
 };
 
-template <int BH, int BW, int BASE>
+template <int BH, int BW, int BASE, int NW = 4>
 __device__ __forceinline__ void block_brick(int wave, int q, int &br, int &bc) {
-    using B = BlockRows<BH, BW>;
+    using B = BlockRows<BH, BW, NW>;
     if constexpr (B::WHOLE) {
-        br = BASE + wave * (BH / 4) + q / BW;
+        br = BASE + wave * (BH / NW) + q / BW;
         bc = BASE + q % BW;
     } else {
-        br = BASE + (wave >> 1) * (BH / 2) + q / (BW / 2);
+        br = BASE + (wave >> 1) * (2 * BH / NW) + q / (BW / 2);
         bc = BASE + (wave & 1) * (BW / 2) + q % (BW / 2);
     }
 }
@@ -855,36 +858,41 @@ void tcs_mfma_edge(SweepArgs a) {
 // equal to k single sweeps.  Values outside the valid region stay finite
 // (stencils of loaded data), which the MFMA form needs: a non-finite operand
 // anywhere in a brick row's window reaches all 16 outputs of that row.
-template <int TH, int TW, int NPL, bool STRIP = false>
-using FusedGeom = Geom<TH + 2, TW + 2, NPL, STRIP>;
+template <int TH, int TW, int NPL, bool STRIP = false, int NT = kWG>
+using FusedGeom = Geom<TH + 2, TW + 2, NPL, STRIP, NT>;
 
 // One sweep of the fused image but the last: every brick is computed into
-// registers, then, after a barrier, written back in place.  Work per wave: its
-// block of the core band (image brick rows 1..TH, every column, the left and
-// right ring bricks included; block_brick: TH/4 whole rows, or for TH = 2 one
-// half row), then its share of the ring rows (image brick rows 0 and TH+1):
-//   !PAIRED  whole bricks: waves 0, 1 the top row, waves 2, 3 the bottom row,
-//            BW/2 columns each;
+// registers, then, after a barrier, written back in place.  Work per wave (NW
+// waves): its block of the core band (image brick rows 1..TH, every column,
+// the left and right ring bricks included; block_brick), then a run of
+// consecutive units of one ring row:
+//   !PAIRED  whole bricks of the top (image brick row 0) or bottom (TH+1) row:
+//            waves 0..NW/2-1 share the top row, the others the bottom row;
 //   PAIRED   (the sweep's results are needed only within 8 points of the core)
-//            one product per ring column pair: lanes 0..7 compute rows 8..15
-//            of the top brick, lanes 8..15 rows 0..7 of the bottom brick (a
-//            lane's operand rows are its own address: lane row m + 8, plus
-//            16 TH rows from lane 8 on); the coefficient operand and the K
-//            slots are those of a whole brick, so every computed value has the
-//            bits of the unfused sweep.  The other halves keep the previous
-//            sweep's values, finite and farther than 8 points from the core.
-//            A column half's BW/2 pairs go ceil(BW/4) to the first wave of the
-//            half and the rest to the second.
-template <class G, int TH, int ZLO, int ZHI, int PREC, bool PAIRED, bool LOOKAHEAD>
+//            one product per ring column pair, all NW waves sharing the row of
+//            pairs: lanes 0..7 compute rows 8..15 of the top brick, lanes 8..15
+//            rows 0..7 of the bottom brick (a lane's operand rows are its own
+//            address: lane row m + 8, plus 16 TH rows from lane 8 on); the
+//            coefficient operand and the K slots are those of a whole brick,
+//            so every computed value has the bits of the unfused sweep.  The
+//            other halves keep the previous sweep's values, finite and farther
+//            than 8 points from the core.
+// A ring row of BW units split over W waves gives the first BW % W waves
+// ceil(BW / W) consecutive units and the others floor(BW / W).
+template <class G, int TH, int ZLO, int ZHI, int PREC, bool PAIRED, bool LOOKAHEAD, int NW>
 __device__ __forceinline__ void fused_sweep(const SweepArgs &a, uint16_t *lds, const Coef<ZLO, ZHI, PREC> &C, int mo,
                                             int g, int wave, bool edge, int er0, int ec0) {
     constexpr int NPL = G::NPL;
-    constexpr int HB = G::BW / 2;        // image columns per column half
-    constexpr int QB = (TH / 2) * HB;    // core-band bricks per wave
-    constexpr int HP = (HB + 1) / 2;     // pairs of the first wave of a column half
-    constexpr int QT = QB + (PAIRED ? HP : HB);
-    static_assert(TH % 2 == 0 && G::BW % 2 == 0, "the image must split 2 x 2 over the waves");
+    using BR = BlockRows<TH, G::BW, NW>;
+    constexpr int QB = BR::Q;                    // core-band bricks per wave
+    constexpr int WPR = PAIRED ? NW : NW / 2;    // waves per ring row
+    constexpr int RC = G::BW / WPR, RX = G::BW % WPR;
+    constexpr int QT = QB + RC + (RX > 0);
     using O = Ops<G, ZLO, ZHI, PREC>;
+    const int wi = wave % WPR;                                // position among the ring row's waves
+    const int ring_br = PAIRED || wave < WPR ? 0 : TH + 1;   // this wave's ring row
+    const int ring_bc = wi * RC + (wi < RX ? wi : RX);       // its first ring unit
+    const bool ring_extra = wi < RX;                         // whether it has RC + 1 of them
     const int mp = mo + 8 + (mo >= 8 ? 16 * TH : 0);  // a paired unit's lane row, from image brick row 0
     const typename O::Offsets offs(mo, g);
     // LDS offsets are linear in the row (G::at), so a paired unit's operands
@@ -897,22 +905,19 @@ __device__ __forceinline__ void fused_sweep(const SweepArgs &a, uint16_t *lds, c
         pair = false;
         valid = true;
         if (q < QB) {
-            block_brick<TH, G::BW, 0>(wave, q, br, bc);  // the core band: image brick rows 1..TH
+            block_brick<TH, G::BW, 0, NW>(wave, q, br, bc);  // the core band: image brick rows 1..TH
             br += 1;
-        } else if constexpr (!PAIRED) {
-            br = (wave >> 1) ? TH + 1 : 0;
-            bc = (wave & 1) * HB + (q - QB);
         } else {
             const int j = q - QB;
-            br = 0;
-            bc = (wave & 1) * HB + (wave >> 1) * HP + j;
-            pair = true;
-            valid = (wave >> 1) == 0 || j < HB - HP;
+            br = ring_br;
+            bc = ring_bc + j;
+            pair = PAIRED;
+            valid = j < RC || ring_extra;
         }
     };
     // Unit q follows its left neighbour (same lane rows) in this wave's order,
     // so its fragment 0 is the previous unit's fragment 1.
-    auto reuse = [&](int q) { return q < QB ? q % BlockRows<TH, G::BW>::RW != 0 : q > QB; };
+    auto reuse = [&](int q) { return q < QB ? q % BR::RW != 0 : q > QB; };
     u32x2 *l64 = reinterpret_cast<u32x2 *>(lds);
     u32x2 res[QT][NPL];
     O cur, nxt;
@@ -977,7 +982,7 @@ __device__ __forceinline__ void fused_sweep(const SweepArgs &a, uint16_t *lds, c
     __syncthreads();
 }
 
-template <int TH, int TW, int ZLO, int ZHI, int PREC>
+template <int TH, int TW, int ZLO, int ZHI, int PREC, int NW>
 __device__ __forceinline__ void fused_tile(const SweepArgs &a, const CoefWindows &cw, uint16_t *lds, int gr0,
                                            int gc0, int nsweeps) {
     constexpr int NPL = base_prec(PREC) == TCS_PREC_SPLIT ? 2 : 1;
@@ -986,9 +991,8 @@ __device__ __forceinline__ void fused_tile(const SweepArgs &a, const CoefWindows
     // 115 us per sweep; box2d1r 16 fused: 94 -> 90 us); it measured slightly
     // slower on 4x4 tiles, and SPLIT has no registers left for it.
     constexpr bool LOOKAHEAD = base_prec(PREC) != TCS_PREC_SPLIT && TH * TW >= 32;
-    using G = FusedGeom<TH, TW, NPL, true>;
-    constexpr int QC = TH * TW / 4;        // bricks per wave, core only
-    static_assert(G::BH * G::BW % 4 == 0 && TH * TW % 4 == 0, "bricks must split over 4 waves");
+    using G = FusedGeom<TH, TW, NPL, true, 64 * NW>;
+    constexpr int QC = BlockRows<TH, TW, NW>::Q;  // bricks per wave, core only
     const int tid = threadIdx.x;
     const int er0 = gr0 - 16, ec0 = gc0 - 16;  // global point of image brick (0, 0)
     {
@@ -1019,27 +1023,27 @@ __device__ __forceinline__ void fused_tile(const SweepArgs &a, const CoefWindows
         // bottom ring bricks are computed half each, paired in one product.
         constexpr int R = ZLO == 0 ? 2 : 1;
         if (R * (nsweeps - 1 - sw) <= 8)
-            fused_sweep<G, TH, ZLO, ZHI, PREC, true, LOOKAHEAD>(a, lds, C, mo, g, wv, edge, er0, ec0);
+            fused_sweep<G, TH, ZLO, ZHI, PREC, true, LOOKAHEAD, NW>(a, lds, C, mo, g, wv, edge, er0, ec0);
         else
-            fused_sweep<G, TH, ZLO, ZHI, PREC, false, LOOKAHEAD>(a, lds, C, mo, g, wv, edge, er0, ec0);
+            fused_sweep<G, TH, ZLO, ZHI, PREC, false, LOOKAHEAD, NW>(a, lds, C, mo, g, wv, edge, er0, ec0);
     }
     Ops<G, ZLO, ZHI, PREC> cur, nxt;
     if constexpr (LOOKAHEAD) {
         int br, bc;
-        block_brick<TH, TW, 1>(wave, 0, br, bc);
+        block_brick<TH, TW, 1, NW>(wave, 0, br, bc);
         cur.load(lds, br, bc, off);
     }
     // block_brick walks the wave's block row by row: brick q follows its left
     // neighbour unless it starts a row.
-    auto reuse = [](int q) { return q % BlockRows<TH, TW>::RW != 0; };
+    auto reuse = [](int q) { return q % BlockRows<TH, TW, NW>::RW != 0; };
 #pragma unroll
     for (int q = 0; q < QC; ++q) {
         int br, bc;
-        block_brick<TH, TW, 1>(wave, q, br, bc);
+        block_brick<TH, TW, 1, NW>(wave, q, br, bc);
         if constexpr (LOOKAHEAD) {
             if (q + 1 < QC) {
                 int nbr, nbc;
-                block_brick<TH, TW, 1>(wave, q + 1, nbr, nbc);
+                block_brick<TH, TW, 1, NW>(wave, q + 1, nbr, nbc);
                 nxt.load(lds, nbr, nbc, off, cur, reuse(q + 1));
             }
         } else {
@@ -1061,31 +1065,48 @@ __device__ __forceinline__ void fused_tile(const SweepArgs &a, const CoefWindows
 // which LDS allows for tiles up to 4x8 (box2d2r 16384^2, 4x8 tile, 8 sweeps
 // fused: 142 -> 134 us per sweep against 2 waves per SIMD).  SPLIT kernels
 // (two image planes) are held to as many waves per SIMD as LDS allows
-// workgroups per CU, at most 3 (168 VGPRs): 2x4, 4x4 and 2x8 tiles 3, 4x8 2,
-// 8x8 1.  Without the bound a 2x8 kernel that grew past 168 VGPRs fell to 2
-// waves per SIMD and ran 18% slower (box2d2r 16384^2, 8 fused sweeps: 391
-// against 331 us per sweep).
-template <int TH, int TW, int PREC>
+// workgroups per CU times the workgroup's waves per SIMD (NW / 4), at most 3
+// (168 VGPRs) for 4-wave and 4 (128 VGPRs) for 8-wave workgroups: 4 waves,
+// 2x4, 4x4 and 2x8 tiles 3, 4x8 2, 8x8 1; 8 waves, 4x8 4.  Without the bound a
+// 2x8 kernel that grew past 168 VGPRs fell to 2 waves per SIMD and ran 18%
+// slower (box2d2r 16384^2, 8 fused sweeps: 391 against 331 us per sweep).
+template <int TH, int TW, int PREC, int NW>
 constexpr int kFusedMinWaves() {
     if constexpr (base_prec(PREC) != TCS_PREC_SPLIT) {
         return 4;
     } else {
         constexpr int wgs = kLdsBytesPerCU / (2 * 2 * FusedGeom<TH, TW, 2, true>::PLANE);
-        return wgs < 1 ? 1 : (wgs > 3 ? 3 : wgs);
+        constexpr int w = wgs * NW / 4, cap = NW == 4 ? 3 : 4;
+        return w < 1 ? 1 : (w > cap ? cap : w);
     }
 }
 
-template <int TH, int TW, int ZLO, int ZHI, int PREC>
-__global__ __launch_bounds__(kWG)
-__attribute__((amdgpu_waves_per_eu(kFusedMinWaves<TH, TW, PREC>(), 8)))
-void tcs_mfma_fused(SweepArgs a, int nsweeps) {
+template <int TH, int TW, int ZLO, int ZHI, int PREC, int NW>
+__device__ __forceinline__ void fused_kernel(const SweepArgs &a, int nsweeps) {
     constexpr int NPL = base_prec(PREC) == TCS_PREC_SPLIT ? 2 : 1;
     __shared__ __attribute__((aligned(16))) uint16_t lds[NPL * FusedGeom<TH, TW, NPL, true>::PLANE];
     const Schedule sc = schedule(a);
     if (sc.tile >= sc.end) return;
     int gr0, gc0;
     tile_origin<TH, TW>(a, sc.tile, gr0, gc0);
-    fused_tile<TH, TW, ZLO, ZHI, PREC>(a, a.cw, lds, gr0, gc0, nsweeps);
+    fused_tile<TH, TW, ZLO, ZHI, PREC, NW>(a, a.cw, lds, gr0, gc0, nsweeps);
+}
+
+// One tile per workgroup of 4 waves.
+template <int TH, int TW, int ZLO, int ZHI, int PREC>
+__global__ __launch_bounds__(kWG)
+__attribute__((amdgpu_waves_per_eu(kFusedMinWaves<TH, TW, PREC, 4>(), 8)))
+void tcs_mfma_fused(SweepArgs a, int nsweeps) {
+    fused_kernel<TH, TW, ZLO, ZHI, PREC, 4>(a, nsweeps);
+}
+
+// One tile per workgroup of 8 waves: for images whose LDS size allows few
+// workgroups per CU (SPLIT 4x8: 2 workgroups, 4 waves per SIMD instead of 2).
+template <int TH, int TW, int ZLO, int ZHI, int PREC>
+__global__ __launch_bounds__(2 * kWG)
+__attribute__((amdgpu_waves_per_eu(kFusedMinWaves<TH, TW, PREC, 8>(), 8)))
+void tcs_mfma_fused8(SweepArgs a, int nsweeps) {
+    fused_kernel<TH, TW, ZLO, ZHI, PREC, 8>(a, nsweeps);
 }
 
 // ----------------------------------------------- MFMA kernels, 32x32 fragments
@@ -1466,9 +1487,9 @@ __device__ __forceinline__ void batch_body(const BatchProblem *__restrict__ prob
             fused_tile32<TH, TW, 1, 3, PREC>(a, P.cw, lds, gr0, gc0, nsweeps);
     } else if constexpr (FUSED) {
         if (a.rs == 2)
-            fused_tile<TH, TW, 0, 4, PREC>(a, P.cw, lds, gr0, gc0, nsweeps);
+            fused_tile<TH, TW, 0, 4, PREC, 4>(a, P.cw, lds, gr0, gc0, nsweeps);
         else
-            fused_tile<TH, TW, 1, 3, PREC>(a, P.cw, lds, gr0, gc0, nsweeps);
+            fused_tile<TH, TW, 1, 3, PREC, 4>(a, P.cw, lds, gr0, gc0, nsweeps);
     } else if constexpr (FRAG == 32) {
         if (a.rs == 2)
             sweep_tile32<TH, TW, 0, 4, PREC>(a, P.cw, lds, gr0, gc0);
@@ -1747,9 +1768,10 @@ struct KernelEntry {
     const char *name;  // as rocprofv3 reports it (demangled, without the namespace)
 };
 
-// Fused (temporally blocked) MFMA kernels.
+// Fused (temporally blocked) MFMA kernels; nt threads per workgroup; the
+// entry serves launches of at least min_sweeps sweeps.
 struct FusedEntry {
-    int prec, radius, th, tw, frag;
+    int prec, radius, th, tw, frag, nt, min_sweeps;
     fused_fn fn;
     const char *name;
 };
@@ -1813,16 +1835,23 @@ struct PartTables {
     TCS_MFMA32(8, 8, ZLO, ZHI, P, R)
 
 #define TCS_FUSED(TH, TW, ZLO, ZHI, P, R) \
-    {P, R, TH, TW, 16, tcs_mfma_fused<TH, TW, ZLO, ZHI, P>, "tcs_mfma_fused<" #TH ", " #TW ", " #ZLO ", " #ZHI ", " #P ">"}
+    {P, R, TH, TW, 16, kWG, 1, tcs_mfma_fused<TH, TW, ZLO, ZHI, P>, \
+     "tcs_mfma_fused<" #TH ", " #TW ", " #ZLO ", " #ZHI ", " #P ">"}
+#define TCS_FUSED8(TH, TW, ZLO, ZHI, P, R, MINS) \
+    {P, R, TH, TW, 16, 2 * kWG, MINS, tcs_mfma_fused8<TH, TW, ZLO, ZHI, P>, \
+     "tcs_mfma_fused8<" #TH ", " #TW ", " #ZLO ", " #ZHI ", " #P ">"}
 #define TCS_FUSED_TILES(ZLO, ZHI, P, R)                                                           \
     TCS_FUSED(2, 4, ZLO, ZHI, P, R), TCS_FUSED(4, 4, ZLO, ZHI, P, R), TCS_FUSED(2, 8, ZLO, ZHI, P, R), \
         TCS_FUSED(4, 8, ZLO, ZHI, P, R)
 #define TCS_FUSED_TILES88(ZLO, ZHI, P, R) TCS_FUSED_TILES(ZLO, ZHI, P, R), TCS_FUSED(8, 8, ZLO, ZHI, P, R)
-// FP16/BF16 with rounded taps fuse 8x8 problems on 4x8 tiles (fused_problem).
-#define TCS_FUSED_PREC(P) TCS_FUSED_TILES(0, 4, P, 2), TCS_FUSED_TILES(1, 3, P, 1)
 #define TCS_FUSED_PREC88(P) TCS_FUSED_TILES88(0, 4, P, 2), TCS_FUSED_TILES88(1, 3, P, 1)
+// FP16/BF16 with rounded taps: 8x8 tiles on 8-wave workgroups (fused_problem
+// maps large 4x8 problems to them), smaller tiles on 4 waves.
+#define TCS_FUSED_PREC(P) \
+    TCS_FUSED8(8, 8, 0, 4, P, 2, 1), TCS_FUSED8(8, 8, 1, 3, P, 1, 1), TCS_FUSED_TILES(0, 4, P, 2), \
+        TCS_FUSED_TILES(1, 3, P, 1)
 #define TCS_FUSED32(TH, TW, ZLO, ZHI, P, R) \
-    {P, R, TH, TW, 32, tcs_mfma32_fused<TH, TW, ZLO, ZHI, P>, \
+    {P, R, TH, TW, 32, kWG, 1, tcs_mfma32_fused<TH, TW, ZLO, ZHI, P>, \
      "tcs_mfma32_fused<" #TH ", " #TW ", " #ZLO ", " #ZHI ", " #P ">"}
 #define TCS_FUSED32_TILES(ZLO, ZHI, P, R) \
     TCS_FUSED32(4, 4, ZLO, ZHI, P, R), TCS_FUSED32(4, 8, ZLO, ZHI, P, R), TCS_FUSED32(8, 8, ZLO, ZHI, P, R)
@@ -1871,7 +1900,12 @@ TCS_PART_16(Bf16, 1, TCS_FUSED_PREC(1), TCS_BATCH16_PREC(1), TCS_BATCH32_PREC(1)
 #elif TCS_PART == 3
 // SPLIT (8x8: two workgroups per CU at 72 KB of LDS); no batched kernels.
 static const KernelEntry kSweepSplit[] = {TCS_MFMA_PREC(2)};
-static const FusedEntry kFusedSplit[] = {TCS_FUSED_PREC88(2)};
+// 4x8 tiles on 8-wave workgroups for launches of 6 or more sweeps, first
+// (select_fused takes the first match): box2d2r 16384^2, us per sweep, 8 waves
+// against 4: k = 6 277 / 280, 7 275 / 281, 8 274 / 281, 9 273 / 283; k = 5
+// ties (278 / 278), k = 4 296 / 287 and k = 2 374 / 341 favour 4 waves.
+static const FusedEntry kFusedSplit[] = {TCS_FUSED8(4, 8, 0, 4, 2, 2, 6), TCS_FUSED8(4, 8, 1, 3, 2, 1, 6),
+                                         TCS_FUSED_PREC88(2)};
 static const EdgeEntry kEdgeSplit[] = {TCS_EDGE_PREC(2)};
 extern const PartTables kPartSplit = {kSweepSplit, TCS_N(kSweepSplit), kFusedSplit, TCS_N(kFusedSplit),
                                       nullptr, 0, kEdgeSplit, TCS_N(kEdgeSplit)};
@@ -1921,30 +1955,37 @@ static const KernelEntry *select_kernel(const tcs_problem *p) {
     return nullptr;
 }
 
-static const FusedEntry *select_fused(const tcs_problem *p) {
+// The kernel of a launch of `nsweeps` fused sweeps of problem p.
+static const FusedEntry *select_fused(const tcs_problem *p, int nsweeps) {
     if (p->method != TCS_METHOD_MFMA) return nullptr;
     const int kp = kernel_prec(p);
     for (const PartTables *t : kParts)
         for (int i = 0; i < t->nfused; ++i) {
             const FusedEntry &k = t->fused[i];
             if (k.prec == kp && k.radius == p->radius && k.th == p->tile_h && k.tw == p->tile_w &&
-                k.frag == frag_of(p))
+                k.frag == frag_of(p) && nsweeps >= k.min_sweeps)
                 return &k;
         }
     return nullptr;
 }
 
 // The problem a fused launch runs: the caller's, except that FP16/BF16 with
-// rounded taps and 16x16 fragments fuses 8x8 problems on 4x8 tiles, which divide
-// the same layout (brick results depend on neither).  8x8 tiles hold 2 waves per
-// SIMD at the fused kernel's 56 KB image and measure slower than 4x8 at every
-// fused depth (box2d2r 16384^2).
+// rounded taps and 16x16 fragments fuses 4x8 problems on 8x8 tiles (8-wave
+// workgroups, tcs_mfma_fused8) where the layout divides into 8x8 tiles and
+// makes at least 2048 of them (brick results depend on the tile in no way).
+// An 8x8 tile recomputes 12% fewer ring bricks than two 4x8 tiles over 8
+// sweeps, and its 56 KB image holds 2 workgroups of 8 waves per CU, 4 waves
+// per SIMD as 4x8 tiles on 4-wave workgroups do (box2d2r 16384^2, us per
+// sweep, 8x8 on 8 waves against 4x8 on 4: k = 4 107.5 / 113.5, 6 100.5 / 103.5,
+// 8 96 / 100.5, 9 95.5 / 99.5).
 static tcs_problem fused_problem(const tcs_problem *p) {
     tcs_problem q = *p;
     const int kp = kernel_prec(p);
+    const tcs_layout &L = p->layout;
     if (p->method == TCS_METHOD_MFMA && frag_of(p) == 16 && (kp == TCS_PREC_FP16 || kp == TCS_PREC_BF16) &&
-        p->tile_h == 8 && p->tile_w == 8)
-        q.tile_h = 4;
+        p->tile_h == 4 && p->tile_w == 8 && L.nbr % 8 == 0 && L.nbc % 8 == 0 &&
+        (int64_t)(L.nbr / 8) * (L.nbc / 8) >= 2048)
+        q.tile_h = 8;
     return q;
 }
 
@@ -1981,7 +2022,7 @@ int launch_batch(int precision, int tile_h, int tile_w, int frag, const BatchPro
 
 const char *fused_kernel_name(const tcs_problem *p) {
     const tcs_problem q = fused_problem(p);
-    const FusedEntry *k = select_fused(&q);
+    const FusedEntry *k = select_fused(&q, p->fuse > 1 ? p->fuse : 1);
     return k ? k->name : nullptr;
 }
 
@@ -2037,7 +2078,7 @@ static int make_args(const tcs_problem *p, const void *d_in, void *d_out, SweepA
 int launch_fused(const tcs_problem *p_in, const void *d_in, void *d_out, int nsweeps, hipStream_t stream) {
     const tcs_problem q = fused_problem(p_in);
     const tcs_problem *p = &q;
-    const FusedEntry *k = select_fused(p);
+    const FusedEntry *k = select_fused(p, nsweeps);
     if (!k)
         return set_error(TCS_ERR_UNSUPPORTED, "no fused kernel for method %d precision %d tile %dx%d", p->method,
                          kernel_prec(p), p->tile_h, p->tile_w);
@@ -2047,7 +2088,7 @@ int launch_fused(const tcs_problem *p_in, const void *d_in, void *d_out, int nsw
     SweepArgs a;
     int st = make_args(p, d_in, d_out, a);
     if (st) return st;
-    hipLaunchKernelGGL(k->fn, dim3((unsigned)a.nvirt), dim3(kWG), 0, stream, a, nsweeps);
+    hipLaunchKernelGGL(k->fn, dim3((unsigned)a.nvirt), dim3((unsigned)k->nt), 0, stream, a, nsweeps);
     TCS_HIP_CHECK(hipGetLastError());
     return TCS_OK;
 }

@@ -498,25 +498,27 @@ def mfma_edge(mode, src, rows, cols, ld, L: Layout, TH, TW, coeff5, rs, vec, inp
     return out, written
 
 
-def _block_rw(BH, BW):
-    """BlockRows<BH, BW>::RW: bricks per row of a wave's block."""
-    return BW if BH % 4 == 0 else BW // 2
+def _block_rw(BH, BW, nw=4):
+    """BlockRows<BH, BW, NW>::RW: bricks per row of a wave's block."""
+    return BW if BH % nw == 0 else BW // 2
 
 
-def _block_brick(BH, BW, base, wave, q):
-    """block_brick of tcs_sweep.hip: with BH a multiple of 4 wave w takes BH/4
-    whole rows from row w * BH/4; else the 4 waves split a BH x BW brick region
-    2 x 2, wave w taking the (BH/2) x (BW/2) block at (w // 2, w % 2)."""
-    assert BH % 2 == 0 and BW % 2 == 0
-    if BH % 4 == 0:
-        return base + wave * (BH // 4) + q // BW, base + q % BW
-    hb, wb = BH // 2, BW // 2
+def _block_brick(BH, BW, base, wave, q, nw=4):
+    """block_brick of tcs_sweep.hip: with BH a multiple of NW wave w takes BH/NW
+    whole rows from row w * BH/NW; else the NW waves split a BH x BW brick
+    region into NW/2 row bands by 2 column halves, wave w taking the
+    (2BH/NW) x (BW/2) block at (w // 2, w % 2)."""
+    if BH % nw == 0:
+        return base + wave * (BH // nw) + q // BW, base + q % BW
+    assert BH % (nw // 2) == 0 and BW % 2 == 0
+    hb, wb = 2 * BH // nw, BW // 2
     return base + (wave >> 1) * hb + q // wb, base + (wave & 1) * wb + q % wb
 
 
-def mfma_fused(inp, L: Layout, TH, TW, coeff5, rs, nsweeps, round_coeff=f16):
-    """tcs_mfma_fused: nsweeps sweeps per tile in an LDS image of the tile plus a
-    one-brick ring and the 2-point frame, rewritten in place between sweeps."""
+def mfma_fused(inp, L: Layout, TH, TW, coeff5, rs, nsweeps, round_coeff=f16, nw=4):
+    """tcs_mfma_fused (nw = 4 waves per workgroup) and tcs_mfma_fused8 (nw = 8):
+    nsweeps sweeps per tile in an LDS image of the tile plus a one-brick ring
+    and the 2-point frame, rewritten in place between sweeps."""
     assert L.nbr % TH == 0 and L.nbc % TW == 0 and 1 <= nsweeps and rs * (nsweeps - 1) <= 16
     G = TileGeom(TH + 2, TW + 2, strip=True)
     afrag, ZLO, ZHI = _afrag(coeff5, rs, round_coeff)
@@ -529,27 +531,28 @@ def mfma_fused(inp, L: Layout, TH, TW, coeff5, rs, nsweeps, round_coeff=f16):
         lds = stage_tile(inp, L, G, er0, ec0, check=True)
         assert not np.isnan(lds[G.image()]).any(), "fused image has unwritten LDS words"
         edge = not (er0 >= rs and er0 + 16 * BH <= L.rows - rs and ec0 >= rs and ec0 + 16 * BW <= L.cols - rs)
-        HB = BW // 2
-        QB, HP = (TH // 2) * HB, (HB + 1) // 2
+        QB = TH * BW // nw
         mp = m + 8 + np.where(m >= 8, 16 * TH, 0)  # paired ring unit lane rows from brick row 0
         for sw in range(nsweeps - 1):
             paired = rs * (nsweeps - 1 - sw) <= 8
-            qt = QB + (HP if paired else HB)
+            wpr = nw if paired else nw // 2  # waves per ring row
+            rc, rx = BW // wpr, BW % wpr
+            qt = QB + rc + (rx > 0)
             res, rows_done = [], {}
-            for wave in range(4):
+            for wave in range(nw):
                 prev = None
+                wi = wave % wpr
+                ring_br = 0 if paired or wave < wpr else TH + 1
+                ring_bc = wi * rc + min(wi, rx)
                 for q in range(qt):
                     if q < QB:
-                        br, bc = _block_brick(TH, BW, 0, wave, q)
+                        br, bc = _block_brick(TH, BW, 0, wave, q, nw)
                         br, row, pair = br + 1, m, False
-                    elif not paired:
-                        br = TH + 1 if wave >> 1 else 0
-                        bc, row, pair = (wave & 1) * HB + (q - QB), m, False
                     else:
                         j = q - QB
-                        if (wave >> 1) and j >= HB - HP:
+                        if j >= rc and wi >= rx:
                             continue
-                        br, bc, row, pair = 0, (wave & 1) * HB + (wave >> 1) * HP + j, mp, True
+                        br, bc, row, pair = ring_br, ring_bc + j, (mp if paired else m), paired
                     # every output row of every computed brick, once
                     for rr in np.unique(16 * br + row):
                         key = (int(rr), bc)
@@ -558,7 +561,7 @@ def mfma_fused(inp, L: Layout, TH, TW, coeff5, rs, nsweeps, round_coeff=f16):
                     frags = []
                     o = _mfma_brick(lds, G, afrag, ZLO, ZHI, br, bc, edge, er0 + 16 * br + row,
                                     ec0 + 16 * bc + 4 * g, L, rs, row=row, frags_out=frags)
-                    if (q % _block_rw(TH, BW) != 0) if q < QB else (q > QB):  # fused_sweep's reuse(q)
+                    if (q % _block_rw(TH, BW, nw) != 0) if q < QB else (q > QB):  # fused_sweep's reuse(q)
                         _check_reuse(frags, prev)
                     prev = frags
                     res.append((br, bc, row, o))
@@ -572,14 +575,14 @@ def mfma_fused(inp, L: Layout, TH, TW, coeff5, rs, nsweeps, round_coeff=f16):
                 _check(d, G.PLANE - 3, 4, "fused write-back (8 B)")
                 for v in range(4):
                     lds[d + v] = o[:, v]
-        for wave in range(4):
+        for wave in range(nw):
             prev = None
-            for q in range(TH * TW // 4):
-                br, bc = _block_brick(TH, TW, 1, wave, q)
+            for q in range(TH * TW // nw):
+                br, bc = _block_brick(TH, TW, 1, wave, q, nw)
                 gr, gcb = er0 + 16 * br + m, ec0 + 16 * bc + 4 * g
                 frags = []
                 o = _mfma_brick(lds, G, afrag, ZLO, ZHI, br, bc, edge, gr, gcb, L, rs, frags_out=frags)
-                if q % _block_rw(TH, TW) != 0:  # fused_tile's reuse(q)
+                if q % _block_rw(TH, TW, nw) != 0:  # fused_tile's reuse(q)
                     _check_reuse(frags, prev)
                 prev = frags
                 _store(out, written, L, gr, gcb, o)

@@ -71,14 +71,25 @@ def test_layout_and_problem_setup():
     assert p.buffer_bytes == 1026 * 1026 * 256 * 2
     assert p.tile == (4, 8) and p.mfma_per_brick == 4
     assert p.kernel_name == "tcs_mfma_sweep<4, 8, 0, 4, 0, false>"
-    # fused FP16/BF16 launches of 8x8 problems run 4x8 tiles; SPLIT keeps 8x8
+    # fused FP16/BF16 launches: 8x8 tiles on 8-wave workgroups for 8x8 problems
+    # and for 4x8 problems with at least 2048 8x8 tiles; SPLIT 4x8 problems
+    # fuse 6 or more sweeps per launch on 8-wave workgroups
+    p.fuse = 8
+    assert p.fused_kernel_name == "tcs_mfma_fused8<8, 8, 0, 4, 0>"
+    assert T.Problem("box2d2r", 8192, 8192, B, fuse=4).fused_kernel_name == "tcs_mfma_fused8<8, 8, 0, 4, 0>"
+    small = T.Problem("box2d2r", 4096, 4096, B, tile=(4, 8), fuse=4)  # 1024 8x8 tiles
+    assert small.fused_kernel_name == "tcs_mfma_fused<4, 8, 0, 4, 0>"
     p.set_tile(8, 8)
     assert p.kernel_name == "tcs_mfma_sweep<8, 8, 0, 4, 0, false>"
-    assert p.fused_kernel_name == "tcs_mfma_fused<4, 8, 0, 4, 0>"
-    bf = T.Problem("box2d1r", 16384, 16384, np.ones((3, 3), np.float32), precision="bf16", tile=(8, 8))
-    assert bf.fused_kernel_name == "tcs_mfma_fused<4, 8, 1, 3, 1>"
-    sp = T.Problem("box2d2r", 16384, 16384, B, precision="split", tile=(8, 8))
+    assert p.fused_kernel_name == "tcs_mfma_fused8<8, 8, 0, 4, 0>"
+    bf = T.Problem("box2d1r", 300, 300, np.ones((3, 3), np.float32), precision="bf16", tile=(8, 8), fuse=3)
+    assert bf.fused_kernel_name == "tcs_mfma_fused8<8, 8, 1, 3, 1>"
+    sp = T.Problem("box2d2r", 16384, 16384, B, precision="split", tile=(8, 8), fuse=8)
     assert sp.fused_kernel_name == "tcs_mfma_fused<8, 8, 0, 4, 2>"
+    sp = T.Problem("box2d2r", 16384, 16384, B, precision="split", fuse=6)
+    assert sp.tile == (4, 8) and sp.fused_kernel_name == "tcs_mfma_fused8<4, 8, 0, 4, 2>"
+    sp.fuse = 5
+    assert sp.fused_kernel_name == "tcs_mfma_fused<4, 8, 0, 4, 2>"
     assert T.Problem("box2d2r", 64, 64, B, precision="split").mfma_per_brick == 12  # hi*hi, hi*lo, lo*hi
     assert list(p.s.coeff) == list(range(25))
     p = T.Problem("star2d1r", 100, 37, np.arange(9, dtype=np.float32).reshape(3, 3))

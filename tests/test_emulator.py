@@ -96,14 +96,17 @@ def test_persistent_schedule_covers_every_tile(ntile_shape):
 
 
 @pytest.mark.parametrize("stencil", STENCILS)
-@pytest.mark.parametrize("case", [(70, 150, 2, 4, 3), (40, 70, 4, 4, 2), (50, 140, 2, 8, 5), (33, 140, 4, 8, 4),
-                                  (40, 70, 4, 4, 9), (35, 140, 2, 8, 9), (140, 150, 8, 8, 6)],
-                         ids=lambda c: "%dx%d_t%dx%d_k%d" % c)
+@pytest.mark.parametrize("case", [(70, 150, 2, 4, 3, 4), (40, 70, 4, 4, 2, 4), (50, 140, 2, 8, 5, 4),
+                                  (33, 140, 4, 8, 4, 4), (40, 70, 4, 4, 9, 4), (35, 140, 2, 8, 9, 4),
+                                  (140, 150, 8, 8, 6, 4), (70, 140, 4, 8, 9, 8), (33, 150, 4, 8, 3, 8),
+                                  (140, 150, 8, 8, 5, 8)],
+                         ids=lambda c: "%dx%d_t%dx%d_k%d_w%d" % c)
 def test_emulated_fused_equals_repeated_sweeps(oracle, stencil, case):
     """k fused sweeps == k single sweeps, bitwise, and both equal the FP64 oracle
     rounded to FP16 after every sweep (each sweep of dyadic FP16 data with
-    taps 1/2, 1/4, 1/4 is exact in FP64)."""
-    rows, cols, TH, TW, k = case
+    taps 1/2, 1/4, 1/4 is exact in FP64); workgroups of 4 waves
+    (tcs_mfma_fused) and 8 (tcs_mfma_fused8)."""
+    rows, cols, TH, TW, k, nw = case
     r = RADIUS[stencil]
     A, _, _ = dyadic_problem(stencil, rows, cols, seed=rows + cols)
     n = 2 * r + 1
@@ -115,7 +118,7 @@ def test_emulated_fused_equals_repeated_sweeps(oracle, stencil, case):
     c5[2 - r:3 + r, 2 - r:3 + r] = B
     L = E.Layout(rows, cols, max(TH, TW))
     st = E.pack(A, L)
-    fused, written = E.mfma_fused(st, L, TH, TW, c5, r, k)
+    fused, written = E.mfma_fused(st, L, TH, TW, c5, r, k, nw=nw)
     data = np.zeros(L.plane, dtype=bool)
     for bi in range(L.nbr):
         base = ((bi + 1) * L.sbc + 1) * 256
@@ -137,7 +140,8 @@ def test_emulated_fused_equals_repeated_sweeps(oracle, stencil, case):
 def test_emulated_fused_every_ring_depth_radius1(oracle, stencil):
     """Radius 1 at the most sweeps per launch (17): the ring rows of sweep s are
     packed at every depth 16, 15, ..., 1, odd ones included."""
-    test_emulated_fused_equals_repeated_sweeps(oracle, stencil, (36, 70, 2, 4, 17))
+    test_emulated_fused_equals_repeated_sweeps(oracle, stencil, (36, 70, 2, 4, 17, 4))
+    test_emulated_fused_equals_repeated_sweeps(oracle, stencil, (36, 70, 4, 8, 17, 8))
 
 
 CASES32 = [(70, 150, 4, 4), (70, 150, 4, 8), (33, 17, 4, 4), (130, 140, 8, 8), (5, 3, 2, 8), (64, 128, 2, 8)]

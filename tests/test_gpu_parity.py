@@ -516,6 +516,10 @@ def test_fused_bitwise_equals_unfused(T, torch, stencil, precision):
                 ref[sweeps] = run_buffers(T, torch, pu, x, sweeps, e)
                 ref[sweeps] = pu.unpack(ref[sweeps], scale_exp=e)
             pf = T.Problem(stencil, 300, 517, B, precision=precision, tile=tile, fuse=fuse)
+            # the 8-wave kernels: FP16/BF16 8x8 tiles; SPLIT 4x8 tiles at 6+ sweeps per launch
+            # (these runs launch ceil(sweeps / fuse) rounded to odd: fuse 9 and 17 give 6+)
+            if tile == (8, 8) and precision != "split" or tile == (4, 8) and precision == "split" and fuse >= 6:
+                assert pf.fused_kernel_name.startswith("tcs_mfma_fused8<"), pf.fused_kernel_name
             got = pf.unpack(run_buffers(T, torch, pf, x, sweeps, e), scale_exp=e)
             assert torch.equal(got, ref[sweeps]), f"fuse {fuse}"
 
