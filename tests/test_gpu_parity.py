@@ -455,6 +455,30 @@ def test_full_size_box2d2r_16384(T, torch, method):
     assert res["max_abs"] <= (2.0 ** -11) * 2.0 * 1.001 + 1e-6
 
 
+@pytest.mark.parametrize("precision", ["fp16", "split"])
+def test_full_size_fused_bitwise_16384(T, torch, precision):
+    """bench.py's fused form at its size: box2d2r 16384^2, 8 sweeps per launch
+    on the 8-wave kernels (FP16: 8x8 tiles for the default 4x8 problem; SPLIT:
+    4x8 tiles), 16 sweeps bitwise equal to 16 single-sweep launches."""
+    say("case full size fused", precision)
+    N = 16384
+    B = T.gen_coeffs("box2d2r", 1)
+    x = T.gen_sinusoid(N, N, seed=1, noise=1e-3).float()
+    e = T.split_scale_exp(1.0) if precision == "split" else 0
+    pu = T.Problem("box2d2r", N, N, B, precision=precision)
+    pf = T.Problem("box2d2r", N, N, B, precision=precision, fuse=8)
+    want = "tcs_mfma_fused8<8, 8, 0, 4, 0>" if precision == "fp16" else "tcs_mfma_fused8<4, 8, 0, 4, 2>"
+    assert pu.tile == (4, 8) and pf.fused_kernel_name == want
+    outs = []
+    for p in (pu, pf):
+        b0, b1 = p.alloc(), p.alloc()
+        p.clear(b1)
+        p.pack(x, b0, e)
+        outs.append(p.unpack(p.run(b0, b1, 16), scale_exp=e))
+        del b0, b1
+    assert torch.equal(outs[0], outs[1])
+
+
 @pytest.mark.parametrize("stencil", ["box2d2r", "star2d2r"])
 def test_full_size_split_16384(T, torch, stencil):
     """The split hi+lo form at the 16384^2 size, with per-grid max-abs scaling:
