@@ -7,8 +7,9 @@ FLAG_DICT script/param.py:1-6), run each under `ncu` and scrape "Duration"
 (script/process_data.py:46-57) -- with a single library and runtime parameters:
 
   python -m tcstencil_amd.harness run --stencil box2d2r --size 16384 --method mfma --sweeps 20
-      one workload: pack, warm up, time `sweeps` back-to-back sweeps with HIP
-      events; --copy-mb adds a streaming copy of known size (the bandwidth
+      one workload: pack, warm up (runs of --warmup sweeps, rounded up to whole
+      fused launches, until 100 ms have passed: tcstencil_amd/timing.py), time
+      `sweeps` back-to-back sweeps with HIP events; --copy-mb adds a streaming copy of known size (the bandwidth
       ceiling, and the calibration kernel for FETCH_SIZE / WRITE_SIZE).
       Prints one JSON line.
 
@@ -167,7 +168,7 @@ def main(argv=None):
     r.add_argument("--precision", default="fp16")
     r.add_argument("--tile", type=int, nargs=2, default=None)
     r.add_argument("--sweeps", type=int, default=20)
-    r.add_argument("--warmup", type=int, default=5)
+    r.add_argument("--warmup", type=int, default=5, help="sweeps per warm-up run (repeated for 100 ms)")
     r.add_argument("--copy-mb", type=float, default=0)
     r.add_argument("--fuse", type=int, default=1)
     r.add_argument("--wg-per-cu", type=int, default=0)
