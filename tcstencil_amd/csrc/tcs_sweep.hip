@@ -44,9 +44,6 @@ __device__ __forceinline__ void store8(uint16_t *p, u32x2 w) {
 __device__ __forceinline__ uint16_t f16_bits(float v) {
     return __builtin_bit_cast(uint16_t, static_cast<_Float16>(v));
 }
-__device__ __forceinline__ uint16_t bf16_bits(float v) {
-    return __builtin_bit_cast(uint16_t, static_cast<__bf16>(v));
-}
 __device__ __forceinline__ uint32_t pack2(uint16_t lo, uint16_t hi) {
     return (uint32_t)lo | ((uint32_t)hi << 16);
 }
@@ -614,9 +611,9 @@ __device__ __forceinline__ void brick_out(const SweepArgs &a, const uint16_t *ld
     // Edge tiles: the input at the 4 output points (columns 4g+4..4g+7 of the
     // image row, 8 aligned bytes of one strip row, read whatever the points
     // are) replaces the results outside the interior: no LDS read waits inside
-    // a branch.  SPLIT selects whole words under a mask of the kept halves,
-    // FP16/BF16 16-bit values (each measured faster for its kernels: 8 fused
-    // sweeps, box2d2r 16384^2, SPLIT 268 -> 252 us, FP16 91.8 -> 94.3 us).
+    // a branch.  SPLIT selects whole words under a mask of the kept halves
+    // (8 fused sweeps, box2d2r 16384^2: 252 us per sweep against 268 with
+    // 16-bit values), FP16/BF16 16-bit values (91.8 us against 94.3 with words).
     static_assert(G::FX == 4, "a lane's 4 points are 8 aligned bytes of the STRIP image");
     const int ck = G::at(16 * br, 16 * bc) + G::at(m + 2, 4 * g + G::FX);
     if constexpr (NPL == 2) {
