@@ -128,10 +128,17 @@ def cpu_baseline_threads(np, threads):
                       f"(oracle/_build/liboracle.so), host {os.cpu_count()} logical CPUs, {cpu_model()}"}
 
 
+def kernel_tile(kernel, tile):
+    """The tile the kernel runs: its first two template arguments (a fused launch
+    may run another tile than the problem's), else the problem's tile."""
+    m = re.match(r"[a-z0-9_]+<(\d+), (\d+),", kernel or "")
+    return [int(m.group(1)), int(m.group(2))] if m else list(tile)
+
+
 def form_record(r, what):
     """One measured form of the sweep for the JSON line."""
     return {"value": round(r["gstencil"], 2), "unit": "GStencil/s", "ms_per_step": round(r["ms"], 5),
-            "kernel": r["kernel"], "tile_bricks": list(r["tile"]), "what": what,
+            "kernel": r["kernel"], "tile_bricks": kernel_tile(r["kernel"], r["tile"]), "what": what,
             "rel_l2_1sweep": r["acc"][1]["rel_l2"], "max_abs_1sweep": r["acc"][1]["max_abs"],
             "rel_l2_10sweeps": r["acc"][10]["rel_l2"], "max_abs_10sweeps": r["acc"][10]["max_abs"]}
 
