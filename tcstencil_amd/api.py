@@ -154,6 +154,9 @@ class Problem:
 
     @property
     def fused_kernel_name(self) -> str:
+        """The kernel of a launch of `fuse` fused sweeps ("" if none): its tile
+        and workgroup size may differ from the problem's (large FP16/BF16 4x8
+        problems fuse on 8x8 tiles of 8 waves)."""
         n = lib().tcs_fused_kernel_name(self.ref)
         return n.decode() if n else ""
 
