@@ -548,6 +548,27 @@ def test_fused_bitwise_equals_unfused(T, torch, stencil, precision):
             assert torch.equal(got, ref[sweeps]), f"fuse {fuse}"
 
 
+@pytest.mark.parametrize("stencil", ["box2d2r", "star2d1r"])
+@pytest.mark.parametrize("precision", ["fp16", "bf16"])
+def test_fused_hi_lo_taps_bitwise_equals_unfused(T, torch, stencil, precision):
+    """Fused sweeps with hi+lo taps (kernel precision ids 4 and 5, two MFMAs per
+    fragment) give the same bits as the unfused hi+lo sweeps, on the tiles their
+    fused kernels run (4-wave workgroups; the 8x8 default of hi+lo problems)."""
+    r = RADIUS[stencil]
+    B = T.gen_coeffs(stencil, 23)
+    x = T.gen_uniform(300, 517, seed=23, dtype=torch.float32).cpu().numpy()
+    for tile in ((2, 4), (4, 8), (8, 8)):
+        for fuse in (2, 1 + 16 // r):
+            sweeps = 2 * fuse + 1
+            say("case fused hi+lo", stencil, precision, tile, fuse, sweeps)
+            pu = T.Problem(stencil, 300, 517, B, precision=precision, tile=tile, coef_lo=True)
+            pf = T.Problem(stencil, 300, 517, B, precision=precision, tile=tile, coef_lo=True, fuse=fuse)
+            assert pf.coef_lo and pf.fused_kernel_name.startswith("tcs_mfma_fused<"), pf.fused_kernel_name
+            want = pu.unpack(run_buffers(T, torch, pu, x, sweeps))
+            got = pf.unpack(run_buffers(T, torch, pf, x, sweeps))
+            assert torch.equal(got, want), (tile, fuse)
+
+
 def test_fused_graph_and_accuracy(T, torch, oracle):
     say("case fused graph")
     A, B = random_problem(T, torch, "box2d2r", 200, 333, seed=22)
