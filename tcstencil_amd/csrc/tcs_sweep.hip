@@ -1888,8 +1888,11 @@ struct PartTables {
         TCS_FUSED(4, 8, ZLO, ZHI, P, R)
 #define TCS_FUSED_TILES88(ZLO, ZHI, P, R) TCS_FUSED_TILES(ZLO, ZHI, P, R), TCS_FUSED(8, 8, ZLO, ZHI, P, R)
 #define TCS_FUSED_PREC88(P) TCS_FUSED_TILES88(0, 4, P, 2), TCS_FUSED_TILES88(1, 3, P, 1)
-// FP16/BF16 with rounded taps: 8x8 tiles on 8-wave workgroups (fused_problem
-// maps large 4x8 problems to them), smaller tiles on 4 waves.
+// FP16/BF16, rounded or hi+lo taps: 8x8 tiles on 8-wave workgroups
+// (fused_problem maps large 4x8 problems to them), smaller tiles on 4 waves.
+// With hi+lo taps, box2d2r 16384^2, 8 sweeps per launch, us per sweep: 8x8 on
+// 8 waves 128.9, 4x8 on 4 waves 135.7, 8x8 on 4 waves 140.1 (radius 1, 16
+// sweeps: 98.7, 102.3, 115.7).
 #define TCS_FUSED_PREC(P) \
     TCS_FUSED8(8, 8, 0, 4, P, 2, 1), TCS_FUSED8(8, 8, 1, 3, P, 1, 1), TCS_FUSED_TILES(0, 4, P, 2), \
         TCS_FUSED_TILES(1, 3, P, 1)
@@ -1953,9 +1956,9 @@ static const EdgeEntry kEdgeSplit[] = {TCS_EDGE_PREC(2)};
 extern const PartTables kPartSplit = {kSweepSplit, TCS_N(kSweepSplit), kFusedSplit, TCS_N(kFusedSplit),
                                       nullptr, 0, kEdgeSplit, TCS_N(kEdgeSplit)};
 #elif TCS_PART == 4
-TCS_PART_16(Fp16Lo, 4, TCS_FUSED_PREC88(4), TCS_BATCH16_PREC(4))
+TCS_PART_16(Fp16Lo, 4, TCS_FUSED_PREC(4), TCS_BATCH16_PREC(4))
 #elif TCS_PART == 5
-TCS_PART_16(Bf16Lo, 5, TCS_FUSED_PREC88(5), TCS_BATCH16_PREC(5))
+TCS_PART_16(Bf16Lo, 5, TCS_FUSED_PREC(5), TCS_BATCH16_PREC(5))
 #elif TCS_PART == 6
 // 32x32 fragments (FP16 and BF16, tiles whose 2x2-brick blocks split evenly over
 // the 4 waves, one tile per workgroup) and the direct-FMA kernel.
@@ -2012,8 +2015,8 @@ static const FusedEntry *select_fused(const tcs_problem *p, int nsweeps) {
     return nullptr;
 }
 
-// The problem a fused launch runs: the caller's, except that FP16/BF16 with
-// rounded taps and 16x16 fragments fuses 4x8 problems on 8x8 tiles (8-wave
+// The problem a fused launch runs: the caller's, except that FP16/BF16 (rounded
+// or hi+lo taps) with 16x16 fragments fuses 4x8 problems on 8x8 tiles (8-wave
 // workgroups, tcs_mfma_fused8) where the layout divides into 8x8 tiles and
 // makes at least 2048 of them (brick results depend on the tile in no way).
 // An 8x8 tile recomputes 12% fewer ring bricks than two 4x8 tiles over 8
@@ -2025,7 +2028,8 @@ static tcs_problem fused_problem(const tcs_problem *p) {
     tcs_problem q = *p;
     const int kp = kernel_prec(p);
     const tcs_layout &L = p->layout;
-    if (p->method == TCS_METHOD_MFMA && frag_of(p) == 16 && (kp == TCS_PREC_FP16 || kp == TCS_PREC_BF16) &&
+    if (p->method == TCS_METHOD_MFMA && frag_of(p) == 16 &&
+        (base_prec(kp) == TCS_PREC_FP16 || base_prec(kp) == TCS_PREC_BF16) &&
         p->tile_h == 4 && p->tile_w == 8 && L.nbr % 8 == 0 && L.nbc % 8 == 0 &&
         (int64_t)(L.nbr / 8) * (L.nbc / 8) >= 2048)
         q.tile_h = 8;

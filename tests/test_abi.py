@@ -388,7 +388,7 @@ def test_storage_buffers_too_small_are_rejected_before_any_launch():
 def test_hi_lo_coefficient_option():
     """coef_lo selects the hi+lo coefficient kernels (kernel precision ids 4 and
     5: two MFMAs per fragment) for MFMA FP16/BF16 with 16x16 fragments, prefers
-    8x8 tiles, and is ignored by SPLIT and the direct method; 32x32 fragments
+    8x8 tiles (fused on 8-wave workgroups), and is ignored by SPLIT and the direct method; 32x32 fragments
     have no such kernel (UNSUPPORTED at launch: no kernel name)."""
     import tcstencil_amd as T
     B = T.gen_coeffs("box2d2r", 1)
@@ -396,7 +396,11 @@ def test_hi_lo_coefficient_option():
         p = T.Problem("box2d2r", 16384, 16384, B, precision=prec, coef_lo=True)
         assert p.coef_lo and p.tile == (8, 8) and p.mfma_per_brick == 8
         assert p.kernel_name == f"tcs_mfma_sweep<8, 8, 0, 4, {kp}, false>"
-        assert p.fused_kernel_name == f"tcs_mfma_fused<8, 8, 0, 4, {kp}>"
+        assert p.fused_kernel_name == f"tcs_mfma_fused8<8, 8, 0, 4, {kp}>"
+        # 4x8 problems fuse on 8x8 tiles of 8-wave workgroups, as rounded taps do
+        p.set_tile(4, 8)
+        assert p.kernel_name == f"tcs_mfma_sweep<4, 8, 0, 4, {kp}, false>"
+        assert p.fused_kernel_name == f"tcs_mfma_fused8<8, 8, 0, 4, {kp}>"
         q = T.Problem("box2d1r", 16384, 16384, T.gen_coeffs("box2d1r", 1), precision=prec, coef_lo=True)
         assert q.mfma_per_brick == 6
     s = T.Problem("box2d2r", 1024, 1024, B, precision="split", coef_lo=True)

@@ -553,7 +553,8 @@ def test_fused_bitwise_equals_unfused(T, torch, stencil, precision):
 def test_fused_hi_lo_taps_bitwise_equals_unfused(T, torch, stencil, precision):
     """Fused sweeps with hi+lo taps (kernel precision ids 4 and 5, two MFMAs per
     fragment) give the same bits as the unfused hi+lo sweeps, on the tiles their
-    fused kernels run (4-wave workgroups; the 8x8 default of hi+lo problems)."""
+    fused kernels run (4-wave workgroups; 8-wave ones on 8x8 tiles, the default of
+    hi+lo problems)."""
     r = RADIUS[stencil]
     B = T.gen_coeffs(stencil, 23)
     x = T.gen_uniform(300, 517, seed=23, dtype=torch.float32).cpu().numpy()
@@ -563,7 +564,8 @@ def test_fused_hi_lo_taps_bitwise_equals_unfused(T, torch, stencil, precision):
             say("case fused hi+lo", stencil, precision, tile, fuse, sweeps)
             pu = T.Problem(stencil, 300, 517, B, precision=precision, tile=tile, coef_lo=True)
             pf = T.Problem(stencil, 300, 517, B, precision=precision, tile=tile, coef_lo=True, fuse=fuse)
-            assert pf.coef_lo and pf.fused_kernel_name.startswith("tcs_mfma_fused<"), pf.fused_kernel_name
+            kind = "tcs_mfma_fused8<" if tile == (8, 8) else "tcs_mfma_fused<"
+            assert pf.coef_lo and pf.fused_kernel_name.startswith(kind), pf.fused_kernel_name
             want = pu.unpack(run_buffers(T, torch, pu, x, sweeps))
             got = pf.unpack(run_buffers(T, torch, pf, x, sweeps))
             assert torch.equal(got, want), (tile, fuse)
