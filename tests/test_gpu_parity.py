@@ -455,19 +455,24 @@ def test_full_size_box2d2r_16384(T, torch, method):
     assert res["max_abs"] <= (2.0 ** -11) * 2.0 * 1.001 + 1e-6
 
 
-@pytest.mark.parametrize("precision", ["fp16", "split"])
+@pytest.mark.parametrize("precision", ["fp16", "split", "bf16lo"])
 def test_full_size_fused_bitwise_16384(T, torch, precision):
     """bench.py's fused form at its size: box2d2r 16384^2, 8 sweeps per launch
-    on the 8-wave kernels (FP16: 8x8 tiles for the default 4x8 problem; SPLIT:
-    4x8 tiles), 16 sweeps bitwise equal to 16 single-sweep launches."""
+    on the 8-wave kernels (FP16, and BF16 with hi+lo taps on a 4x8 problem: 8x8
+    tiles for the 4x8 problem; SPLIT: 4x8 tiles), 16 sweeps bitwise equal to 16
+    single-sweep launches."""
     say("case full size fused", precision)
     N = 16384
     B = T.gen_coeffs("box2d2r", 1)
     x = T.gen_sinusoid(N, N, seed=1, noise=1e-3).float()
     e = T.split_scale_exp(1.0) if precision == "split" else 0
-    pu = T.Problem("box2d2r", N, N, B, precision=precision)
-    pf = T.Problem("box2d2r", N, N, B, precision=precision, fuse=8)
-    want = "tcs_mfma_fused8<8, 8, 0, 4, 0>" if precision == "fp16" else "tcs_mfma_fused8<4, 8, 0, 4, 2>"
+    lo = precision.endswith("lo")
+    prec = precision[:-2] if lo else precision
+    tile = (4, 8) if lo else None  # hi+lo problems default to 8x8 tiles
+    pu = T.Problem("box2d2r", N, N, B, precision=prec, tile=tile, coef_lo=lo)
+    pf = T.Problem("box2d2r", N, N, B, precision=prec, tile=tile, coef_lo=lo, fuse=8)
+    want = {"fp16": "tcs_mfma_fused8<8, 8, 0, 4, 0>", "split": "tcs_mfma_fused8<4, 8, 0, 4, 2>",
+            "bf16lo": "tcs_mfma_fused8<8, 8, 0, 4, 5>"}[precision]
     assert pu.tile == (4, 8) and pf.fused_kernel_name == want
     outs = []
     for p in (pu, pf):
