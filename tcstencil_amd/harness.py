@@ -47,7 +47,7 @@ TILES = [(2, 4), (4, 4), (2, 8), (4, 8), (8, 8)]
 
 
 def run_workload(stencil, rows, cols, method="mfma", precision="fp16", tile=None, sweeps=20,
-                 warmup=5, copy_mb=0, seed=1, fuse=1, wg_per_cu=0, frag=16):
+                 warmup=5, copy_mb=0, seed=1, fuse=1, wg_per_cu=0, frag=16, coef_lo=False):
     import numpy as np
     import torch
 
@@ -55,7 +55,7 @@ def run_workload(stencil, rows, cols, method="mfma", precision="fp16", tile=None
 
     B = T.gen_coeffs(stencil, seed)
     p = T.Problem(stencil, rows, cols, B, method=method, precision=precision, tile=tile, fuse=fuse,
-                  wg_per_cu=wg_per_cu, frag=frag)
+                  wg_per_cu=wg_per_cu, frag=frag, coef_lo=coef_lo)
     x = T.gen_uniform(rows, cols, seed=seed, dtype=torch.float32)
     e = T.split_scale_exp(1.0) if precision == "split" else 0
     b0, b1 = p.alloc(), p.alloc()
@@ -72,7 +72,7 @@ def run_workload(stencil, rows, cols, method="mfma", precision="fp16", tile=None
     ms = ev[0].elapsed_time(ev[1]) / sweeps
     pts = p.interior_points
     out = {"stencil": stencil, "rows": rows, "cols": cols, "method": method, "precision": precision,
-           "tile": list(p.tile), "fuse": fuse, "frag": p.frag,
+           "tile": list(p.tile), "fuse": fuse, "frag": p.frag, "coef_lo": p.coef_lo,
            "kernel": p.fused_kernel_name if fuse > 1 else p.kernel_name, "sweeps": sweeps, "ms_per_sweep": ms,
            "gstencil_s": pts / (ms * 1e-3) / 1e9,
            "hbm_gbs": BYTES_PER_UPDATE[precision] / max(fuse, 1) * pts / (ms * 1e-3) / 1e9}
@@ -173,6 +173,7 @@ def main(argv=None):
     r.add_argument("--fuse", type=int, default=1)
     r.add_argument("--wg-per-cu", type=int, default=0)
     r.add_argument("--frag", type=int, default=16, choices=(16, 32))
+    r.add_argument("--coef-lo", action="store_true", help="carry every tap as a hi+lo pair (FP16/BF16 MFMA)")
     e = sub.add_parser("edges")
     e.add_argument("--stencil", default="box2d2r")
     e.add_argument("--size", type=int, default=16384)
@@ -197,7 +198,7 @@ def main(argv=None):
     if a.cmd == "run":
         res = run_workload(a.stencil, a.size, a.cols or a.size, a.method, a.precision,
                            tuple(a.tile) if a.tile else None, a.sweeps, a.warmup, a.copy_mb,
-                           fuse=a.fuse, wg_per_cu=a.wg_per_cu, frag=a.frag)
+                           fuse=a.fuse, wg_per_cu=a.wg_per_cu, frag=a.frag, coef_lo=a.coef_lo)
         print(json.dumps(res), flush=True)
         return 0
 
