@@ -136,6 +136,20 @@ def test_emulated_fused_equals_repeated_sweeps(oracle, stencil, case):
     assert np.array_equal(E.unpack(fused, L), ref)
 
 
+@pytest.mark.parametrize("stencil", ["star2d1r", "box2d2r"])
+def test_emulated_fused_tiny_grids(oracle, stencil):
+    """Grids smaller than one tile (1x1 up to 17x1, one padded tile of bricks):
+    every fused form's image loads and write-backs stay inside the storage
+    plane (the emulator's bounds checks) and k fused sweeps equal k single
+    sweeps, on 4-wave and 8-wave workgroups, at 2 sweeps per launch and (1x1,
+    17x1) at the most."""
+    kmax = 1 + 16 // RADIUS[stencil]
+    for rows, cols in ((1, 1), (2, 3), (17, 1), (5, 33), (16, 16)):
+        for TH, TW, nw in ((2, 4, 4), (4, 8, 4), (4, 8, 8), (8, 8, 4), (8, 8, 8)):
+            for k in (2, kmax) if cols == 1 else (2,):
+                test_emulated_fused_equals_repeated_sweeps(oracle, stencil, (rows, cols, TH, TW, k, nw))
+
+
 @pytest.mark.parametrize("stencil", ["star2d1r", "box2d1r"])
 def test_emulated_fused_every_ring_depth_radius1(oracle, stencil):
     """Radius 1 at the most sweeps per launch (17): the ring rows of sweep s are

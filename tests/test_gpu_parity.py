@@ -576,6 +576,32 @@ def test_fused_hi_lo_taps_bitwise_equals_unfused(T, torch, stencil, precision):
             assert torch.equal(got, want), (tile, fuse)
 
 
+@pytest.mark.parametrize("stencil", ["star2d1r", "box2d2r"])
+def test_fused_tiny_grids_bitwise_equals_unfused(T, torch, stencil):
+    """Grids of one padded tile or less (1x1 to 17x1) through every fused form:
+    4-wave tiles, 8x8 tiles on 8 waves (FP16, and with hi+lo taps), SPLIT 4x8
+    on 8 waves at 6+ sweeps per launch; bitwise the single sweeps."""
+    r = RADIUS[stencil]
+    kmax = 1 + 16 // r
+    B = T.gen_coeffs(stencil, 29)
+    for rows, cols in ((1, 1), (2, 3), (17, 1), (5, 33), (16, 16)):
+        x = T.gen_uniform(rows, cols, seed=rows * 37 + cols, dtype=torch.float32).cpu().numpy()
+        for precision, lo in (("fp16", False), ("split", False), ("fp16", True)):
+            e = T.split_scale_exp(1.0) if precision == "split" else 0
+            for tile in ((2, 4), (4, 8), (8, 8)):
+                for fuse in (2, kmax):
+                    sweeps = 2 * fuse + 1
+                    say("case fused tiny", stencil, rows, cols, precision, lo, tile, fuse)
+                    kw = dict(precision=precision, tile=tile, coef_lo=lo)
+                    pu = T.Problem(stencil, rows, cols, B, **kw)
+                    pf = T.Problem(stencil, rows, cols, B, fuse=fuse, **kw)
+                    if tile == (8, 8) and precision != "split" or tile == (4, 8) and precision == "split" and fuse >= 6:
+                        assert pf.fused_kernel_name.startswith("tcs_mfma_fused8<"), pf.fused_kernel_name
+                    want = pu.unpack(run_buffers(T, torch, pu, x, sweeps, e), scale_exp=e)
+                    got = pf.unpack(run_buffers(T, torch, pf, x, sweeps, e), scale_exp=e)
+                    assert torch.equal(got, want), (rows, cols, precision, lo, tile, fuse)
+
+
 def test_fused_graph_and_accuracy(T, torch, oracle):
     say("case fused graph")
     A, B = random_problem(T, torch, "box2d2r", 200, 333, seed=22)
