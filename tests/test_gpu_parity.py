@@ -602,6 +602,35 @@ def test_fused_tiny_grids_bitwise_equals_unfused(T, torch, stencil):
                     assert torch.equal(got, want), (rows, cols, precision, lo, tile, fuse)
 
 
+@pytest.mark.parametrize("precision", ["fp16", "split"])
+def test_fused_nan_propagation(T, torch, oracle, precision):
+    """Fused launches with NaN inputs: every point that k FP64 oracle sweeps
+    poison is NaN, and wherever the fused and the single-sweep results are both
+    finite they are bitwise equal.  (The two forms poison different supersets of
+    the oracle's set: a NaN enters every output of the brick rows whose window
+    holds it, so it spreads up to a brick per sweep across, and a fused launch
+    recomputes only its tile's one-brick ring.)"""
+    rows, cols = 200, 400
+    B = T.gen_coeffs("box2d2r", 31)
+    A = T.gen_uniform(rows, cols, seed=31, dtype=torch.float32).cpu().numpy().astype(np.float64)
+    A[70, 131] = np.nan
+    A[3, 300] = np.nan  # next to the zero ring
+    e = T.split_scale_exp(1.0) if precision == "split" else 0
+    for tile, fuse in (((4, 8), 4), ((8, 8), 8), ((2, 4), 9)):
+        say("case fused nan", precision, tile, fuse)
+        ref = A
+        for _ in range(fuse):
+            ref = oracle.sweep_f64("box2d2r", ref, B)
+        uf = gpu_sweeps(T, torch, "box2d2r", A, B, precision=precision, tile=tile, sweeps=fuse, scale_exp=e)
+        fu = gpu_sweeps(T, torch, "box2d2r", A, B, precision=precision, tile=tile, sweeps=fuse, scale_exp=e,
+                        fuse=fuse)
+        poisoned = np.isnan(ref)
+        assert poisoned.any() and np.all(np.isnan(uf[poisoned])) and np.all(np.isnan(fu[poisoned]))
+        both = np.isfinite(uf) & np.isfinite(fu)
+        assert both.sum() > rows * cols // 4, both.sum()
+        assert np.array_equal(uf[both], fu[both]), (tile, fuse)
+
+
 def test_fused_graph_and_accuracy(T, torch, oracle):
     say("case fused graph")
     A, B = random_problem(T, torch, "box2d2r", 200, 333, seed=22)
