@@ -885,13 +885,16 @@ def three_steps(T, torch, p, src, sweeps, scale_exp=0):
     return p.unpack(p.run(b0, b1, sweeps), scale_exp=scale_exp, ring=src)
 
 
-@pytest.mark.parametrize("precision", ["fp16", "bf16", "split"])
+@pytest.mark.parametrize("precision", ["fp16", "bf16", "split", "fp16lo", "bf16lo"])
 def test_run_f32_fused_edges_bitwise(T, torch, precision):
     """tcs_run_f32 with the layout transforms fused into the first and last sweep
     (tcs_mfma_edge) equals pack -> run -> unpack bitwise: random inputs, every
     stencil and edge tile, 1-3 sweeps (and 4 with fused middle sweeps), widths
     that do and do not allow 16-byte row accesses, and a source 4 bytes off a
-    16-byte boundary.  SPLIT with a per-grid scale exponent."""
+    16-byte boundary.  SPLIT with a per-grid scale exponent; FP16 and BF16 also
+    with hi+lo taps (kernel precisions 4 and 5)."""
+    lo = precision.endswith("lo")
+    precision = precision[:-2] if lo else precision
     e = T.split_scale_exp(1.0) if precision == "split" else 0
     tiles = ((2, 4), (4, 4), (2, 8), (4, 8), (8, 8))
     for stencil in STENCILS:
@@ -902,7 +905,8 @@ def test_run_f32_fused_edges_bitwise(T, torch, precision):
                 src = base[off:off + rows * cols].view(rows, cols)
                 src.copy_(T.gen_uniform(rows, cols, seed=rows + cols, dtype=torch.float32))
                 for tile in tiles:
-                    p = T.Problem(stencil, rows, cols, B, precision=precision, tile=tile)
+                    p = T.Problem(stencil, rows, cols, B, precision=precision, tile=tile, coef_lo=lo)
+                    assert p.coef_lo == lo and p.edge_kernel_name(3), p.edge_kernel_name(3)
                     b0, b1 = p.alloc(), p.alloc()
                     p.clear(b0)
                     p.clear(b1)
