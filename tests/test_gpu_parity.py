@@ -642,6 +642,31 @@ def test_fused_graph_and_accuracy(T, torch, oracle):
     check_against_models(oracle, "box2d2r", A, B.astype(np.float32), out, "fp16", 10, what="fused 4")
 
 
+@pytest.mark.parametrize("precision", ["fp16", "split"])
+def test_fused8_graph_replays_match_eager(T, torch, precision):
+    """The 8-wave fused kernels captured in a hipGraph (FP16: 8x8 tiles, two
+    launches of 8 sweeps; SPLIT: 4x8 tiles, two launches of 6): three replays,
+    each after eager work on the same buffers, each bitwise the eager run."""
+    tile, fuse, sweeps = ((8, 8), 8, 16) if precision == "fp16" else ((4, 8), 6, 12)
+    say("case fused8 graph", precision, tile, fuse, sweeps)
+    B = T.gen_coeffs("box2d2r", 41)
+    x = T.gen_uniform(300, 517, seed=41, dtype=torch.float32)
+    e = T.split_scale_exp(1.0) if precision == "split" else 0
+    p = T.Problem("box2d2r", 300, 517, B, precision=precision, tile=tile, fuse=fuse)
+    assert p.fused_kernel_name.startswith("tcs_mfma_fused8<"), p.fused_kernel_name
+    b0, b1 = p.alloc(), p.alloc()
+    p.clear(b1)
+    p.pack(x, b0, e)
+    eager = p.unpack(p.run(b0, b1, sweeps), scale_exp=e).clone()
+    g = p.graph(b0, b1, sweeps)
+    for rep in range(3):
+        p.pack(x, b0, e)  # eager work between replays
+        g.launch()
+        torch.cuda.synchronize()
+        assert torch.equal(p.unpack(b0, scale_exp=e), eager), f"replay {rep}"
+    g.destroy()
+
+
 def test_fused_limits(T, torch):
     B = T.gen_coeffs("box2d2r", 1)
     p = T.Problem("box2d2r", 64, 64, B, fuse=10)  # radius 2 allows at most 9
