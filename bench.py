@@ -198,21 +198,23 @@ def main():
 
     results = {}
     forms = (("mfma", "mfma", {}), ("mfma_hi_lo", "mfma", {"coef_lo": True}), ("direct", "direct", {}),
-             ("direct_dot2", "direct_dot2", {}), ("mfma_fused8", "mfma", {"fuse": 8}))
+             ("direct_dot2", "direct_dot2", {}), ("mfma_fused8", "mfma", {"fuse": 8}),
+             ("mfma_hi_lo_fused8", "mfma", {"coef_lo": True, "fuse": 8}))
     for name, method, kw in forms:
         p = T.Problem(STENCIL, N, N, B, method=method, precision="fp16", **kw)
+        fused = name.endswith("fused8")
         # fused sweeps are bitwise the unfused ones (tests/test_gpu_parity.py)
-        acc = results["mfma"]["acc"] if name == "mfma_fused8" else accuracy(torch, T, p, x64, refs)
+        acc = results[name[:-len("_fused8")]]["acc"] if fused else accuracy(torch, T, p, x64, refs)
         b0, b1 = p.alloc(), p.alloc()
         p.clear(b1)
         p.pack(x64, b0)
-        steps = args.steps - args.steps % 8 if name == "mfma_fused8" else args.steps
+        steps = args.steps - args.steps % 8 if fused else args.steps
         ms = time_sweeps(torch, p, b0, b1, steps, args.warmup)
         gst = p.interior_points / (ms * 1e-3) / 1e9
-        kernel = p.fused_kernel_name if name == "mfma_fused8" else p.kernel_name
+        kernel = p.fused_kernel_name if fused else p.kernel_name
         results[name] = {"ms": ms, "gstencil": gst, "kernel": kernel, "tile": p.tile,
                          "mfma_per_brick": p.mfma_per_brick, "acc": acc}
-        log(f"bench: {name:11s} {kernel}: {ms * 1e3:.1f} us/sweep, {gst:.1f} GStencil/s, "
+        log(f"bench: {name:17s} {kernel}: {ms * 1e3:.1f} us/sweep, {gst:.1f} GStencil/s, "
             f"rel-L2 1 sweep {acc[1]['rel_l2']:.3e}, 10 sweeps {acc[10]['rel_l2']:.3e}")
         del b0, b1
     del x64, refs
@@ -256,6 +258,8 @@ def main():
         "mfma_hi_lo": form_record(results["mfma_hi_lo"], "MFMA with hi+lo FP16 taps (FP32-tap accuracy)"),
         "mfma_fused8": form_record(results["mfma_fused8"], "MFMA, 8 sweeps fused per launch in LDS "
                                                             "(bitwise the unfused sweeps; 0.5 B/update of HBM)"),
+        "mfma_hi_lo_fused8": form_record(results["mfma_hi_lo_fused8"], "MFMA with hi+lo FP16 taps, 8 sweeps fused "
+                                                                        "per launch (bitwise the unfused hi+lo sweeps)"),
         "mfma_vs_direct": round(m["gstencil"] / results["direct"]["gstencil"], 3),
         # the same FP16 taps, and the fastest direct form measured
         "mfma_vs_direct_dot2": round(m["gstencil"] / results["direct_dot2"]["gstencil"], 3),
