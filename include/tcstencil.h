@@ -151,7 +151,7 @@ int tcs_problem_init(tcs_problem *p, int stencil, int method, int precision, int
 int tcs_radius(int stencil);
 /* Choose the workgroup tile (bricks, rows x cols): 2x4, 4x4, 2x8, 4x8 or 8x8.
  * With 16x16 fragments single-sweep and fused kernels exist for all five
- * (FP16/BF16 with rounded taps fuse large 4x8 problems -- at least 2048 8x8
+ * (FP16/BF16, with rounded or hi+lo taps, fuse large 4x8 problems -- at least 2048 8x8
  * tiles -- on 8x8 tiles of 8-wave workgroups, which measure faster at every
  * fused depth; SPLIT fuses 6 or more sweeps per launch of 4x8 problems on
  * 8-wave workgroups); a missing combination (32x32 fragments: see tcs_problem_set_frag) fails at
