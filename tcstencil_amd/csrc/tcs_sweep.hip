@@ -1136,7 +1136,9 @@ void tcs_mfma_fused(SweepArgs a, int nsweeps) {
 }
 
 // One tile per workgroup of 8 waves: for images whose LDS size allows few
-// workgroups per CU (SPLIT 4x8: 2 workgroups, 4 waves per SIMD instead of 2).
+// workgroups per CU (SPLIT 4x8: 2 workgroups, 4 waves per SIMD instead of 2;
+// FP16/BF16 8x8, rounded or hi+lo taps: 2 workgroups, 4 waves per SIMD where
+// 4-wave workgroups give 2).
 template <int TH, int TW, int ZLO, int ZHI, int PREC>
 __global__ __launch_bounds__(2 * kWG)
 __attribute__((amdgpu_waves_per_eu(kFusedMinWaves<TH, TW, PREC, 8>(), 8)))
